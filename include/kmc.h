@@ -1,0 +1,146 @@
+/* kmc.h -- C ABI of the MI355X k-mer counting engine (libkmc.so).
+ *
+ * Drop-in boundary for the counting path of KmerTreeCounter (tree_v4).  The
+ * reference has no FFI: its path is a set of template classes wired together in
+ * process_main<N> (src/main.cpp:111-312).  These entry points replace, one to
+ * one, the seams the reference host code calls there:
+ *
+ *   kc_create        <- construction of ParserThreadPool / ClassifierThreadPool /
+ *                       SchedulerThreadPool / KmerTree (src/main.cpp:195-215)
+ *   kc_sample_block  <- FastqPreParser::parse_and_push + FastqPrefixCounter::
+ *                       count_prefixes (Stage 0, src/main.cpp:140-171,
+ *                       src/FastqPrefixCounter.h:31-82)
+ *   kc_plan          <- prefix clamp + lpt + calculate_bloom_filter_capacity
+ *                       (src/main.cpp:173-189, :38-108)
+ *   kc_push_block    <- the SPMCRingMemoryPool consumer side: one sequence-only
+ *                       block {data, length} as FastqReader publishes it
+ *                       (src/FastqReader.h:367-388, src/RingMemoryPool.h:219-237,
+ *                       content_type src/definition.h:59-63)
+ *   kc_finish        <- parser/classifier/scheduler join + final_drain_parallel
+ *                       (src/main.cpp:242-264)
+ *   kc_emit_firsts   <- the ExportWriter<N> input: first-occurrence keys that go
+ *                       to low.bin (src/ExportWriter.h:400-497)
+ *   kc_emit_root     <- FinalDrainWriter::open/write/close per root id
+ *                       (src/FinalDrainWriter.h:31-91): ExportRecord<N> records
+ *                       (src/definition.h:159-163) whose key has key.data[0]>>56 == root_id
+ *   kc_stats         <- "Total read k-mer count" and the phase timers printed by
+ *                       process_main (src/main.cpp:266-309)
+ *   kc_destroy       <- teardown at the end of process_main
+ *
+ * Conventions.  Every entry point returns 0 on success and a negative kc_status
+ * on failure and never throws.  The engine owns all device memory.  Blocks passed
+ * to kc_push_block are copied before the call returns, so the caller may reuse
+ * them at once.  Emit callbacks lend buffers that are valid only during the
+ * call.  One host thread drives a handle; handles are independent.
+ *
+ * Key layout: N = 1 (k <= 32) or 2 (33 <= k <= 63) little-endian u64 words per
+ * key, word 0 most significant, 2 bits per base MSB-first, left aligned
+ * (src/kmer.h:12-21).  Canonical key = min(pack(reverse(w)), pack(complement(w)))
+ * for a window w, as GetKmer computes it (src/GetKmer.h:58-129).
+ */
+#ifndef KMC_H
+#define KMC_H
+
+#include <stdint.h>
+
+#ifndef KC_API
+#define KC_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kc_status {
+    KC_OK = 0,
+    KC_ERR_ARG = -1,         /* invalid argument (k out of range, null pointer, ...) */
+    KC_ERR_HIP = -2,         /* a HIP runtime call failed */
+    KC_ERR_NOMEM = -3,       /* device or pinned host allocation failed */
+    KC_ERR_STATE = -4,       /* call out of order (e.g. emit before finish) */
+    KC_ERR_NODEV = -5,       /* no usable GPU */
+    KC_ERR_CAPACITY = -6     /* a device table could not grow enough */
+} kc_status;
+
+typedef enum kc_partition {
+    KC_PARTITION_PREFIX = 0, /* fixed 256 root prefixes, then the next key bits */
+    KC_PARTITION_LEARNED = 1 /* learned CDF model -> balanced key-range buckets */
+} kc_partition;
+
+typedef struct kc_config {
+    uint32_t k;               /* k-mer length, 1..63 */
+    uint32_t partition;       /* kc_partition */
+    uint32_t min_count;       /* root record filter on the repeat count, inclusive */
+    uint32_t max_count;       /*   (append_export_record, src/NewKmerTree.h:1159-1168); 0 = UINT32_MAX */
+    uint64_t expected_bytes;  /* hint: total sequence bytes to be pushed (0 = grow on demand) */
+    uint32_t sub_bits;        /* fine fan-out bits below the root prefix (0 = automatic, max 8) */
+    uint32_t device;          /* HIP device ordinal */
+    uint32_t classifier_num;  /* LPT owner count reported by kc_plan (reference: (n-2-n/8)/2) */
+    uint32_t reserved;
+} kc_config;
+
+typedef struct kc_stats {
+    uint32_t k;                  /* k-mer length of the handle */
+    uint32_t key_words;          /* N: u64 words per key */
+    uint64_t total_kmers;        /* K: every window of k valid bases, counted once */
+    uint64_t firsts;             /* keys reported first by the Bloom filter (low.bin) */
+    uint64_t records;            /* root records after the min/max filter */
+    uint64_t records_unfiltered; /* distinct keys seen again (before the filter) */
+    uint64_t spilled_keys;       /* distinct keys whose count lived in the global spill table */
+    uint64_t sequence_bytes;     /* bytes resident on the device (blocks + separators) */
+    uint64_t fine_buckets;       /* work units of the counting kernel */
+    uint64_t max_bucket_keys;    /* largest fine bucket, in keys */
+    double mean_bucket_keys;     /* K / fine_buckets */
+    double ms_extract_count;     /* device time per phase of the last kc_finish/kc_count_resident */
+    double ms_extract_scatter;
+    double ms_sub_count;
+    double ms_sub_scatter;
+    double ms_count;
+    double ms_compact;
+    double ms_total;             /* first kernel start to last kernel end */
+    double ms_h2d;               /* host -> device sequence copies (pushes) */
+    uint64_t stage0_sample_kmers;
+    uint64_t prefix_counts[256]; /* Stage-0 root prefix histogram, after the >= 1000 clamp */
+    uint64_t bloom_capacity[256];/* reference Bloom sizing (words) from the Stage-0 sample */
+    uint8_t prefix_owners[256];  /* LPT owner of each root prefix */
+} kc_stats;
+
+typedef struct kc_engine kc_engine;
+
+/* Records callback: `records` holds n ExportRecord<N> structs (N key words, u32
+ * count, zero padding to 8 bytes: 16 bytes at N=1, 24 at N=2). */
+typedef int (*kc_records_cb)(void* user, const void* records, uint64_t n);
+/* Keys callback: `keys` holds n keys of N u64 words each. */
+typedef int (*kc_keys_cb)(void* user, const uint64_t* keys, uint64_t n);
+
+KC_API int kc_create(const kc_config* cfg, kc_engine** out);
+KC_API int kc_sample_block(kc_engine* h, const char* data, uint64_t len);
+KC_API int kc_plan(kc_engine* h);
+KC_API int kc_push_block(kc_engine* h, const char* data, uint64_t len);
+KC_API int kc_finish(kc_engine* h);
+KC_API int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user);
+KC_API int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user);
+KC_API int kc_stats_get(kc_engine* h, kc_stats* out);
+KC_API int kc_destroy(kc_engine* h);
+
+/* Re-runs the whole device pipeline on the sequence already resident in HBM
+ * (no host transfer inside).  Used to time the counting path with inputs in
+ * HBM; results replace those of the previous run.  No reference counterpart:
+ * the reference cannot separate ingest from counting. */
+KC_API int kc_count_resident(kc_engine* h);
+
+/* Number of u64 words per key for k (1 or 2), 0 if unsupported. */
+KC_API uint32_t kc_words_for_k(uint32_t k);
+
+/* Human-readable text of the last error on this handle (never NULL). */
+KC_API const char* kc_last_error(kc_engine* h);
+
+/* Test hook, no reference counterpart: the device XXH3-128 of n keys of nw
+ * words (the Bloom filter's hash, src/BloomFilter.h:209); out gets n (low64,
+ * high64) pairs.  Lets tests pin the device hash against the host one. */
+KC_API int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KMC_H */

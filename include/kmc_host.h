@@ -1,0 +1,63 @@
+/* kmc_host.h -- host side of the drop-in: FASTQ chunking, output files and the
+ * `Tree`-equivalent driver, all calling the engine through include/kmc.h.
+ */
+#ifndef KMC_HOST_H
+#define KMC_HOST_H
+
+#include <stdint.h>
+
+#include "kmc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Block sink: one sequence-only block as FastqReader publishes it. */
+typedef int (*kmc_block_cb)(void* user, const char* data, uint64_t len);
+
+/* FASTQ bytes -> sequence-only blocks of at most block_size bytes
+ * (replaces FastqReader<N>::read, src/FastqReader.h:140-290: 4-line records,
+ * only the sequence line kept and '\n'-terminated, a line split at a full
+ * block carries its last k-1 bytes -- or the bytes after the block's last
+ * '\n', if nearer -- into the next block, :338-365).  `fastq` is the whole
+ * input in memory; kmc_fastq_blocks_file streams a file (plain or gzip, the
+ * reference's gzip detection, :92-104) in 2 MiB reads.  Returns 0 or the first
+ * nonzero callback / I/O status. */
+KC_API int kmc_fastq_blocks(const char* fastq, uint64_t len, uint32_t k, uint64_t block_size,
+                            kmc_block_cb cb, void* user);
+KC_API int kmc_fastq_blocks_file(const char* path, uint32_t k, uint64_t block_size,
+                                 uint64_t max_bytes, kmc_block_cb cb, void* user);
+
+/* low.bin record packing (src/ExportWriter.h:522-550): per key k/32 raw
+ * little-endian words, then the top ceil(2*(k%32)/8) bytes of the masked tail
+ * word.  out needs n * kmc_packed_bytes(k) bytes. */
+KC_API uint64_t kmc_packed_bytes(uint32_t k);
+KC_API void kmc_pack_low(const uint64_t* keys, uint64_t n, uint32_t k, uint8_t* out);
+
+/* Write tmp_dir/low.bin and tmp_dir/root_{0..255}.bin from finished engine
+ * results (ExportWriter + FinalDrainWriter formats; all 256 root files are
+ * created, empty ones included, as the reference does). */
+KC_API int kmc_write_outputs(kc_engine* h, const char* tmp_dir);
+
+/* The `Tree` pipeline (src/main.cpp:111-312) on the GPU engine: Stage 0 on the
+ * first sample_bytes of FASTQ, then the whole file, then outputs in tmp_dir.
+ * Fills *stats.  Prints the reference's summary lines when verbose. */
+typedef struct kmc_tree_opts {
+    const char* fastq_path;
+    const char* tmp_dir;
+    uint32_t k;
+    uint32_t n_thread;      /* reference thread budget; sets the LPT owner count */
+    uint32_t min_count;
+    uint32_t max_count;     /* 0 = UINT32_MAX */
+    uint32_t partition;     /* kc_partition */
+    uint32_t verbose;
+    uint64_t sample_bytes;  /* Stage-0 sample size (0 = 256 MiB, the reference's cap) */
+} kmc_tree_opts;
+
+KC_API int kmc_tree_run(const kmc_tree_opts* opts, kc_stats* stats, double* wall_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KMC_HOST_H */

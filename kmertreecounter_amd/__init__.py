@@ -1,0 +1,313 @@
+"""kmertreecounter_amd -- MI355X k-mer counting engine (Python mirror of the C ABI).
+
+The product is the native library ``libkmc.so`` (HIP kernels for gfx950 plus
+the C++ host layer) declared by ``include/kmc.h`` / ``include/kmc_host.h``.
+This module binds it with ctypes for tests, the benchmark and scripting.  It
+has no compute path of its own: when the library is missing, or no GPU is
+visible, the calls fail loudly.
+
+Reference seams mirrored here (KmerTreeCounter tree_v4):
+  Engine.push_block  <- FastqReader -> parser ring (src/FastqReader.h:367-388)
+  Engine.finish      <- pipeline joins + final drain (src/main.cpp:242-264)
+  Engine.firsts      <- low.bin contents (src/ExportWriter.h:400-550)
+  Engine.root        <- root_{id}.bin contents (src/FinalDrainWriter.h:31-91)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libkmc.so"
+TREE_PATH = PKG / "kmc_tree"
+
+KC_PARTITION_PREFIX = 0
+KC_PARTITION_LEARNED = 1
+
+
+class KmcError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_uint32), ("partition", ctypes.c_uint32),
+        ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
+        ("expected_bytes", ctypes.c_uint64), ("sub_bits", ctypes.c_uint32),
+        ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_uint32), ("key_words", ctypes.c_uint32),
+        ("total_kmers", ctypes.c_uint64), ("firsts", ctypes.c_uint64),
+        ("records", ctypes.c_uint64), ("records_unfiltered", ctypes.c_uint64),
+        ("spilled_keys", ctypes.c_uint64), ("sequence_bytes", ctypes.c_uint64),
+        ("fine_buckets", ctypes.c_uint64), ("max_bucket_keys", ctypes.c_uint64),
+        ("mean_bucket_keys", ctypes.c_double),
+        ("ms_extract_count", ctypes.c_double), ("ms_extract_scatter", ctypes.c_double),
+        ("ms_sub_count", ctypes.c_double), ("ms_sub_scatter", ctypes.c_double),
+        ("ms_count", ctypes.c_double), ("ms_compact", ctypes.c_double),
+        ("ms_total", ctypes.c_double), ("ms_h2d", ctypes.c_double),
+        ("stage0_sample_kmers", ctypes.c_uint64),
+        ("prefix_counts", ctypes.c_uint64 * 256), ("bloom_capacity", ctypes.c_uint64 * 256),
+        ("prefix_owners", ctypes.c_uint8 * 256),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class SynthSpec(ctypes.Structure):
+    _fields_ = [
+        ("config", ctypes.c_uint32), ("threads", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+        ("scale", ctypes.c_double), ("format", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+        ("genome_len", ctypes.c_uint64), ("n_reads", ctypes.c_uint64),
+    ]
+
+
+class TreeOpts(ctypes.Structure):
+    _fields_ = [
+        ("fastq_path", ctypes.c_char_p), ("tmp_dir", ctypes.c_char_p), ("k", ctypes.c_uint32),
+        ("n_thread", ctypes.c_uint32), ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
+        ("partition", ctypes.c_uint32), ("verbose", ctypes.c_uint32), ("sample_bytes", ctypes.c_uint64),
+    ]
+
+
+KEYS_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64)
+RECS_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64)
+BLOCK_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_char), ctypes.c_uint64)
+
+EXPORTED = [
+    "kc_create", "kc_sample_block", "kc_plan", "kc_push_block", "kc_finish", "kc_emit_firsts",
+    "kc_emit_root", "kc_stats_get", "kc_destroy", "kc_count_resident", "kc_words_for_k",
+    "kc_last_error", "kc_debug_xxh3", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
+    "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
+]
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libkmc.so (built in-tree by kmertreecounter_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise KmcError(f"{LIB_PATH} is missing: build it with `python -m kmertreecounter_amd.build`")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, u32, u64, cp = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_char_p
+    L.kc_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+    L.kc_sample_block.argtypes = [vp, cp, u64]
+    L.kc_plan.argtypes = [vp]
+    L.kc_push_block.argtypes = [vp, cp, u64]
+    L.kc_finish.argtypes = [vp]
+    L.kc_count_resident.argtypes = [vp]
+    L.kc_emit_firsts.argtypes = [vp, KEYS_CB, vp]
+    L.kc_emit_root.argtypes = [vp, u32, RECS_CB, vp]
+    L.kc_stats_get.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.kc_destroy.argtypes = [vp]
+    L.kc_words_for_k.argtypes = [u32]
+    L.kc_words_for_k.restype = u32
+    L.kc_last_error.argtypes = [vp]
+    L.kc_last_error.restype = cp
+    L.kc_debug_xxh3.argtypes = [vp, u64, u32, vp]
+    L.kcs_generate.argtypes = [ctypes.POINTER(SynthSpec), ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(u64),
+                               ctypes.POINTER(u64)]
+    L.kcs_free.argtypes = [vp]
+    L.kmc_fastq_blocks.argtypes = [cp, u64, u32, u64, BLOCK_CB, vp]
+    L.kmc_fastq_blocks_file.argtypes = [cp, u32, u64, u64, BLOCK_CB, vp]
+    L.kmc_packed_bytes.argtypes = [u32]
+    L.kmc_packed_bytes.restype = u64
+    L.kmc_pack_low.argtypes = [vp, u64, u32, vp]
+    L.kmc_write_outputs.argtypes = [vp, cp]
+    L.kmc_tree_run.argtypes = [ctypes.POINTER(TreeOpts), ctypes.POINTER(Stats), ctypes.POINTER(ctypes.c_double)]
+    _lib = L
+    return L
+
+
+def words_for_k(k: int) -> int:
+    return 1 if k <= 32 else 2
+
+
+def record_dtype(nw: int) -> np.dtype:
+    """numpy view of ExportRecord<N> (src/definition.h:159-163)."""
+    return np.dtype([("key", "<u8", (nw,)), ("count", "<u4"), ("pad", "<u4")])
+
+
+# ------------------------------------------------------------------ synthetic data
+
+def generate(config: int, scale: float = 1.0, seed: int = 1, fmt: str = "fastq", threads: int | None = None,
+             genome_len: int = 0, n_reads: int = 0) -> bytes:
+    """Deterministic synthetic input of workload `config` (1..5); fmt 'fastq' or 'seq'."""
+    L = lib()
+    spec = SynthSpec(config, threads or min(16, os.cpu_count() or 1), seed, float(scale),
+                     0 if fmt == "fastq" else 1, 0, genome_len, n_reads)
+    out = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    nr = ctypes.c_uint64()
+    rc = L.kcs_generate(ctypes.byref(spec), ctypes.byref(out), ctypes.byref(n), ctypes.byref(nr))
+    if rc != 0:
+        raise KmcError(f"kcs_generate failed: {rc}")
+    try:
+        return ctypes.string_at(out, n.value)
+    finally:
+        L.kcs_free(out)
+
+
+# ------------------------------------------------------------------ host helpers
+
+def fastq_blocks(fastq: bytes, k: int, block_size: int = 32 * 1024) -> list[bytes]:
+    """Sequence-only blocks of a FASTQ buffer, as the reference FastqReader emits them."""
+    blocks: list[bytes] = []
+
+    @BLOCK_CB
+    def cb(_user, data, n):
+        blocks.append(ctypes.string_at(data, n))
+        return 0
+
+    rc = lib().kmc_fastq_blocks(fastq, len(fastq), k, block_size, cb, None)
+    if rc != 0:
+        raise KmcError(f"kmc_fastq_blocks failed: {rc}")
+    return blocks
+
+
+def pack_low(keys: np.ndarray, k: int) -> bytes:
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    nw = words_for_k(k)
+    n = keys.size // nw
+    out = ctypes.create_string_buffer(int(lib().kmc_packed_bytes(k)) * n or 1)
+    lib().kmc_pack_low(keys.ctypes.data, n, k, out)
+    return out.raw[: int(lib().kmc_packed_bytes(k)) * n]
+
+
+# ------------------------------------------------------------------ engine
+
+class Engine:
+    """One counting job on one GPU (a kc_engine handle)."""
+
+    def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
+                 expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3):
+        self.k = k
+        self.nw = words_for_k(k)
+        self._L = lib()
+        cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, 0)
+        h = ctypes.c_void_p()
+        rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise KmcError(f"kc_create failed with status {rc}")
+        self._h = h
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self._L.kc_last_error(self._h)
+            raise KmcError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.kc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sample_block(self, data: bytes) -> None:
+        self._check(self._L.kc_sample_block(self._h, data, len(data)), "kc_sample_block")
+
+    def plan(self) -> None:
+        self._check(self._L.kc_plan(self._h), "kc_plan")
+
+    def push_block(self, data: bytes) -> None:
+        self._check(self._L.kc_push_block(self._h, data, len(data)), "kc_push_block")
+
+    def finish(self) -> None:
+        self._check(self._L.kc_finish(self._h), "kc_finish")
+
+    def count_resident(self) -> None:
+        self._check(self._L.kc_count_resident(self._h), "kc_count_resident")
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._check(self._L.kc_stats_get(self._h, ctypes.byref(s)), "kc_stats_get")
+        return s
+
+    def firsts(self) -> np.ndarray:
+        """First-occurrence keys (the low.bin key set), shape (n, N)."""
+        parts: list[np.ndarray] = []
+        nw = self.nw
+
+        @KEYS_CB
+        def cb(_user, keys, n):
+            parts.append(np.ctypeslib.as_array(keys, shape=(n * nw,)).copy())
+            return 0
+
+        self._check(self._L.kc_emit_firsts(self._h, cb, None), "kc_emit_firsts")
+        flat = np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint64)
+        return flat.reshape(-1, nw)
+
+    def root(self, root_id: int) -> np.ndarray:
+        """Root records of one root id as a structured ExportRecord<N> array."""
+        dt = record_dtype(self.nw)
+        parts: list[bytes] = []
+
+        @RECS_CB
+        def cb(_user, recs, n):
+            parts.append(ctypes.string_at(recs, n * dt.itemsize))
+            return 0
+
+        self._check(self._L.kc_emit_root(self._h, root_id, cb, None), "kc_emit_root")
+        return np.frombuffer(b"".join(parts), dtype=dt)
+
+    def records(self) -> np.ndarray:
+        return np.concatenate([self.root(r) for r in range(256)])
+
+    def write_outputs(self, tmp_dir: str | os.PathLike) -> None:
+        self._check(self._L.kmc_write_outputs(self._h, str(tmp_dir).encode()), "kmc_write_outputs")
+
+
+def count_sequence(seq: bytes, k: int, **kw) -> tuple[np.ndarray, np.ndarray, Stats]:
+    """Count a sequence-only buffer ('\\n'-separated lines) on the GPU."""
+    with Engine(k, **kw) as e:
+        e.push_block(seq)
+        e.finish()
+        return e.firsts(), e.records(), e.stats()
+
+
+def debug_xxh3(keys: np.ndarray, nw: int) -> np.ndarray:
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    n = keys.size // nw
+    out = np.zeros(2 * n, dtype=np.uint64)
+    rc = lib().kc_debug_xxh3(keys.ctypes.data, n, nw, out.ctypes.data)
+    if rc != 0:
+        raise KmcError(f"kc_debug_xxh3 failed: {rc}")
+    return out.reshape(n, 2)
+
+
+def tree_run(fastq_path: str, tmp_dir: str, k: int, n_thread: int = 8, min_count: int = 1, max_count: int = 0,
+             partition: int = KC_PARTITION_PREFIX, verbose: bool = False, sample_bytes: int = 0) -> tuple[Stats, float]:
+    o = TreeOpts(str(fastq_path).encode(), str(tmp_dir).encode(), k, n_thread, min_count, max_count, partition,
+                 1 if verbose else 0, sample_bytes)
+    s = Stats()
+    wall = ctypes.c_double()
+    rc = lib().kmc_tree_run(ctypes.byref(o), ctypes.byref(s), ctypes.byref(wall))
+    if rc != 0:
+        raise KmcError(f"kmc_tree_run failed: {rc}")
+    return s, wall.value

@@ -1,0 +1,188 @@
+// kmc_device.h -- device-side building blocks shared by the engine kernels:
+// base encoding, canonical k-mer construction, XXH3-128 for 8/16-byte keys,
+// Bloom probe and key routing.  gfx950 (CDNA4), wave64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kmc {
+
+constexpr int kWave = 64;
+constexpr uint64_t kEmpty = ~0ULL;  // never a canonical key word for k <= 63
+
+template <int NW>
+struct Key {
+    uint64_t w[NW];
+};
+
+// ---------------------------------------------------------------- encoding
+// ASCII -> 2-bit code: A/a=0, C/c=1, G/g=2, T/t/U/u=3 (reference table,
+// src/GetKmer.h:22-36).  ((c >> 1) & 3) maps A,C,G,T to 0,1,3,2; xor with
+// its high bit swaps G and T.  Validity is the case-folded byte being one of
+// A C G T U (src/FastqParser.h:211-236); everything else resets a window.
+__device__ __forceinline__ void encode_word(uint32_t w, int first_idx, uint32_t& code, uint32_t& valid)
+{
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t ch = (w >> (8 * b)) & 0xFFu;
+        const uint32_t u = ch & 0xDFu;
+        const uint32_t ok = (u == 'A') | (u == 'C') | (u == 'G') | (u == 'T') | (u == 'U');
+        uint32_t cd = (ch >> 1) & 3u;
+        cd ^= cd >> 1;
+        const int idx = first_idx + b;
+        code |= cd << (30 - 2 * idx);
+        valid |= ok << (15 - idx);
+    }
+}
+
+// 16 sequence bytes -> 16 codes (MSB-first, first byte in bits 31..30) and a
+// 16-bit validity mask (first byte in bit 15).
+__device__ __forceinline__ void encode16(uint4 q, uint32_t& code, uint32_t& valid)
+{
+    code = 0;
+    valid = 0;
+    encode_word(q.x, 0, code, valid);
+    encode_word(q.y, 4, code, valid);
+    encode_word(q.z, 8, code, valid);
+    encode_word(q.w, 12, code, valid);
+}
+
+// reverse the order of the 32 two-bit groups of x
+__device__ __forceinline__ uint64_t rev2(uint64_t x)
+{
+    x = __builtin_bitreverse64(x);
+    return ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+}
+
+// Canonical key of the window whose forward 2-bit pack (first base most
+// significant, right aligned, 2k bits) is F.  The reference keeps
+//   seq = pack(reverse(window))    (src/GetKmer.h:71-73, shift right, new base on top)
+//   rev = pack(complement(window)) (src/GetKmer.h:75-77, shift left, complement at the low slot)
+// left aligned in N words and takes the smaller (src/GetKmer.h:181-206).
+__device__ __forceinline__ Key<1> canon1(uint64_t F, uint32_t k)
+{
+    const uint64_t mask = (k == 32) ? ~0ULL : ((1ULL << (2 * k)) - 1ULL);
+    F &= mask;
+    const uint64_t rev = (~F & mask) << (64 - 2 * k);
+    const uint64_t seq = rev2(F);
+    Key<1> r;
+    r.w[0] = seq < rev ? seq : rev;
+    return r;
+}
+
+// 33 <= k <= 63: F = (fhi, flo) is a 2k-bit value
+__device__ __forceinline__ Key<2> canon2(uint64_t fhi, uint64_t flo, uint32_t k)
+{
+    const uint32_t hb = 2 * k - 64;  // bits of fhi in use, 2..62
+    const uint64_t hmask = (1ULL << hb) - 1ULL;
+    fhi &= hmask;
+    const uint64_t chi = ~fhi & hmask, clo = ~flo;
+    const uint32_t sh = 128 - 2 * k;  // 2..62
+    const uint64_t rhi = (chi << sh) | (clo >> (64 - sh));
+    const uint64_t rlo = clo << sh;
+    const uint64_t shi = rev2(flo), slo = rev2(fhi);
+    const bool seq_less = (shi < rhi) || (shi == rhi && slo < rlo);
+    Key<2> r;
+    r.w[0] = seq_less ? shi : rhi;
+    r.w[1] = seq_less ? slo : rlo;
+    return r;
+}
+
+template <int NW>
+__device__ __forceinline__ bool key_eq(const Key<NW>& a, const Key<NW>& b)
+{
+    bool e = true;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) e = e && (a.w[i] == b.w[i]);
+    return e;
+}
+
+// ---------------------------------------------------------------- XXH3-128
+// Restatement of XXH3_128bits for 8- and 16-byte inputs with seed 0 and the
+// default secret (include/xxhash.h v0.8.3: XXH3_len_4to8_128b :6900,
+// XXH3_len_9to16_128b :6927, XXH3_avalanche :4616).  The constants below are
+// the secret words those paths read: XOR of secret bytes 16..23/24..31,
+// 32..39/40..47 and 48..55/56..63, read little-endian.
+constexpr uint64_t kXxP64_1 = 0x9E3779B185EBCA87ULL;
+constexpr uint64_t kXxP64_2 = 0xC2B2AE3D27D4EB4FULL;
+constexpr uint64_t kXxP32_2 = 0x85EBCA77ULL;
+constexpr uint64_t kXxMx1 = 0x165667919E3779F9ULL;
+constexpr uint64_t kXxMx2 = 0x9FB21C651E98DF25ULL;
+constexpr uint64_t kXxFlip8 = 0xdb979083e96dd4deULL ^ 0x1f67b3b7a4a44072ULL;
+constexpr uint64_t kXxFlip16L = 0x78e5c0cc4ee679cbULL ^ 0x2172ffcc7dd05a82ULL;
+constexpr uint64_t kXxFlip16H = 0x8e2443f7744608b8ULL ^ 0x4c263a81e69035e0ULL;
+
+__device__ __forceinline__ uint64_t xx_avalanche(uint64_t h)
+{
+    h ^= h >> 37;
+    h *= kXxMx1;
+    return h ^ (h >> 32);
+}
+
+__device__ __forceinline__ void xxh3_128(const Key<1>& key, uint64_t& lo, uint64_t& hi)
+{
+    const uint64_t keyed = key.w[0] ^ kXxFlip8;
+    const uint64_t m = kXxP64_1 + (8ULL << 2);
+    uint64_t l = keyed * m;
+    uint64_t h = __umul64hi(keyed, m);
+    h += l << 1;
+    l ^= h >> 3;
+    l ^= l >> 35;
+    l *= kXxMx2;
+    l ^= l >> 28;
+    lo = l;
+    hi = xx_avalanche(h);
+}
+
+__device__ __forceinline__ void xxh3_128(const Key<2>& key, uint64_t& lo, uint64_t& hi)
+{
+    const uint64_t in_lo = key.w[0];
+    uint64_t in_hi = key.w[1];
+    const uint64_t a = in_lo ^ in_hi ^ kXxFlip16L;
+    uint64_t ml = a * kXxP64_1;
+    uint64_t mh = __umul64hi(a, kXxP64_1);
+    ml += 15ULL << 54;
+    in_hi ^= kXxFlip16H;
+    mh += in_hi + (uint64_t)(uint32_t)in_hi * (kXxP32_2 - 1);
+    ml ^= __builtin_bswap64(mh);
+    uint64_t hl = ml * kXxP64_2;
+    uint64_t hh = __umul64hi(ml, kXxP64_2);
+    hh += mh * kXxP64_2;
+    lo = xx_avalanche(hl);
+    hi = xx_avalanche(hh);
+}
+
+// Bloom probe of ConcurrentBloomFilter (src/BloomFilter.h:207-214, :248-258):
+// word = low64 & (cap-1); three bits (low64 + i*(high64|1)) & 63.
+__device__ __forceinline__ uint64_t bloom_mask(uint64_t lo, uint64_t hi)
+{
+    const uint64_t h2 = hi | 1ULL;
+    return (1ULL << (lo & 63)) | (1ULL << ((lo + h2) & 63)) | (1ULL << ((lo + 2 * h2) & 63));
+}
+
+// ---------------------------------------------------------------- routing
+// Coarse bucket (0..255) and fine sub-bucket (0..2^sbits-1).
+//   prefix: coarse = top 8 key bits (the reference's root prefix, ROOT_BASES=4,
+//           src/definition.h:28), fine = the next sbits bits.
+//   learned: coarse/fine from monotone boundaries on key word 0 produced by the
+//           learned CDF model (bucket = number of boundaries <= key).
+struct Route {
+    uint32_t mode;   // 0 prefix, 1 learned
+    uint32_t sbits;  // 0..8
+    const uint64_t* coarse_bounds;  // learned: 255 ascending boundaries
+    const uint64_t* fine_bounds;    // learned: 256 x (2^sbits - 1) ascending boundaries
+};
+
+__device__ __forceinline__ uint32_t upper_count(const uint64_t* b, uint32_t n, uint64_t x)
+{
+    // number of entries <= x in ascending b[0..n)
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+}  // namespace kmc

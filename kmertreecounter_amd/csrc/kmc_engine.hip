@@ -1,0 +1,610 @@
+// kmc_engine.hip -- host side of the engine: device memory, block staging,
+// the pipeline driver and the C ABI declared in include/kmc.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kmc.h"
+#include "kmc_kernels.h"
+#include "kmc_plan.h"
+
+using namespace kmc;
+
+namespace {
+
+constexpr uint64_t kStageBytes = 64ULL << 20;   // pinned staging buffer for pushed blocks
+constexpr uint64_t kEmitChunk = 32ULL << 20;    // pinned bounce buffer for emits
+constexpr uint64_t kFineTarget = 32768;         // keys per fine bucket the fan-out aims for
+
+enum Ctr { C_FIRST = 0, C_REC, C_SPILLED, C_ERR, C_MAXB, C_DISTINCT, C_NCTR = 8 };
+
+}  // namespace
+
+struct kc_engine {
+    kc_config cfg{};
+    uint32_t k = 0, nw = 0;
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+
+    // resident sequence
+    uint8_t* d_seq = nullptr;
+    uint64_t seq_len = 0, seq_alloc = 0;
+    char* h_stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_busy[2] = {false, false};
+    int cur_stage = 0;
+    uint64_t stage_len = 0;
+    char last_byte = '\n';
+    double h2d_ms = 0;
+
+    // Stage 0
+    std::vector<char> sample;
+    bool planned = false;
+    PlanResult plan;
+
+    // work buffers
+    uint64_t key_cap = 0;  // keys
+    uint64_t* d_A = nullptr;
+    uint64_t* d_B = nullptr;
+    unsigned long long* d_small = nullptr;  // coarse counts/cursors/offsets/tile starts
+    unsigned long long* d_fcounts = nullptr;
+    unsigned long long* d_fcursor = nullptr;
+    unsigned long long* d_fine_off = nullptr;
+    uint32_t nf_alloc = 0;
+    uint8_t* d_recs = nullptr;
+    uint8_t* d_recs_sorted = nullptr;
+    uint64_t rec_cap = 0;
+    unsigned long long* d_spill_keys = nullptr;
+    uint32_t* d_spill_cnt = nullptr;
+    uint64_t spill_cap = 0;
+    bool spill_dirty = true;
+    unsigned long long* d_ctr = nullptr;
+    uint64_t* d_bounds = nullptr;  // learned: 255 coarse + 256*255 fine boundaries
+    bool bounds_valid = false;
+
+    hipEvent_t ev[8] = {};
+
+    // results
+    bool have_results = false;
+    uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0;
+    uint32_t nf_last = 0;
+    const uint64_t* res_firsts = nullptr;
+    std::vector<unsigned long long> root_off;
+    double phase_ms[7] = {};
+    char* h_bounce = nullptr;
+
+    int fail(int code, const std::string& what)
+    {
+        err = what;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* where)
+    {
+        err = std::string(where) + ": " + hipGetErrorString(e);
+        return KC_ERR_HIP;
+    }
+};
+
+#define HIPX(expr)                                              \
+    do {                                                        \
+        hipError_t e_ = (expr);                                 \
+        if (e_ != hipSuccess) return h->hip_fail(e_, #expr);    \
+    } while (0)
+
+namespace {
+
+uint64_t ext_tile(uint32_t nw) { return nw == 1 ? ExtractCfg<1>::kThreads * 16ULL : ExtractCfg<2>::kThreads * 16ULL; }
+uint64_t sub_tile(uint32_t nw) { return nw == 1 ? (uint64_t)SubCfg<1>::kTile : (uint64_t)SubCfg<2>::kTile; }
+uint64_t rec_bytes(uint32_t nw) { return 8ULL * nw + 8; }
+
+uint64_t pow2_at_least(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int ensure_seq_capacity(kc_engine* h, uint64_t need_len)
+{
+    // allocation covers need_len rounded up to a whole extraction tile plus slack
+    const uint64_t tile = ext_tile(h->nw);
+    const uint64_t want = ((need_len + tile - 1) / tile) * tile + 256;
+    if (want <= h->seq_alloc) return KC_OK;
+    uint64_t alloc = std::max<uint64_t>(want, h->seq_alloc * 2);
+    uint8_t* nd = nullptr;
+    hipError_t e = hipMalloc(&nd, alloc);
+    if (e != hipSuccess) {
+        alloc = want;
+        e = hipMalloc(&nd, alloc);
+        if (e != hipSuccess) return h->fail(KC_ERR_NOMEM, "device allocation for the sequence buffer failed");
+    }
+    if (h->d_seq) {
+        HIPX(hipStreamSynchronize(h->st));
+        HIPX(hipMemcpy(nd, h->d_seq, h->seq_len, hipMemcpyDeviceToDevice));
+        HIPX(hipFree(h->d_seq));
+    }
+    h->d_seq = nd;
+    h->seq_alloc = alloc;
+    return KC_OK;
+}
+
+int flush_stage(kc_engine* h)
+{
+    if (h->stage_len == 0) return KC_OK;
+    int rc = ensure_seq_capacity(h, h->seq_len + h->stage_len);
+    if (rc) return rc;
+    const int s = h->cur_stage;
+    HIPX(hipMemcpyAsync(h->d_seq + h->seq_len, h->h_stage[s], h->stage_len, hipMemcpyHostToDevice, h->st));
+    HIPX(hipEventRecord(h->stage_ev[s], h->st));
+    h->stage_busy[s] = true;
+    h->seq_len += h->stage_len;
+    h->stage_len = 0;
+    h->cur_stage ^= 1;
+    const int n = h->cur_stage;
+    if (h->stage_busy[n]) {
+        HIPX(hipEventSynchronize(h->stage_ev[n]));
+        h->stage_busy[n] = false;
+    }
+    return KC_OK;
+}
+
+int append_bytes(kc_engine* h, const char* data, uint64_t len)
+{
+    while (len) {
+        const uint64_t room = kStageBytes - h->stage_len;
+        const uint64_t n = std::min(room, len);
+        std::memcpy(h->h_stage[h->cur_stage] + h->stage_len, data, n);
+        h->stage_len += n;
+        data += n;
+        len -= n;
+        if (h->stage_len == kStageBytes) {
+            int rc = flush_stage(h);
+            if (rc) return rc;
+        }
+    }
+    return KC_OK;
+}
+
+void free_work(kc_engine* h)
+{
+    hipFree(h->d_A); h->d_A = nullptr;
+    hipFree(h->d_B); h->d_B = nullptr;
+    h->key_cap = 0;
+}
+
+int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
+{
+    const uint64_t kb = 8ULL * h->nw;
+    if (h->key_cap < padded_len) {
+        free_work(h);
+        if (hipMalloc(&h->d_A, padded_len * kb) != hipSuccess || hipMalloc(&h->d_B, padded_len * kb) != hipSuccess) {
+            free_work(h);
+            return h->fail(KC_ERR_NOMEM, "device allocation for the key buffers failed");
+        }
+        h->key_cap = padded_len;
+    }
+    if (!h->d_small) {
+        // counts[8*256] cursor[8*256] coarse_off[257] tile_start[257] root_counts[256] root_off[257] root_cursor[256]
+        HIPX(hipMalloc(&h->d_small, sizeof(unsigned long long) * (8 * 256 * 2 + 257 * 2 + 256 + 257 + 256 + 64)));
+        HIPX(hipMalloc(&h->d_ctr, sizeof(unsigned long long) * C_NCTR));
+    }
+    if (h->nf_alloc < nf) {
+        hipFree(h->d_fcounts); hipFree(h->d_fcursor); hipFree(h->d_fine_off);
+        HIPX(hipMalloc(&h->d_fcounts, sizeof(unsigned long long) * kXcds * nf));
+        HIPX(hipMalloc(&h->d_fcursor, sizeof(unsigned long long) * kXcds * nf));
+        HIPX(hipMalloc(&h->d_fine_off, sizeof(unsigned long long) * (nf + 1)));
+        h->nf_alloc = nf;
+    }
+    const uint64_t want_rec = std::max<uint64_t>(1 << 16, padded_len / 8);
+    if (h->rec_cap < want_rec) {
+        hipFree(h->d_recs); hipFree(h->d_recs_sorted);
+        h->d_recs = h->d_recs_sorted = nullptr;
+        if (hipMalloc(&h->d_recs, want_rec * rec_bytes(h->nw)) != hipSuccess ||
+            hipMalloc(&h->d_recs_sorted, want_rec * rec_bytes(h->nw)) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the record buffers failed");
+        h->rec_cap = want_rec;
+    }
+    const uint64_t want_spill = std::min<uint64_t>(1ULL << 28, std::max<uint64_t>(1ULL << 16, pow2_at_least(padded_len / 64)));
+    if (h->spill_cap < want_spill) {
+        hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
+        if (hipMalloc(&h->d_spill_keys, want_spill * 8ULL * h->nw) != hipSuccess ||
+            hipMalloc(&h->d_spill_cnt, want_spill * 4ULL) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the spill table failed");
+        h->spill_cap = want_spill;
+        h->spill_dirty = true;
+    }
+    return KC_OK;
+}
+
+uint32_t choose_sbits(const kc_engine* h, uint64_t padded_len)
+{
+    if (h->cfg.sub_bits) return std::min<uint32_t>(8, h->cfg.sub_bits);
+    const double per_coarse = (double)padded_len / 256.0;
+    uint32_t sb = 0;
+    while (sb < 8 && per_coarse / (double)(1u << sb) > (double)kFineTarget) sb++;
+    return sb;
+}
+
+template <int NW>
+int run_pipeline_t(kc_engine* h)
+{
+    hipStream_t st = h->st;
+    const uint64_t tile = ext_tile(h->nw);
+    const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
+    const uint32_t sbits = (h->cfg.partition == KC_PARTITION_LEARNED && h->bounds_valid) ? h->plan.sbits
+                                                                                           : choose_sbits(h, padded);
+    const uint32_t nf = 256u << sbits;
+    int rc = ensure_work(h, padded > 0 ? padded : tile, nf);
+    if (rc) return rc;
+
+    Route rt{};
+    rt.mode = (h->cfg.partition == KC_PARTITION_LEARNED && h->bounds_valid) ? 1u : 0u;
+    rt.sbits = sbits;
+    rt.coarse_bounds = h->d_bounds;
+    rt.fine_bounds = h->d_bounds ? h->d_bounds + 255 : nullptr;
+
+    unsigned long long* counts = h->d_small;
+    unsigned long long* cursor = counts + 8 * 256;
+    unsigned long long* coarse_off = cursor + 8 * 256;
+    unsigned long long* tile_start = coarse_off + 257;
+    unsigned long long* root_counts = tile_start + 257;
+    unsigned long long* root_off = root_counts + 256;
+    unsigned long long* root_cursor = root_off + 257;
+
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        if (padded > h->seq_len) HIPX(hipMemsetAsync(h->d_seq + h->seq_len, '\n', padded - h->seq_len, st));
+        HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * (8 * 256 * 2 + 257 * 2 + 256 + 257 + 256), st));
+        HIPX(hipMemsetAsync(h->d_ctr, 0, sizeof(unsigned long long) * C_NCTR, st));
+        if (h->spill_dirty) {
+            HIPX(hipMemsetAsync(h->d_spill_keys, 0xFF, h->spill_cap * 8ULL * NW, st));
+            HIPX(hipMemsetAsync(h->d_spill_cnt, 0, h->spill_cap * 4ULL, st));
+            h->spill_dirty = false;
+        }
+        if (sbits) HIPX(hipMemsetAsync(h->d_fcounts, 0, sizeof(unsigned long long) * kXcds * nf, st));
+
+        HIPX(hipEventRecord(h->ev[0], st));
+        HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
+        HIPX(hipEventRecord(h->ev[1], st));
+        HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st));
+        HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
+        HIPX(hipEventRecord(h->ev[2], st));
+        const uint64_t* src = h->d_A;
+        uint64_t* firsts = h->d_B;
+        const unsigned long long* fine_off = coarse_off;
+        if (sbits) {
+            const uint64_t max_tiles = padded / sub_tile(NW) + 256;
+            HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
+            HIPX(hipEventRecord(h->ev[3], st));
+            HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, st));
+            HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
+            HIPX(hipEventRecord(h->ev[4], st));
+            src = h->d_B;
+            firsts = h->d_A;
+            fine_off = h->d_fine_off;
+        } else {
+            HIPX(hipEventRecord(h->ev[3], st));
+            HIPX(hipEventRecord(h->ev[4], st));
+        }
+        CountArgs a{};
+        a.firsts = firsts;
+        a.first_count = h->d_ctr + C_FIRST;
+        a.recs = h->d_recs;
+        a.rec_count = h->d_ctr + C_REC;
+        a.rec_cap = h->rec_cap;
+        a.spill_keys = h->d_spill_keys;
+        a.spill_cnt = h->d_spill_cnt;
+        a.spill_cap = h->spill_cap;
+        a.spilled = h->d_ctr + C_SPILLED;
+        a.error_flags = h->d_ctr + C_ERR;
+        a.max_bucket = h->d_ctr + C_MAXB;
+        a.distinct_rep = h->d_ctr + C_DISTINCT;
+        a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
+        a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
+        HIPX(launch_count<NW>(nf, src, fine_off, a, st));
+        HIPX(hipEventRecord(h->ev[5], st));
+        HIPX(launch_group_records<NW>(false, h->d_recs, h->d_ctr + C_REC, h->rec_cap, root_counts, nullptr, nullptr, st));
+        HIPX(launch_rec_scan(root_counts, root_off, root_cursor, st));
+        HIPX(launch_group_records<NW>(true, h->d_recs, h->d_ctr + C_REC, h->rec_cap, root_counts, root_cursor, h->d_recs_sorted, st));
+        HIPX(hipEventRecord(h->ev[6], st));
+        HIPX(hipStreamSynchronize(st));
+
+        unsigned long long ctr[C_NCTR];
+        HIPX(hipMemcpy(ctr, h->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+        unsigned long long coff256 = 0;
+        HIPX(hipMemcpy(&coff256, coarse_off + 256, sizeof(coff256), hipMemcpyDeviceToHost));
+        h->spill_dirty = ctr[C_SPILLED] != 0;
+        if (ctr[C_ERR] & kErrSpillFull) {
+            if (h->spill_cap >= (1ULL << 31)) return h->fail(KC_ERR_CAPACITY, "spill table cannot grow further");
+            const uint64_t ncap = h->spill_cap * 4;
+            hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
+            h->d_spill_keys = nullptr; h->d_spill_cnt = nullptr; h->spill_cap = 0;
+            if (hipMalloc(&h->d_spill_keys, ncap * 8ULL * NW) != hipSuccess || hipMalloc(&h->d_spill_cnt, ncap * 4ULL) != hipSuccess)
+                return h->fail(KC_ERR_NOMEM, "spill table growth failed");
+            h->spill_cap = ncap;
+            h->spill_dirty = true;
+            continue;
+        }
+        if (ctr[C_ERR] & kErrRecFull) {
+            const uint64_t ncap = std::max<uint64_t>(h->rec_cap * 2, ctr[C_REC] + 1024);
+            hipFree(h->d_recs); hipFree(h->d_recs_sorted);
+            h->d_recs = h->d_recs_sorted = nullptr; h->rec_cap = 0;
+            if (hipMalloc(&h->d_recs, ncap * rec_bytes(NW)) != hipSuccess || hipMalloc(&h->d_recs_sorted, ncap * rec_bytes(NW)) != hipSuccess)
+                return h->fail(KC_ERR_NOMEM, "record buffer growth failed");
+            h->rec_cap = ncap;
+            continue;
+        }
+        h->K = coff256;
+        h->n_first = ctr[C_FIRST];
+        h->n_rec = ctr[C_REC];
+        h->n_spilled = ctr[C_SPILLED];
+        h->max_bucket = ctr[C_MAXB];
+        h->n_distinct_rep = ctr[C_DISTINCT];
+        h->nf_last = nf;
+        h->res_firsts = firsts;
+        h->root_off.assign(257, 0);
+        HIPX(hipMemcpy(h->root_off.data(), root_off, sizeof(unsigned long long) * 257, hipMemcpyDeviceToHost));
+        float ms = 0;
+        for (int p = 0; p < 6; ++p) {
+            HIPX(hipEventElapsedTime(&ms, h->ev[p], h->ev[p + 1]));
+            h->phase_ms[p] = ms;
+        }
+        HIPX(hipEventElapsedTime(&ms, h->ev[0], h->ev[6]));
+        h->phase_ms[6] = ms;
+        h->have_results = true;
+        return KC_OK;
+    }
+    return h->fail(KC_ERR_CAPACITY, "device tables did not converge after growth");
+}
+
+int run_pipeline(kc_engine* h)
+{
+    h->have_results = false;
+    return h->nw == 1 ? run_pipeline_t<1>(h) : run_pipeline_t<2>(h);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t kc_words_for_k(uint32_t k)
+{
+    if (k == 0 || k > 63) return 0;
+    return k <= 32 ? 1u : 2u;
+}
+
+const char* kc_last_error(kc_engine* h)
+{
+    return (h && !h->err.empty()) ? h->err.c_str() : "";
+}
+
+int kc_create(const kc_config* cfg, kc_engine** out)
+{
+    if (!cfg || !out) return KC_ERR_ARG;
+    *out = nullptr;
+    const uint32_t nw = kc_words_for_k(cfg->k);
+    if (!nw) return KC_ERR_ARG;
+    if (cfg->partition > 1 || cfg->sub_bits > 8) return KC_ERR_ARG;
+    if (cfg->max_count && cfg->min_count > cfg->max_count) return KC_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_ERR_NODEV;
+    if ((int)cfg->device >= ndev) return KC_ERR_ARG;
+    kc_engine* h = new (std::nothrow) kc_engine();
+    if (!h) return KC_ERR_NOMEM;
+    h->cfg = *cfg;
+    h->k = cfg->k;
+    h->nw = nw;
+    h->device = (int)cfg->device;
+    auto bail = [&](int code) { kc_destroy(h); return code; };
+    if (hipSetDevice(h->device) != hipSuccess) return bail(KC_ERR_HIP);
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return bail(KC_ERR_HIP);
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc((void**)&h->h_stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+        if (hipEventCreateWithFlags(&h->stage_ev[i], hipEventDisableTiming) != hipSuccess) return bail(KC_ERR_HIP);
+    }
+    if (hipHostMalloc((void**)&h->h_bounce, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+    for (auto& e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(KC_ERR_HIP);
+    if (cfg->expected_bytes) {
+        if (ensure_seq_capacity(h, cfg->expected_bytes + cfg->expected_bytes / 64 + 4096) != KC_OK) return bail(KC_ERR_NOMEM);
+    }
+    *out = h;
+    return KC_OK;
+}
+
+int kc_destroy(kc_engine* h)
+{
+    if (!h) return KC_OK;
+    hipSetDevice(h->device);
+    if (h->st) hipStreamSynchronize(h->st);
+    hipFree(h->d_seq);
+    free_work(h);
+    hipFree(h->d_small);
+    hipFree(h->d_ctr);
+    hipFree(h->d_fcounts);
+    hipFree(h->d_fcursor);
+    hipFree(h->d_fine_off);
+    hipFree(h->d_recs);
+    hipFree(h->d_recs_sorted);
+    hipFree(h->d_spill_keys);
+    hipFree(h->d_spill_cnt);
+    hipFree(h->d_bounds);
+    for (int i = 0; i < 2; ++i) {
+        if (h->h_stage[i]) hipHostFree(h->h_stage[i]);
+        if (h->stage_ev[i]) hipEventDestroy(h->stage_ev[i]);
+    }
+    if (h->h_bounce) hipHostFree(h->h_bounce);
+    for (auto& e : h->ev)
+        if (e) hipEventDestroy(e);
+    if (h->st) hipStreamDestroy(h->st);
+    delete h;
+    return KC_OK;
+}
+
+int kc_sample_block(kc_engine* h, const char* data, uint64_t len)
+{
+    if (!h || (!data && len)) return KC_ERR_ARG;
+    if (!len) return KC_OK;
+    h->sample.insert(h->sample.end(), data, data + len);
+    if (h->sample.back() != '\n') h->sample.push_back('\n');
+    return KC_OK;
+}
+
+int kc_plan(kc_engine* h)
+{
+    if (!h) return KC_ERR_ARG;
+    HIPX(hipSetDevice(h->device));
+    int rc = stage0_plan(h->k, h->cfg, h->sample, h->st, h->plan, h->err);
+    if (rc) return rc;
+    if (h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.bounds.empty()) {
+        if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * 255)));
+        HIPX(hipMemcpy(h->d_bounds, h->plan.bounds.data(), sizeof(uint64_t) * h->plan.bounds.size(), hipMemcpyHostToDevice));
+        h->bounds_valid = true;
+    }
+    h->planned = true;
+    return KC_OK;
+}
+
+int kc_push_block(kc_engine* h, const char* data, uint64_t len)
+{
+    if (!h || (!data && len)) return KC_ERR_ARG;
+    if (!len) return KC_OK;
+    HIPX(hipSetDevice(h->device));
+    h->have_results = false;
+    int rc = append_bytes(h, data, len);
+    if (rc) return rc;
+    // a block ending inside a line is followed by one that starts with its carry:
+    // a separator keeps windows from spanning the two (each block is parsed from a
+    // cleared state, src/FastqParser.h:193)
+    if (data[len - 1] != '\n') {
+        const char nl = '\n';
+        rc = append_bytes(h, &nl, 1);
+        if (rc) return rc;
+    }
+    return KC_OK;
+}
+
+int kc_finish(kc_engine* h)
+{
+    if (!h) return KC_ERR_ARG;
+    HIPX(hipSetDevice(h->device));
+    hipEvent_t t0, t1;
+    HIPX(hipEventCreate(&t0));
+    HIPX(hipEventCreate(&t1));
+    HIPX(hipEventRecord(t0, h->st));
+    int rc = flush_stage(h);
+    if (rc) return rc;
+    HIPX(hipEventRecord(t1, h->st));
+    HIPX(hipEventSynchronize(t1));
+    float ms = 0;
+    hipEventElapsedTime(&ms, t0, t1);
+    h->h2d_ms += ms;
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    for (int i = 0; i < 2; ++i) {
+        if (h->stage_busy[i]) {
+            HIPX(hipEventSynchronize(h->stage_ev[i]));
+            h->stage_busy[i] = false;
+        }
+    }
+    return run_pipeline(h);
+}
+
+int kc_count_resident(kc_engine* h)
+{
+    if (!h) return KC_ERR_ARG;
+    HIPX(hipSetDevice(h->device));
+    if (h->stage_len) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
+    return run_pipeline(h);
+}
+
+int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
+{
+    if (!h || !cb) return KC_ERR_ARG;
+    if (!h->have_results) return h->fail(KC_ERR_STATE, "no results: call kc_finish first");
+    HIPX(hipSetDevice(h->device));
+    const uint64_t kb = 8ULL * h->nw;
+    const uint64_t per = kEmitChunk / kb;
+    for (uint64_t i = 0; i < h->n_first; i += per) {
+        const uint64_t n = std::min(per, h->n_first - i);
+        HIPX(hipMemcpy(h->h_bounce, h->res_firsts + i * h->nw, n * kb, hipMemcpyDeviceToHost));
+        int rc = cb(user, reinterpret_cast<const uint64_t*>(h->h_bounce), n);
+        if (rc) return h->fail(rc, "keys callback returned an error");
+    }
+    return KC_OK;
+}
+
+int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
+{
+    if (!h || !cb || root_id > 255) return KC_ERR_ARG;
+    if (!h->have_results) return h->fail(KC_ERR_STATE, "no results: call kc_finish first");
+    HIPX(hipSetDevice(h->device));
+    const uint64_t rb = rec_bytes(h->nw);
+    const uint64_t per = kEmitChunk / rb;
+    const uint64_t b = h->root_off[root_id], e = h->root_off[root_id + 1];
+    for (uint64_t i = b; i < e; i += per) {
+        const uint64_t n = std::min(per, e - i);
+        HIPX(hipMemcpy(h->h_bounce, h->d_recs_sorted + i * rb, n * rb, hipMemcpyDeviceToHost));
+        int rc = cb(user, h->h_bounce, n);
+        if (rc) return h->fail(rc, "records callback returned an error");
+    }
+    return KC_OK;
+}
+
+int kc_stats_get(kc_engine* h, kc_stats* s)
+{
+    if (!h || !s) return KC_ERR_ARG;
+    std::memset(s, 0, sizeof(*s));
+    s->k = h->k;
+    s->key_words = h->nw;
+    s->total_kmers = h->K;
+    s->firsts = h->n_first;
+    s->records = h->n_rec;
+    s->records_unfiltered = h->n_distinct_rep;
+    s->spilled_keys = h->n_spilled;
+    s->sequence_bytes = h->seq_len;
+    s->fine_buckets = h->nf_last;
+    s->max_bucket_keys = h->max_bucket;
+    s->mean_bucket_keys = h->nf_last ? (double)h->K / h->nf_last : 0.0;
+    s->ms_extract_count = h->phase_ms[0];
+    s->ms_extract_scatter = h->phase_ms[1];
+    s->ms_sub_count = h->phase_ms[2];
+    s->ms_sub_scatter = h->phase_ms[3];
+    s->ms_count = h->phase_ms[4];
+    s->ms_compact = h->phase_ms[5];
+    s->ms_total = h->phase_ms[6];
+    s->ms_h2d = h->h2d_ms;
+    if (h->planned) {
+        s->stage0_sample_kmers = h->plan.sample_kmers;
+        for (int i = 0; i < 256; ++i) {
+            s->prefix_counts[i] = h->plan.prefix_counts[i];
+            s->bloom_capacity[i] = h->plan.bloom_capacity[i];
+            s->prefix_owners[i] = h->plan.owners[i];
+        }
+    }
+    return KC_OK;
+}
+
+int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out)
+{
+    if ((!keys || !out) && n) return KC_ERR_ARG;
+    if (nw != 1 && nw != 2) return KC_ERR_ARG;
+    if (!n) return KC_OK;
+    uint64_t *dk = nullptr, *dout = nullptr;
+    if (hipMalloc(&dk, n * 8 * nw) != hipSuccess) return KC_ERR_NOMEM;
+    if (hipMalloc(&dout, n * 16) != hipSuccess) { hipFree(dk); return KC_ERR_NOMEM; }
+    int rc = KC_OK;
+    if (hipMemcpy(dk, keys, n * 8 * nw, hipMemcpyHostToDevice) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && (nw == 1 ? launch_debug_hash<1>(dk, n, dout, nullptr) : launch_debug_hash<2>(dk, n, dout, nullptr)) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && hipMemcpy(out, dout, n * 16, hipMemcpyDeviceToHost) != hipSuccess) rc = KC_ERR_HIP;
+    hipFree(dk);
+    hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

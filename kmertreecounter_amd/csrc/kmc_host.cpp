@@ -1,0 +1,351 @@
+// kmc_host.cpp -- FASTQ chunking, output files and the Tree-equivalent driver.
+//
+// Everything here runs on the host and reaches the GPU only through the C ABI
+// of include/kmc.h.
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <zlib.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kmc_host.h"
+
+namespace {
+
+// Sequence-only block builder with the reference reader's semantics
+// (src/FastqReader.h:140-388).  The output does not depend on how the input
+// is split into feed() calls.
+class FastqChunker {
+public:
+    FastqChunker(uint32_t k, uint64_t block_size, kmc_block_cb cb, void* user)
+        : overlap_(k > 1 ? k - 1 : 0), block_size_(block_size), cb_(cb), user_(user), block_(block_size) {}
+
+    int feed(const char* data, uint64_t len)
+    {
+        uint64_t pos = 0;
+        while (pos < len) {
+            if (state_ != kSeq) {
+                const void* nl = std::memchr(data + pos, '\n', len - pos);
+                if (!nl) return 0;  // rest of a non-sequence line
+                pos = (uint64_t)((const char*)nl - data) + 1;
+                state_ = (state_ + 1) & 3;
+                continue;
+            }
+            const void* nl = std::memchr(data + pos, '\n', len - pos);
+            const uint64_t seq_len = nl ? (uint64_t)((const char*)nl - (data + pos)) : len - pos;
+            uint64_t copied = 0;
+            while (copied < seq_len) {
+                if (fill_ == block_size_) {
+                    // full block inside a line: keep the tail that the next
+                    // block needs to complete windows (:338-365)
+                    take_carry();
+                    int rc = publish();
+                    if (rc) return rc;
+                    std::memcpy(block_.data(), carry_, carry_len_);
+                    fill_ = carry_len_;
+                    last_nl_ = kNone;
+                }
+                const uint64_t n = std::min(seq_len - copied, block_size_ - fill_);
+                std::memcpy(block_.data() + fill_, data + pos + copied, n);
+                fill_ += n;
+                copied += n;
+            }
+            pos += seq_len;
+            if (!nl) return 0;  // the line continues in the next feed()
+            if (fill_ == block_size_) {
+                int rc = publish();
+                if (rc) return rc;
+            }
+            last_nl_ = fill_;
+            block_[fill_++] = '\n';
+            pos++;
+            state_ = kPlus;
+        }
+        return 0;
+    }
+
+    int finish() { return publish(); }
+
+private:
+    static constexpr int kSeq = 1, kPlus = 2;
+    static constexpr uint64_t kNone = ~0ULL;
+
+    void take_carry()
+    {
+        carry_len_ = 0;
+        if (overlap_ == 0 || fill_ == 0) return;
+        const uint64_t keep = fill_ < overlap_ ? fill_ : overlap_;
+        const uint64_t start = fill_ - keep;
+        if (last_nl_ != kNone && last_nl_ >= start) {
+            carry_len_ = fill_ - (last_nl_ + 1);
+            std::memcpy(carry_, block_.data() + last_nl_ + 1, carry_len_);
+            return;
+        }
+        std::memcpy(carry_, block_.data() + start, keep);
+        carry_len_ = keep;
+    }
+
+    int publish()
+    {
+        int rc = 0;
+        if (fill_ > 0) rc = cb_(user_, block_.data(), fill_);
+        fill_ = 0;
+        last_nl_ = kNone;
+        return rc;
+    }
+
+    uint64_t overlap_, block_size_;
+    kmc_block_cb cb_;
+    void* user_;
+    std::vector<char> block_;
+    uint64_t fill_ = 0;
+    uint64_t last_nl_ = kNone;
+    int state_ = 0;
+    char carry_[128];
+    uint64_t carry_len_ = 0;
+};
+
+constexpr uint64_t kReadChunk = 2ULL << 20;  // FASTQ_FILE_CHUNK_SIZE, src/definition.h:90
+
+struct FileSource {
+    int fd = -1;
+    gzFile gz = nullptr;
+    ~FileSource()
+    {
+        if (gz) gzclose(gz);
+        if (fd >= 0) ::close(fd);
+    }
+    int open(const char* path)
+    {
+        fd = ::open(path, O_RDONLY);
+        if (fd < 0) return -1;
+        unsigned char magic[2] = {0, 0};
+        const ssize_t n = ::read(fd, magic, 2);
+        if (n == 2 && magic[0] == 0x1F && magic[1] == 0x8B) {
+            gz = gzopen(path, "rb");
+            if (!gz) return -1;
+        } else if (::lseek(fd, 0, SEEK_SET) != 0) {
+            return -1;
+        }
+        return 0;
+    }
+    int64_t read(char* buf, uint64_t cap)
+    {
+        if (gz) return gzread(gz, buf, (unsigned)cap);
+        return ::read(fd, buf, cap);
+    }
+};
+
+bool write_all(int fd, const void* p, uint64_t n)
+{
+    const char* c = (const char*)p;
+    while (n) {
+        const ssize_t w = ::write(fd, c, n);
+        if (w < 0) {
+            if (errno == EINTR) continue;
+            return false;
+        }
+        c += w;
+        n -= (uint64_t)w;
+    }
+    return true;
+}
+
+struct LowWriter {
+    int fd = -1;
+    uint32_t k = 0;
+    uint32_t nw = 1;
+    uint64_t pb = 0;
+    std::vector<uint8_t> buf;
+    bool ok = true;
+};
+
+int low_cb(void* user, const uint64_t* keys, uint64_t n)
+{
+    LowWriter* w = (LowWriter*)user;
+    const uint64_t per = (4ULL << 20) / w->pb;
+    for (uint64_t i = 0; i < n; i += per) {
+        const uint64_t m = std::min(per, n - i);
+        w->buf.resize(m * w->pb);
+        kmc_pack_low(keys + i * w->nw, m, w->k, w->buf.data());
+        if (!write_all(w->fd, w->buf.data(), w->buf.size())) return -1;
+    }
+    return 0;
+}
+
+int root_cb(void* user, const void* recs, uint64_t n)
+{
+    const int fd = *(int*)user;
+    // ExportRecord<N> bytes straight from the device buffer (padding is zero)
+    return write_all(fd, recs, n * (8ULL * (uint64_t)((int*)user)[1] + 8)) ? 0 : -1;
+}
+
+struct PushCtx {
+    kc_engine* h;
+    uint64_t bytes = 0;
+    int (*fn)(kc_engine*, const char*, uint64_t);
+};
+
+int push_cb(void* user, const char* data, uint64_t len)
+{
+    PushCtx* c = (PushCtx*)user;
+    c->bytes += len;
+    return c->fn(c->h, data, len);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kmc_fastq_blocks(const char* fastq, uint64_t len, uint32_t k, uint64_t block_size, kmc_block_cb cb, void* user)
+{
+    if ((!fastq && len) || !cb || block_size <= 128 || k == 0 || k > 128) return KC_ERR_ARG;
+    FastqChunker ch(k, block_size, cb, user);
+    int rc = ch.feed(fastq, len);
+    if (rc) return rc;
+    return ch.finish();
+}
+
+int kmc_fastq_blocks_file(const char* path, uint32_t k, uint64_t block_size, uint64_t max_bytes, kmc_block_cb cb,
+                          void* user)
+{
+    if (!path || !cb || block_size <= 128 || k == 0 || k > 128) return KC_ERR_ARG;
+    FileSource src;
+    if (src.open(path) != 0) return KC_ERR_ARG;
+    FastqChunker ch(k, block_size, cb, user);
+    std::vector<char> buf(kReadChunk);
+    uint64_t total = 0;
+    for (;;) {
+        uint64_t want = kReadChunk;
+        if (max_bytes && total + want > max_bytes) want = max_bytes - total;
+        if (want == 0) break;
+        const int64_t n = src.read(buf.data(), want);
+        if (n < 0) return KC_ERR_ARG;
+        if (n == 0) break;
+        total += (uint64_t)n;
+        int rc = ch.feed(buf.data(), (uint64_t)n);
+        if (rc) return rc;
+    }
+    return ch.finish();
+}
+
+uint64_t kmc_packed_bytes(uint32_t k)
+{
+    const uint64_t full = k / 32, tail_bits = 2ULL * (k % 32), tail_bytes = (tail_bits + 7) / 8;
+    return full * 8 + tail_bytes;
+}
+
+void kmc_pack_low(const uint64_t* keys, uint64_t n, uint32_t k, uint8_t* out)
+{
+    const uint32_t nw = k <= 32 ? 1 : (k <= 64 ? 2 : 4);
+    const uint64_t full = k / 32, tail_bits = 2ULL * (k % 32), tail_bytes = (tail_bits + 7) / 8;
+    const uint64_t mask = tail_bytes ? (~0ULL) << (64 - tail_bits) : 0;
+    uint8_t* p = out;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t* key = keys + i * nw;
+        std::memcpy(p, key, full * 8);  // little-endian host
+        p += full * 8;
+        if (tail_bytes) {
+            const uint64_t t = key[full] & mask;
+            std::memcpy(p, reinterpret_cast<const uint8_t*>(&t) + (8 - tail_bytes), tail_bytes);
+            p += tail_bytes;
+        }
+    }
+}
+
+int kmc_write_outputs(kc_engine* h, const char* tmp_dir)
+{
+    if (!h || !tmp_dir) return KC_ERR_ARG;
+    kc_stats st;
+    int src = kc_stats_get(h, &st);
+    if (src) return src;
+    const uint32_t k = st.k;
+    std::string dir(tmp_dir);
+    if (!dir.empty() && dir.back() != '/') dir.push_back('/');
+    LowWriter lw;
+    lw.k = k;
+    lw.nw = k <= 32 ? 1 : 2;
+    lw.pb = kmc_packed_bytes(k);
+    lw.fd = ::open((dir + "low.bin").c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (lw.fd < 0) return KC_ERR_ARG;
+    int rc = kc_emit_firsts(h, low_cb, &lw);
+    ::close(lw.fd);
+    if (rc) return rc;
+    for (uint32_t r = 0; r < 256; ++r) {
+        int ctx[2];
+        ctx[0] = ::open((dir + "root_" + std::to_string(r) + ".bin").c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        ctx[1] = (int)lw.nw;
+        if (ctx[0] < 0) return KC_ERR_ARG;
+        rc = kc_emit_root(h, r, root_cb, ctx);
+        ::close(ctx[0]);
+        if (rc) return rc;
+    }
+    return KC_OK;
+}
+
+int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
+{
+    if (!o || !o->fastq_path || !o->tmp_dir || !kc_words_for_k(o->k)) return KC_ERR_ARG;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    struct stat sb;
+    if (::stat(o->fastq_path, &sb) != 0) {
+        std::fprintf(stderr, "Failed to open file\n");
+        return KC_ERR_ARG;
+    }
+    kc_config cfg{};
+    cfg.k = o->k;
+    cfg.partition = o->partition;
+    cfg.min_count = o->min_count ? o->min_count : 1;
+    cfg.max_count = o->max_count;
+    cfg.expected_bytes = (uint64_t)sb.st_size;
+    // classifier thread count of the reference split (src/main.cpp:114-117)
+    const uint32_t n = o->n_thread ? o->n_thread : 8;
+    const uint32_t parsers = n / 8 > 0 ? n / 8 : 1;
+    const uint32_t budget = n > 2 + parsers ? n - 2 - parsers : 1;
+    cfg.classifier_num = budget / 2 ? budget / 2 : 1;
+    kc_engine* h = nullptr;
+    int rc = kc_create(&cfg, &h);
+    if (rc) return rc;
+    // Stage 0: head of the file (the reference samples up to 256 MiB, src/FastqPreReader.h:110)
+    PushCtx sc{h, 0, kc_sample_block};
+    const uint64_t sample = o->sample_bytes ? o->sample_bytes : (256ULL << 20);
+    rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, sample, push_cb, &sc);
+    if (!rc) rc = kc_plan(h);
+    const auto t1 = clk::now();
+    PushCtx pc{h, 0, kc_push_block};
+    if (!rc) rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, 0, push_cb, &pc);
+    if (!rc) rc = kc_finish(h);
+    const auto t2 = clk::now();
+    if (!rc) rc = kmc_write_outputs(h, o->tmp_dir);
+    const auto t3 = clk::now();
+    kc_stats st{};
+    if (!rc) rc = kc_stats_get(h, &st);
+    if (rc) {
+        std::fprintf(stderr, "kmc_tree: %s (status %d)\n", kc_last_error(h), rc);
+        kc_destroy(h);
+        return rc;
+    }
+    auto us = [](clk::duration d) { return (long long)std::chrono::duration_cast<std::chrono::microseconds>(d).count(); };
+    if (o->verbose) {
+        std::printf("Total read k-mer count: %llu\n", (unsigned long long)st.total_kmers);
+        std::printf("Init elapsed us: %lld\n", us(t1 - t0));
+        std::printf("Mid elapsed us: %lld\n", us(t2 - t1));
+        std::printf("Final elapsed us: %lld\n", us(t3 - t2));
+        std::printf("Total elapsed us: %lld\n", us(t3 - t0));
+        std::printf("Device counting ms: %.3f (k-mers/s %.3e)\n", st.ms_total,
+                    st.ms_total > 0 ? st.total_kmers / (st.ms_total * 1e-3) : 0.0);
+        std::printf("High frequency unique k-mer : %llu\n", (unsigned long long)st.records);
+    }
+    if (stats) *stats = st;
+    if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
+    kc_destroy(h);
+    return KC_OK;
+}
+
+}  // extern "C"

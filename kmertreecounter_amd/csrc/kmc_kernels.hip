@@ -1,0 +1,809 @@
+// kmc_kernels.hip -- the counting path as hand-written HIP for gfx950.
+//
+// Pipeline over the sequence bytes resident in HBM (one pass of each kernel):
+//   P0  k_extract<count>    bytes -> canonical keys -> coarse-bucket histogram
+//   S0  k_scan_coarse       bucket offsets, per-XCD write cursors, tile lists
+//   P1  k_extract<scatter>  bytes -> keys, LDS counting sort by coarse bucket,
+//                           coalesced runs into A (bucket-major, XCD-minor)
+//   P2a k_sub<count>        per coarse bucket: fine-bucket histogram
+//   S1  k_scan_fine
+//   P2b k_sub<scatter>      A -> B grouped by fine bucket
+//   P3  k_count             one workgroup per fine bucket: LDS Bloom filter
+//                           (first occurrences -> firsts list) and LDS
+//                           open-addressing count table for repeats, with a
+//                           global spill table behind it; emits root records
+//   P3b k_spill_emit        records of keys that lived in the spill table
+//   P4  k_rec_hist/scatter  records grouped by root id (key word 0 >> 56)
+// Workgroup-to-XCD: workgroup i runs on XCD i % 8, so write cursors are kept
+// per XCD and each XCD fills its own sub-range of a bucket (partial cache lines
+// merge in that XCD's L2 instead of crossing XCDs).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kmc_device.h"
+#include "kmc_kernels.h"
+
+namespace kmc {
+
+// ------------------------------------------------------------------ helpers
+
+// exclusive scan of one value per thread across the workgroup; returns the
+// exclusive prefix, total in *total.  scratch: >= blockDim/64 + 1 entries.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* scratch, T* total)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nwaves = (blockDim.x + kWave - 1) / kWave;
+    T x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        T y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) scratch[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (int w = 0; w < nwaves; ++w) {
+            T t = scratch[w];
+            scratch[w] = acc;
+            acc += t;
+        }
+        scratch[nwaves] = acc;
+    }
+    __syncthreads();
+    T r = scratch[wid] + x - v;
+    *total = scratch[nwaves];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    return (uint32_t)__popcll(mask & ((1ULL << lane) - 1ULL));
+}
+
+template <int NW>
+__device__ __forceinline__ Key<NW> load_key(const uint64_t* p, uint64_t i)
+{
+    Key<NW> k;
+    if constexpr (NW == 1) {
+        k.w[0] = p[i];
+    } else {
+        const ulonglong2 v = reinterpret_cast<const ulonglong2*>(p)[i];
+        k.w[0] = v.x;
+        k.w[1] = v.y;
+    }
+    return k;
+}
+
+template <int NW>
+__device__ __forceinline__ void store_key(uint64_t* p, uint64_t i, const Key<NW>& k)
+{
+    if constexpr (NW == 1) {
+        p[i] = k.w[0];
+    } else {
+        ulonglong2 v;
+        v.x = k.w[0];
+        v.y = k.w[1];
+        reinterpret_cast<ulonglong2*>(p)[i] = v;
+    }
+}
+
+__device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s_cb, uint64_t w0)
+{
+    if (rt.mode == 0) return (uint32_t)(w0 >> 56);
+    return upper_count(s_cb, 255, w0);
+}
+
+__device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_fb, uint64_t w0)
+{
+    if (rt.sbits == 0) return 0;
+    if (rt.mode == 0) return (uint32_t)((w0 >> (56 - rt.sbits)) & ((1u << rt.sbits) - 1u));
+    return upper_count(s_fb, (1u << rt.sbits) - 1u, w0);
+}
+
+// ------------------------------------------------------------------ P0 / P1
+
+// Window ending at position i (0..15) of the thread's 16-byte word.  c/v hold
+// the code/validity words from H words before the thread's word up to it
+// (c[H] is the thread's own word).
+template <int NW>
+__device__ __forceinline__ bool window_key(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<NW>& key);
+
+template <>
+__device__ __forceinline__ bool window_key<1>(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<1>& key)
+{
+    // 96-bit code string c[0]:c[1]:c[2], base i of c[2] at bit 2*(15-i)
+    const int s = 2 * (15 - i);
+    const uint64_t lo = ((uint64_t)c[1] << 32) | c[2];
+    const uint64_t hi = c[0];
+    const uint64_t F = s ? ((lo >> s) | (hi << (64 - s))) : lo;
+    const uint64_t V = ((uint64_t)v[0] << 32) | ((uint64_t)v[1] << 16) | (uint64_t)v[2];
+    const uint64_t km = (k == 64) ? ~0ULL : ((1ULL << k) - 1ULL);
+    const bool ok = ((V >> (15 - i)) & km) == km;
+    key = canon1(F, k);
+    return ok;
+}
+
+template <>
+__device__ __forceinline__ bool window_key<2>(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<2>& key)
+{
+    // 160-bit code string c[0]:..:c[4]
+    const int s = 2 * (15 - i);
+    const uint64_t x0 = ((uint64_t)c[3] << 32) | c[4];
+    const uint64_t x1 = ((uint64_t)c[1] << 32) | c[2];
+    const uint64_t x2 = c[0];
+    const uint64_t flo = s ? ((x0 >> s) | (x1 << (64 - s))) : x0;
+    const uint64_t fhi = s ? ((x1 >> s) | (x2 << (64 - s))) : x1;
+    // 80-bit validity string v[0]:..:v[4]
+    const int t = 15 - i;
+    const uint64_t vlo = ((uint64_t)v[1] << 48) | ((uint64_t)v[2] << 32) | ((uint64_t)v[3] << 16) | (uint64_t)v[4];
+    const uint64_t vhi = v[0];
+    const uint64_t W = t ? ((vlo >> t) | (vhi << (64 - t))) : vlo;
+    const uint64_t km = (k == 64) ? ~0ULL : ((1ULL << k) - 1ULL);
+    const bool ok = (W & km) == km;
+    key = canon2(fhi, flo, k);
+    return ok;
+}
+
+template <int NW, bool SCATTER>
+__global__ void __launch_bounds__(ExtractCfg<NW>::kThreads)
+k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Route rt,
+          unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
+          uint64_t* __restrict__ out)
+{
+    constexpr int T = ExtractCfg<NW>::kThreads;
+    constexpr int H = (NW == 1) ? 2 : 4;
+    constexpr int TILE = T * 16;
+    __shared__ uint32_t s_code[T + H];
+    __shared__ uint32_t s_valid[T + H];
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_off[256];
+    __shared__ unsigned long long s_gbase[256];
+    __shared__ uint32_t s_scan[T / kWave + 1];
+    __shared__ uint64_t s_cb[256];
+    __shared__ Key<NW> s_keys[SCATTER ? TILE : 1];
+    __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
+
+    const int t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * TILE;
+    const uint32_t xcd = blockIdx.x % kXcds;
+
+    // 16 bytes per thread, aligned; the buffer is padded with '\n' to a whole tile
+    {
+        const uint4 q = reinterpret_cast<const uint4*>(seq + base)[t];
+        uint32_t cd, vd;
+        encode16(q, cd, vd);
+        s_code[H + t] = cd;
+        s_valid[H + t] = vd;
+    }
+    if (t < H) {
+        const int64_t p = (int64_t)base - (int64_t)(H - t) * 16;
+        uint32_t cd = 0, vd = 0;
+        if (p >= 0) encode16(*reinterpret_cast<const uint4*>(seq + p), cd, vd);
+        s_code[t] = cd;
+        s_valid[t] = vd;
+    }
+    if (t < 256) s_hist[t] = 0;
+    if (rt.mode == 1 && t < 255) s_cb[t] = rt.coarse_bounds[t];
+    __syncthreads();
+
+    uint32_t c[H + 1], v[H + 1];
+#pragma unroll
+    for (int j = 0; j <= H; ++j) {
+        c[j] = s_code[t + j];
+        v[j] = s_valid[t + j];
+    }
+    (void)padded_len;
+
+    Key<NW> keys[16];
+    uint32_t bk[16];
+    uint32_t rank[16];
+    uint32_t okm = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        Key<NW> key;
+        const bool ok = window_key<NW>(c, v, i, k, key);
+        keys[i] = key;
+        if (ok) {
+            okm |= 1u << i;
+            const uint32_t b = coarse_of(rt, s_cb, key.w[0]);
+            bk[i] = b;
+            rank[i] = atomicAdd(&s_hist[b], 1u);
+        }
+    }
+    __syncthreads();
+
+    if constexpr (!SCATTER) {
+        if (t < 256) {
+            const uint32_t h = s_hist[t];
+            if (h) atomicAdd(&counts[(size_t)xcd * 256 + t], (unsigned long long)h);
+        }
+        return;
+    } else {
+        uint32_t h = (t < 256) ? s_hist[t] : 0u;
+        uint32_t total;
+        const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
+        if (t < 256) {
+            s_off[t] = off;
+            if (h) s_gbase[t] = atomicAdd(&cursor[(size_t)xcd * 256 + t], (unsigned long long)h);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (okm & (1u << i)) {
+                const uint32_t slot = s_off[bk[i]] + rank[i];
+                s_keys[slot] = keys[i];
+                s_bkt[slot] = (uint8_t)bk[i];
+            }
+        }
+        __syncthreads();
+        for (uint32_t slot = t; slot < total; slot += T) {
+            const uint32_t b = s_bkt[slot];
+            const uint64_t dst = s_gbase[b] + (slot - s_off[b]);
+            store_key<NW>(out, dst, s_keys[slot]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ scans
+
+// counts[XCDS][256] -> coarse_off[257], cursor[XCDS][256], tile_start[257]
+__global__ void __launch_bounds__(256)
+k_scan_coarse(const unsigned long long* __restrict__ counts, unsigned long long* __restrict__ coarse_off,
+              unsigned long long* __restrict__ cursor, unsigned long long* __restrict__ tile_start,
+              uint32_t sub_tile)
+{
+    __shared__ unsigned long long s_scan[8];
+    const int b = threadIdx.x;
+    unsigned long long tot = 0;
+    unsigned long long per[kXcds];
+#pragma unroll
+    for (int x = 0; x < kXcds; ++x) {
+        per[x] = counts[(size_t)x * 256 + b];
+        tot += per[x];
+    }
+    unsigned long long all;
+    const unsigned long long off = block_exclusive_scan<unsigned long long>(tot, s_scan, &all);
+    coarse_off[b] = off;
+    if (b == 255) coarse_off[256] = all;
+    unsigned long long run = off;
+#pragma unroll
+    for (int x = 0; x < kXcds; ++x) {
+        cursor[(size_t)x * 256 + b] = run;
+        run += per[x];
+    }
+    const unsigned long long tiles = (tot + sub_tile - 1) / sub_tile;
+    unsigned long long tall;
+    const unsigned long long toff = block_exclusive_scan<unsigned long long>(tiles, s_scan, &tall);
+    tile_start[b] = toff;
+    if (b == 255) tile_start[256] = tall;
+}
+
+// counts[XCDS][nf] -> fine_off[nf+1], cursor[XCDS][nf] (nf multiple of 1024 or
+// smaller power of two; one workgroup of 1024 threads)
+__global__ void __launch_bounds__(1024)
+k_scan_fine(const unsigned long long* __restrict__ counts, uint32_t nf,
+            unsigned long long* __restrict__ fine_off, unsigned long long* __restrict__ cursor)
+{
+    __shared__ unsigned long long s_scan[17];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nf + 1023) / 1024;
+    const uint32_t f0 = t * per;
+    unsigned long long sum = 0;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t f = f0 + j;
+        if (f < nf) {
+            for (int x = 0; x < kXcds; ++x) sum += counts[(size_t)x * nf + f];
+        }
+    }
+    unsigned long long all;
+    unsigned long long run = block_exclusive_scan<unsigned long long>(sum, s_scan, &all);
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t f = f0 + j;
+        if (f < nf) {
+            fine_off[f] = run;
+            for (int x = 0; x < kXcds; ++x) {
+                cursor[(size_t)x * nf + f] = run;
+                run += counts[(size_t)x * nf + f];
+            }
+        }
+    }
+    if (t == 1023) fine_off[nf] = all;
+}
+
+// ------------------------------------------------------------------ P2
+
+// Tiles of SubCfg::kTile keys inside one coarse bucket of A.
+template <int NW, bool SCATTER>
+__global__ void __launch_bounds__(SubCfg<NW>::kThreads)
+k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coarse_off,
+      const unsigned long long* __restrict__ tile_start, Route rt,
+      unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
+      uint64_t* __restrict__ B)
+{
+    constexpr int T = SubCfg<NW>::kThreads;
+    constexpr int TILE = SubCfg<NW>::kTile;
+    constexpr int PER = TILE / T;
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_off[256];
+    __shared__ unsigned long long s_gbase[256];
+    __shared__ uint32_t s_scan[T / kWave + 1];
+    __shared__ uint64_t s_fb[256];
+    __shared__ Key<NW> s_keys[SCATTER ? TILE : 1];
+    __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
+
+    const int t = threadIdx.x;
+    const unsigned long long tid = blockIdx.x;
+    const unsigned long long ntiles = tile_start[256];
+    if (tid >= ntiles) return;
+    // coarse bucket of this tile: last r with tile_start[r] <= tid
+    uint32_t lo = 0, hi = 256;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tile_start[mid] <= tid) lo = mid; else hi = mid;
+    }
+    const uint32_t r = lo;
+    const unsigned long long beg = coarse_off[r] + (tid - tile_start[r]) * (unsigned long long)TILE;
+    const unsigned long long endr = coarse_off[r + 1];
+    const uint32_t n = (uint32_t)((endr - beg) < (unsigned long long)TILE ? (endr - beg) : TILE);
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t nsub = 1u << rt.sbits;
+    const uint32_t nf = 256u << rt.sbits;
+
+    if (t < 256) s_hist[t] = 0;
+    if (rt.mode == 1 && t < (int)(nsub - 1)) s_fb[t] = rt.fine_bounds[(size_t)r * (nsub - 1) + t];
+    __syncthreads();
+
+    Key<NW> keys[PER];
+    uint32_t sb[PER];
+    uint32_t rank[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint32_t idx = t + j * T;
+        if (idx < n) {
+            keys[j] = load_key<NW>(A, beg + idx);
+            sb[j] = fine_of(rt, s_fb, keys[j].w[0]);
+            rank[j] = atomicAdd(&s_hist[sb[j]], 1u);
+        }
+    }
+    __syncthreads();
+
+    if constexpr (!SCATTER) {
+        if (t < (int)nsub) {
+            const uint32_t h = s_hist[t];
+            if (h) atomicAdd(&counts[(size_t)xcd * nf + ((size_t)r << rt.sbits) + t], (unsigned long long)h);
+        }
+        return;
+    } else {
+        uint32_t h = (t < (int)nsub) ? s_hist[t] : 0u;
+        uint32_t total;
+        const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
+        if (t < (int)nsub) {
+            s_off[t] = off;
+            if (h) s_gbase[t] = atomicAdd(&cursor[(size_t)xcd * nf + ((size_t)r << rt.sbits) + t], (unsigned long long)h);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t idx = t + j * T;
+            if (idx < n) {
+                const uint32_t slot = s_off[sb[j]] + rank[j];
+                s_keys[slot] = keys[j];
+                s_bkt[slot] = (uint8_t)sb[j];
+            }
+        }
+        __syncthreads();
+        for (uint32_t slot = t; slot < total; slot += T) {
+            const uint32_t b = s_bkt[slot];
+            const uint64_t dst = s_gbase[b] + (slot - s_off[b]);
+            store_key<NW>(B, dst, s_keys[slot]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ P3
+
+// Two-word-safe open addressing insert.  Slot s holds key words w[0..NW) that
+// only ever change from kEmpty to a value (each word claimed by CAS in order),
+// so every thread inserting the same key walks the same permanent slot states
+// and lands on the same slot.  Returns the slot or -1 after `limit` probes.
+template <int NW, typename WordPtr>
+__device__ __forceinline__ int64_t table_find_or_claim(WordPtr words, uint64_t stride, uint64_t cap_mask,
+                                                       uint64_t start, uint32_t limit, const Key<NW>& key,
+                                                       bool& claimed)
+{
+    claimed = false;
+    uint64_t s = start & cap_mask;
+    for (uint32_t p = 0; p < limit; ++p) {
+        bool match = true;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (!match) break;
+            unsigned long long* slotw = (unsigned long long*)&words[(uint64_t)w * stride + s];
+            unsigned long long cur = *slotw;
+            if (cur == kEmpty) {
+                cur = atomicCAS(slotw, (unsigned long long)kEmpty, (unsigned long long)key.w[w]);
+                if (cur == kEmpty) {
+                    cur = key.w[w];
+                    if (w == NW - 1) claimed = true;
+                }
+            }
+            match = (cur == key.w[w]);
+        }
+        if (match) return (int64_t)s;
+        s = (s + 1) & cap_mask;
+    }
+    return -1;
+}
+
+template <int NW>
+__device__ __forceinline__ void write_record(uint8_t* recs, uint64_t i, const Key<NW>& key, uint32_t count)
+{
+    uint64_t* p = reinterpret_cast<uint64_t*>(recs + i * RecBytes<NW>::value);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) p[w] = key.w[w];
+    p[NW] = (uint64_t)count;  // count in the low 4 bytes, zero padding above
+}
+
+template <int NW>
+__global__ void __launch_bounds__(CountCfg<NW>::kThreads)
+k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ fine_off, CountArgs args)
+{
+    constexpr int T = CountCfg<NW>::kThreads;
+    constexpr uint32_t BW = CountCfg<NW>::kBloomWords;
+    constexpr uint32_t TS = CountCfg<NW>::kTableSlots;
+    constexpr uint32_t FB = CountCfg<NW>::kFirstBuf;
+    __shared__ unsigned long long s_bloom[BW];
+    __shared__ unsigned long long s_tkey[NW * TS];
+    __shared__ uint32_t s_tcnt[TS];
+    __shared__ Key<NW> s_first[FB];
+    __shared__ uint32_t s_nfirst;
+    __shared__ unsigned long long s_fbase;
+    __shared__ uint32_t s_rec;
+    __shared__ unsigned long long s_rbase;
+    __shared__ uint32_t s_distinct;
+
+    const int t = threadIdx.x;
+    const uint32_t f = blockIdx.x;
+    const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
+    for (uint32_t i = t; i < BW; i += T) s_bloom[i] = 0ULL;
+    for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
+    for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
+    if (t == 0) { s_nfirst = 0; s_rec = 0; s_distinct = 0; }
+    __syncthreads();
+    if (t == 0 && hi > lo) atomicMax(args.max_bucket, (unsigned long long)(hi - lo));
+
+    const unsigned long long n = hi - lo;
+    const unsigned long long iters = (n + T - 1) / T;
+    for (unsigned long long it = 0; it < iters; ++it) {
+        const unsigned long long i = lo + it * T + t;
+        const bool active = i < hi;
+        Key<NW> key;
+        bool first = false;
+        if (active) {
+            key = load_key<NW>(B, i);
+            uint64_t hl, hh;
+            xxh3_128(key, hl, hh);
+            const uint32_t word = (uint32_t)(hl & (BW - 1));
+            const uint64_t mask = bloom_mask(hl, hh);
+            // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
+            unsigned long long old = s_bloom[word];
+            if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
+            first = (old & mask) != mask;
+            if (!first) {
+                bool claimed;
+                const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
+                                                         (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
+                if (s >= 0) {
+                    atomicAdd(&s_tcnt[s], 1u);
+                } else {
+                    // spill to the global table
+                    const uint64_t gs = (hh >> 32) ^ (hl >> 17);
+                    const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap,
+                                                             args.spill_cap - 1, gs, (uint32_t)args.spill_cap,
+                                                             key, claimed);
+                    if (g >= 0) {
+                        atomicAdd(&args.spill_cnt[g], 1u);
+                        if (claimed) atomicAdd(args.spilled, 1ULL);
+                    } else {
+                        atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+                    }
+                }
+            }
+        }
+        // append firsts through the LDS buffer
+        const uint64_t bal = __ballot(first);
+        uint32_t wbase = 0;
+        if ((threadIdx.x & (kWave - 1)) == 0 && bal) wbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bal));
+        wbase = __shfl(wbase, 0, kWave);
+        if (first) s_first[wbase + lanes_below(bal)] = key;
+        __syncthreads();
+        const uint32_t nfb = s_nfirst;
+        if (nfb + T > FB || it + 1 == iters) {
+            if (t == 0 && nfb) s_fbase = atomicAdd(args.first_count, (unsigned long long)nfb);
+            __syncthreads();
+            for (uint32_t j = t; j < nfb; j += T) store_key<NW>(args.firsts, s_fbase + j, s_first[j]);
+            __syncthreads();
+            if (t == 0) s_nfirst = 0;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+
+    // root records (append_export_record filter on the repeat count,
+    // src/NewKmerTree.h:1159-1168)
+    uint32_t mine = 0, distinct = 0;
+    for (uint32_t s = t; s < TS; s += T) {
+        const uint32_t cnt = s_tcnt[s];
+        if (cnt) {
+            distinct++;
+            if (cnt >= args.min_count && cnt <= args.max_count) mine++;
+        }
+    }
+    if (distinct) atomicAdd(&s_distinct, distinct);
+    const uint32_t mbase = mine ? atomicAdd(&s_rec, mine) : 0u;
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t nr = s_rec;
+        s_rbase = nr ? atomicAdd(args.rec_count, (unsigned long long)nr) : 0ULL;
+        if (s_distinct) atomicAdd(args.distinct_rep, (unsigned long long)s_distinct);
+    }
+    __syncthreads();
+    uint64_t pos = s_rbase + mbase;
+    for (uint32_t s = t; s < TS; s += T) {
+        const uint32_t cnt = s_tcnt[s];
+        if (cnt && cnt >= args.min_count && cnt <= args.max_count) {
+            if (pos < args.rec_cap) {
+                Key<NW> key;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
+                write_record<NW>(args.recs, pos, key, cnt);
+            } else {
+                atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
+            }
+            pos++;
+        }
+    }
+}
+
+// P3b: records of spilled keys (slots with a claimed final word)
+template <int NW>
+__global__ void __launch_bounds__(256)
+k_spill_emit(CountArgs args)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    uint32_t cnt = 0;
+    Key<NW> key;
+    if (s < args.spill_cap && *args.spilled) {
+        cnt = args.spill_cnt[s];
+        if (cnt) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) key.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
+            keep = cnt >= args.min_count && cnt <= args.max_count;
+            atomicAdd(args.distinct_rep, 1ULL);
+        }
+    }
+    const uint64_t bal = __ballot(keep);
+    unsigned long long wbase = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0 && bal) wbase = atomicAdd(args.rec_count, (unsigned long long)__popcll(bal));
+    wbase = __shfl(wbase, 0, kWave);
+    if (keep) {
+        const uint64_t pos = wbase + lanes_below(bal);
+        if (pos < args.rec_cap) write_record<NW>(args.recs, pos, key, cnt);
+        else atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
+    }
+}
+
+// ------------------------------------------------------------------ P4
+
+// Records grouped by root id (word 0 >> 56).  Grid-stride over tiles of
+// kRecTile records; count pass adds per-root totals, scatter pass reserves a
+// run per (tile, root) and writes it from LDS.
+constexpr int kRecThreads = 256;
+constexpr int kRecPer = 8;
+constexpr int kRecTile = kRecThreads * kRecPer;
+
+template <int NW, bool SCATTER>
+__global__ void __launch_bounds__(kRecThreads)
+k_rec_group(const uint8_t* __restrict__ recs, const unsigned long long* __restrict__ n_ptr, uint64_t cap,
+            unsigned long long* __restrict__ root_counts, unsigned long long* __restrict__ root_cursor,
+            uint8_t* __restrict__ out)
+{
+    constexpr uint32_t RW = RecBytes<NW>::value / 8;  // u64 words per record
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_off[256];
+    __shared__ unsigned long long s_gbase[256];
+    __shared__ uint32_t s_scan[kRecThreads / kWave + 1];
+    __shared__ uint64_t s_rec[SCATTER ? kRecTile * RW : 1];
+    __shared__ uint8_t s_bkt[SCATTER ? kRecTile : 1];
+
+    const int t = threadIdx.x;
+    const uint64_t n = *n_ptr < cap ? *n_ptr : cap;
+    const uint64_t ntiles = (n + kRecTile - 1) / kRecTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        s_hist[t] = 0;
+        __syncthreads();
+        const uint64_t base = tile * kRecTile;
+        uint64_t r[kRecPer][RW];
+        uint32_t root[kRecPer], rank[kRecPer];
+#pragma unroll
+        for (int j = 0; j < kRecPer; ++j) {
+            const uint64_t i = base + t + (uint64_t)j * kRecThreads;
+            if (i < n) {
+                const uint64_t* src = reinterpret_cast<const uint64_t*>(recs + i * (RW * 8));
+#pragma unroll
+                for (uint32_t w = 0; w < RW; ++w) r[j][w] = src[w];
+                root[j] = (uint32_t)(r[j][0] >> 56);
+                rank[j] = atomicAdd(&s_hist[root[j]], 1u);
+            }
+        }
+        __syncthreads();
+        if constexpr (!SCATTER) {
+            const uint32_t h = s_hist[t];
+            if (h) atomicAdd(&root_counts[t], (unsigned long long)h);
+            __syncthreads();
+        } else {
+            const uint32_t h = s_hist[t];
+            uint32_t total;
+            const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
+            s_off[t] = off;
+            if (h) s_gbase[t] = atomicAdd(&root_cursor[t], (unsigned long long)h);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kRecPer; ++j) {
+                const uint64_t i = base + t + (uint64_t)j * kRecThreads;
+                if (i < n) {
+                    const uint32_t slot = s_off[root[j]] + rank[j];
+#pragma unroll
+                    for (uint32_t w = 0; w < RW; ++w) s_rec[slot * RW + w] = r[j][w];
+                    s_bkt[slot] = (uint8_t)root[j];
+                }
+            }
+            __syncthreads();
+            for (uint32_t slot = t; slot < total; slot += kRecThreads) {
+                const uint32_t b = s_bkt[slot];
+                const uint64_t dst = s_gbase[b] + (slot - s_off[b]);
+                uint64_t* d = reinterpret_cast<uint64_t*>(out + dst * (RW * 8));
+#pragma unroll
+                for (uint32_t w = 0; w < RW; ++w) d[w] = s_rec[slot * RW + w];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_rec_scan(const unsigned long long* __restrict__ root_counts, unsigned long long* __restrict__ root_off,
+           unsigned long long* __restrict__ root_cursor)
+{
+    __shared__ unsigned long long s_scan[5];
+    unsigned long long all;
+    const unsigned long long c = root_counts[threadIdx.x];
+    const unsigned long long off = block_exclusive_scan<unsigned long long>(c, s_scan, &all);
+    root_off[threadIdx.x] = off;
+    root_cursor[threadIdx.x] = off;
+    if (threadIdx.x == 255) root_off[256] = all;
+}
+
+// ------------------------------------------------------------------ debug hooks
+
+template <int NW>
+__global__ void k_debug_hash(const uint64_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Key<NW> key = load_key<NW>(keys, i);
+    uint64_t lo, hi;
+    xxh3_128(key, lo, hi);
+    out[2 * i] = lo;
+    out[2 * i + 1] = hi;
+}
+
+// ------------------------------------------------------------------ launchers
+
+#define KMC_LAUNCH_CHECK() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return e_; } while (0)
+
+template <int NW>
+hipError_t launch_extract(bool scatter, const uint8_t* seq, uint64_t padded_len, uint32_t k, const Route& rt,
+                          unsigned long long* counts, unsigned long long* cursor, uint64_t* out, hipStream_t st)
+{
+    constexpr int T = ExtractCfg<NW>::kThreads;
+    const uint64_t tiles = padded_len / (T * 16);
+    if (tiles == 0) return hipSuccess;
+    if (scatter)
+        hipLaunchKernelGGL((k_extract<NW, true>), dim3((uint32_t)tiles), dim3(T), 0, st, seq, padded_len, k, rt, counts, cursor, out);
+    else
+        hipLaunchKernelGGL((k_extract<NW, false>), dim3((uint32_t)tiles), dim3(T), 0, st, seq, padded_len, k, rt, counts, cursor, out);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_scan_coarse(const unsigned long long* counts, unsigned long long* coarse_off, unsigned long long* cursor,
+                              unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan_coarse, dim3(1), dim3(256), 0, st, counts, coarse_off, cursor, tile_start, sub_tile);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_scan_fine(const unsigned long long* counts, uint32_t nf, unsigned long long* fine_off,
+                            unsigned long long* cursor, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan_fine, dim3(1), dim3(1024), 0, st, counts, nf, fine_off, cursor);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NW>
+hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const unsigned long long* coarse_off,
+                      const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,
+                      unsigned long long* cursor, uint64_t* B, hipStream_t st)
+{
+    constexpr int T = SubCfg<NW>::kThreads;
+    if (max_tiles == 0) return hipSuccess;
+    if (scatter)
+        hipLaunchKernelGGL((k_sub<NW, true>), dim3((uint32_t)max_tiles), dim3(T), 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+    else
+        hipLaunchKernelGGL((k_sub<NW, false>), dim3((uint32_t)max_tiles), dim3(T), 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NW>
+hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_count<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
+    KMC_LAUNCH_CHECK();
+    const uint64_t blocks = (args.spill_cap + 255) / 256;
+    hipLaunchKernelGGL((k_spill_emit<NW>), dim3((uint32_t)blocks), dim3(256), 0, st, args);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NW>
+hipError_t launch_group_records(bool scatter, const uint8_t* recs, const unsigned long long* n_ptr, uint64_t cap,
+                                unsigned long long* root_counts, unsigned long long* root_cursor, uint8_t* out,
+                                hipStream_t st)
+{
+    if (scatter)
+        hipLaunchKernelGGL((k_rec_group<NW, true>), dim3(2048), dim3(kRecThreads), 0, st, recs, n_ptr, cap, root_counts, root_cursor, out);
+    else
+        hipLaunchKernelGGL((k_rec_group<NW, false>), dim3(2048), dim3(kRecThreads), 0, st, recs, n_ptr, cap, root_counts, root_cursor, out);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_rec_scan(const unsigned long long* root_counts, unsigned long long* root_off,
+                           unsigned long long* root_cursor, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(256), 0, st, root_counts, root_off, root_cursor);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NW>
+hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_debug_hash<NW>), dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, keys, n, out);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template hipError_t launch_extract<1>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
+template hipError_t launch_extract<2>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
+template hipError_t launch_sub<1>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
+template hipError_t launch_sub<2>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
+template hipError_t launch_count<1>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<2>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_group_records<1>(bool, const uint8_t*, const unsigned long long*, uint64_t, unsigned long long*, unsigned long long*, uint8_t*, hipStream_t);
+template hipError_t launch_group_records<2>(bool, const uint8_t*, const unsigned long long*, uint64_t, unsigned long long*, unsigned long long*, uint8_t*, hipStream_t);
+template hipError_t launch_debug_hash<1>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
+template hipError_t launch_debug_hash<2>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
+
+}  // namespace kmc

@@ -1,0 +1,156 @@
+"""Write the golden fixtures of tests/golden/ from the reference binaries.
+
+Run in the build container only (it needs oracle/_ref, compiled from the
+reference sources by oracle/Makefile):
+
+    python tests/golden/make_golden.py
+
+For each case the deterministic generator writes a FASTQ; the reference `Tree`
+counts it (k, 8 threads) and `histogram_tool precise` builds the histogram.
+The merged table (count = [key in low.bin] + root record count) is read back
+from the reference's own output files.  Stored per case: K (total k-mers, from
+Tree's "Total read k-mer count"), D (distinct keys), the SHA-256 of the merged
+table sorted by key (per entry: N little-endian u64 key words, then the count as
+little-endian u32), sum of counts, and the nonzero histogram lines.  The
+reference can drop input on small files (a consumer-drain race), so a run is
+only kept when its k-mer count equals the closed-form K of the input.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+import kmertreecounter_amd as K  # noqa: E402
+
+REF = ROOT / "oracle" / "_ref"
+OUT = Path(__file__).resolve().parent / "reference_runs.json"
+
+# (name, workload, scale, seed, k, max_freq)
+CASES = [
+    ("c1_s10_k31", 1, 10.0, 11, 31, 10000),
+    ("c1_s10_k17", 1, 10.0, 12, 17, 10000),
+    ("c1_s10_k32", 1, 10.0, 13, 32, 10000),
+    ("c1_s10_k33", 1, 10.0, 14, 33, 10000),
+    ("c1_s10_k63", 1, 10.0, 15, 63, 10000),
+    ("c1_full_k31", 1, 1.0, 1, 31, 10000),
+    ("c2_s200_k31", 2, 200.0, 2, 31, 10000),
+    ("c3_s200_k31", 3, 200.0, 3, 31, 100000),
+    ("c4_s1000_k31", 4, 1000.0, 4, 31, 10000),
+    ("c5_s2000_k63", 5, 2000.0, 5, 63, 100000),
+]
+
+
+def oracle():
+    L = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+    u64, vp, u32 = ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint32
+    L.orc_count_kmers.restype = u64
+    L.orc_count_kmers.argtypes = [ctypes.c_char_p, u64, u32]
+    L.orc_merge_outputs.restype = u64
+    L.orc_merge_outputs.argtypes = [vp, u64, vp, u64, u32, vp, vp]
+    L.orc_unpack_low.argtypes = [vp, u64, u32, u32, vp]
+    L.orc_packed_bytes.restype = u64
+    L.orc_packed_bytes.argtypes = [u32]
+    return L
+
+
+def table_sha(keys: np.ndarray, counts: np.ndarray) -> str:
+    nw = keys.shape[1]
+    dt = np.dtype([("k", "<u8", (nw,)), ("c", "<u4")], align=False)
+    rec = np.zeros(keys.shape[0], dtype=dt)
+    rec["k"] = keys
+    rec["c"] = counts
+    return hashlib.sha256(rec.tobytes()).hexdigest()
+
+
+def read_reference_table(tmp: Path, k: int, O) -> tuple[np.ndarray, np.ndarray]:
+    nw = K.words_for_k(k)
+    pb = int(O.orc_packed_bytes(k))
+    low = (tmp / "low.bin").read_bytes()
+    nl = len(low) // pb
+    lowk = np.zeros(nl * nw, dtype=np.uint64)
+    O.orc_unpack_low(low, nl, k, nw, lowk.ctypes.data)
+    recs = b"".join((tmp / f"root_{r}.bin").read_bytes() for r in range(256) if (tmp / f"root_{r}.bin").exists())
+    rb = 8 * nw + 8
+    nr = len(recs) // rb
+    keys = np.zeros((nl + nr) * nw, dtype=np.uint64)
+    counts = np.zeros(nl + nr, dtype=np.uint32)
+    d = O.orc_merge_outputs(lowk.ctypes.data, nl, recs, nr, nw, keys.ctypes.data, counts.ctypes.data)
+    if d == 2**64 - 1:
+        raise RuntimeError("reference outputs repeat a key")
+    return keys[: d * nw].reshape(d, nw), counts[:d]
+
+
+def run_case(name, cfg, scale, seed, k, max_freq, O) -> dict:
+    fq = K.generate(cfg, scale=scale, seed=seed)
+    seqstream = K.generate(cfg, scale=scale, seed=seed, fmt="seq")
+    expect_k = int(O.orc_count_kmers(seqstream, len(seqstream), k))
+    with tempfile.TemporaryDirectory(dir="/dev/shm" if os.path.isdir("/dev/shm") else None) as td:
+        td = Path(td)
+        (td / "in.fq").write_bytes(fq)
+        for attempt in range(6):
+            tmp = td / "tmp"
+            if tmp.exists():
+                for f in tmp.iterdir():
+                    f.unlink()
+            else:
+                tmp.mkdir()
+            r = subprocess.run([str(REF / "Tree"), str(td / "in.fq"), str(k), "8", "8"], cwd=td,
+                               capture_output=True, text=True, timeout=600)
+            m = re.search(r"Total read k-mer count: (\d+)", r.stdout)
+            got = int(m.group(1)) if m else -1
+            if r.returncode == 0 and got == expect_k:
+                break
+            print(f"  {name}: reference run {attempt} read {got} of {expect_k} k-mers; retrying", flush=True)
+        else:
+            raise RuntimeError(f"{name}: reference never read the whole input")
+        h = subprocess.run([str(REF / "histogram_tool"), "precise", str(tmp), str(k), "8", "8",
+                            str(td / "hist.tsv"), "1", str(max_freq)], capture_output=True, text=True, timeout=600)
+        if h.returncode != 0:
+            raise RuntimeError(f"{name}: histogram_tool failed: {h.stderr}")
+        hist = {}
+        for line in (td / "hist.tsv").read_text().splitlines():
+            f, n = line.split("\t")
+            if int(n):
+                hist[f] = int(n)
+        keys, counts = read_reference_table(tmp, k, O)
+        low_n = (tmp / "low.bin").stat().st_size // int(O.orc_packed_bytes(k))
+        rec_n = sum((tmp / f"root_{r}.bin").stat().st_size for r in range(256)) // (8 * K.words_for_k(k) + 8)
+    total = int(counts.sum(dtype=np.uint64))
+    assert total == expect_k, (name, total, expect_k)
+    return {
+        "workload": cfg, "scale": scale, "seed": seed, "k": k, "max_freq": max_freq,
+        "K": expect_k, "D": int(keys.shape[0]), "sha256": table_sha(keys, counts), "sum_counts": total,
+        "max_count": int(counts.max()) if counts.size else 0,
+        "histogram_nonzero": hist,
+        "reference_low_keys": int(low_n), "reference_root_records": int(rec_n),
+    }
+
+
+def main() -> None:
+    if not (REF / "Tree").exists():
+        sys.exit("oracle/_ref/Tree missing: run `make -C oracle` in the build container")
+    O = oracle()
+    only = set(sys.argv[1:])
+    data = json.loads(OUT.read_text()) if OUT.exists() else {}
+    for case in CASES:
+        if only and case[0] not in only:
+            continue
+        print(f"{case[0]} ...", flush=True)
+        data[case[0]] = run_case(*case, O)
+        print(f"  K={data[case[0]]['K']} D={data[case[0]]['D']}", flush=True)
+        OUT.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+"""GPU parity: the HIP engine against the oracle and against the reference's
+own outputs (tests/golden/reference_runs.json), through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import kmertreecounter_amd as K
+from tests import common as C  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def run_engine(seq_blocks, k, **kw):
+    with K.Engine(k, **kw) as e:
+        for b in seq_blocks:
+            e.push_block(b)
+        e.finish()
+        return e.firsts(), e.records(), e.stats()
+
+
+def as_rows(keys: np.ndarray) -> np.ndarray:
+    """(n, N) u64 keys -> 1-D structured array that sorts/searches lexicographically."""
+    nw = keys.shape[1]
+    dt = np.dtype([(f"w{i}", "<u8") for i in range(nw)])
+    return np.ascontiguousarray(keys).view(dt).ravel()
+
+
+def check_outputs(firsts, recs, stats, k, exp_keys, exp_counts):
+    nw = C.nwords(k)
+    assert stats.total_kmers == int(exp_counts.sum(dtype=np.uint64))
+    keys, counts = C.merge_outputs(firsts, recs, nw)
+    assert keys.shape == exp_keys.shape
+    np.testing.assert_array_equal(keys, exp_keys)
+    np.testing.assert_array_equal(counts, exp_counts)
+    assert stats.firsts == firsts.shape[0]
+    assert stats.records == len(recs)
+    if len(recs):
+        assert (recs["pad"] == 0).all()
+        assert (recs["count"] >= 1).all()
+        # Bloom split (src/FastqClassifier.h:241-349): a key reported first has its
+        # remaining c-1 occurrences in its record; a key whose first occurrence was a
+        # false positive is absent from the firsts and its record holds all c
+        rk = recs["key"].reshape(-1, nw)
+        exp_rows = as_rows(exp_keys)
+        idx = np.searchsorted(exp_rows, as_rows(rk))
+        c = exp_counts[idx]
+        frows = np.sort(as_rows(firsts.reshape(-1, nw).copy()))
+        pos = np.searchsorted(frows, as_rows(rk))
+        in_firsts = (pos < frows.size) & (frows[np.minimum(pos, max(frows.size - 1, 0))] == as_rows(rk)) if frows.size else np.zeros(len(rk), bool)
+        np.testing.assert_array_equal(recs["count"], np.where(in_firsts, c - 1, c))
+
+
+def test_device_xxh3_matches_oracle():
+    rng = np.random.default_rng(5)
+    O = C.oracle()
+    for nw in (1, 2):
+        keys = rng.integers(0, 2**63, size=(4096, nw), dtype=np.uint64) * 2 + 1
+        keys[:3] = 0
+        got = K.debug_xxh3(keys, nw)
+        exp = np.zeros((keys.shape[0], 2), dtype=np.uint64)
+        for i in range(keys.shape[0]):
+            lo, hi = C.ctypes.c_uint64(), C.ctypes.c_uint64()
+            kk = np.ascontiguousarray(keys[i])
+            O.orc_xxh3_128(kk.ctypes.data, nw, C.ctypes.byref(lo), C.ctypes.byref(hi))
+            exp[i] = (lo.value, hi.value)
+        np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("k", [31, 17, 32, 33, 63, 5])
+def test_small_random_stream(k):
+    rng = np.random.default_rng(k)
+    alphabet = np.frombuffer(b"ACGTacgtuUN\n", dtype=np.uint8)
+    p = np.array([.22, .22, .22, .22, .02, .02, .02, .02, .01, .01, .01, .01])
+    p /= p.sum()
+    seq = bytes(rng.choice(alphabet, size=300_000, p=p))
+    keys, counts = C.count_table(C.extract(seq, k))
+    f, r, s = run_engine([seq], k)
+    check_outputs(f, r, s, k, keys, counts)
+
+
+def test_empty_and_degenerate_inputs():
+    for blocks in ([], [b"\n"], [b"NNNNNNNNNN\n"], [b"ACGT\n"], [b"A" * 31 + b"\n"]):
+        f, r, s = run_engine(blocks, 31)
+        seq = b"".join(blocks)
+        exp = C.extract(seq, 31)
+        assert s.total_kmers == exp.shape[0]
+        if exp.shape[0]:
+            keys, counts = C.count_table(exp)
+            check_outputs(f, r, s, 31, keys, counts)
+
+
+@pytest.mark.parametrize("sub_bits", [0, 3, 8])
+def test_fanout_does_not_change_table(sub_bits):
+    seq = K.generate(1, scale=20, seed=3, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    f, r, s = run_engine([seq], 31, sub_bits=sub_bits)
+    assert s.fine_buckets == 256 << sub_bits
+    check_outputs(f, r, s, 31, keys, counts)
+
+
+def test_block_carry_path_matches():
+    # the host reader's 32 KiB blocks with k-1 carries count exactly like whole lines
+    fq = K.generate(4, scale=2000, seed=9)
+    blocks = K.fastq_blocks(fq, 31, 4096)
+    seq = K.generate(4, scale=2000, seed=9, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    f, r, s = run_engine(blocks, 31)
+    check_outputs(f, r, s, 31, keys, counts)
+
+
+def test_spill_table_path():
+    # sub_bits=0 leaves ~190k keys per fine bucket, more distinct repeats than the
+    # LDS table holds: the global spill table must keep the counts exact
+    seq = K.generate(1, scale=1, seed=1, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    f, r, s = run_engine([seq], 31, sub_bits=0)
+    assert s.spilled_keys > 0
+    check_outputs(f, r, s, 31, keys, counts)
+
+
+@pytest.mark.parametrize("name", sorted(C.reference_runs().keys()))
+def test_reference_fixture(name):
+    fx = C.reference_runs()[name]
+    k = fx["k"]
+    fq = K.generate(fx["workload"], scale=fx["scale"], seed=fx["seed"])
+    blocks = K.fastq_blocks(fq, k)
+    f, r, s = run_engine(blocks, k)
+    assert s.total_kmers == fx["K"]
+    keys, counts = C.merge_outputs(f, r, C.nwords(k))
+    assert keys.shape[0] == fx["D"]
+    assert C.table_sha(keys, counts) == fx["sha256"]
+    h = C.histogram(counts, 1, fx["max_freq"])
+    got = {str(i + 1): int(v) for i, v in enumerate(h) if v}
+    assert got == fx["histogram_nonzero"]
+
+
+def test_min_max_filter():
+    seq = K.generate(1, scale=20, seed=4, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    with K.Engine(31, min_count=3, max_count=9) as e:
+        e.push_block(seq)
+        e.finish()
+        f = e.firsts()
+        r = e.records()
+        s = e.stats()
+    # records obey the filter on the repeat count (src/NewKmerTree.h:1159-1168)
+    assert ((r["count"] >= 3) & (r["count"] <= 9)).all()
+    # unfiltered, each repeated key would have a record; the filter keeps a subset
+    assert s.records <= s.records_unfiltered
+    assert s.total_kmers == int(counts.sum())
+
+
+def test_root_files_hold_their_prefix():
+    seq = K.generate(2, scale=400, seed=6, fmt="seq")
+    with K.Engine(31) as e:
+        e.push_block(seq)
+        e.finish()
+        for rid in (0, 7, 128, 255):
+            recs = e.root(rid)
+            if len(recs):
+                assert (recs["key"][:, 0] >> np.uint64(56) == rid).all()
