@@ -96,6 +96,11 @@ EXPORTED = [
 
 _lib = None
 
+# bytes from a C buffer of any size (ctypes.string_at takes a C int length)
+_bytes_from = ctypes.pythonapi.PyBytes_FromStringAndSize
+_bytes_from.restype = ctypes.py_object
+_bytes_from.argtypes = [ctypes.c_void_p, ctypes.c_ssize_t]
+
 
 def lib() -> ctypes.CDLL:
     """Load libkmc.so (built in-tree by kmertreecounter_amd.build)."""
@@ -159,7 +164,7 @@ def generate(config: int, scale: float = 1.0, seed: int = 1, fmt: str = "fastq",
     if rc != 0:
         raise KmcError(f"kcs_generate failed: {rc}")
     try:
-        return ctypes.string_at(out, n.value)
+        return _bytes_from(out, n.value)
     finally:
         L.kcs_free(out)
 

@@ -68,7 +68,7 @@ struct kc_engine {
     uint64_t* d_bounds = nullptr;  // learned: 255 coarse + 256*255 fine boundaries
     bool bounds_valid = false;
 
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
 
     // results
     bool have_results = false;
@@ -268,28 +268,31 @@ int run_pipeline_t(kc_engine* h)
         }
         if (sbits) HIPX(hipMemsetAsync(h->d_fcounts, 0, sizeof(unsigned long long) * kXcds * nf, st));
 
+        // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]
         HIPX(hipEventRecord(h->ev[0], st));
         HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
         HIPX(hipEventRecord(h->ev[1], st));
         HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st));
-        HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
         HIPX(hipEventRecord(h->ev[2], st));
+        HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
+        HIPX(hipEventRecord(h->ev[3], st));
         const uint64_t* src = h->d_A;
         uint64_t* firsts = h->d_B;
         const unsigned long long* fine_off = coarse_off;
         if (sbits) {
             const uint64_t max_tiles = padded / sub_tile(NW) + 256;
-            HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
-            HIPX(hipEventRecord(h->ev[3], st));
-            HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, st));
-            HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
             HIPX(hipEventRecord(h->ev[4], st));
+            HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
+            HIPX(hipEventRecord(h->ev[5], st));
+            HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, st));
+            HIPX(hipEventRecord(h->ev[6], st));
+            HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
+            HIPX(hipEventRecord(h->ev[7], st));
             src = h->d_B;
             firsts = h->d_A;
             fine_off = h->d_fine_off;
         } else {
-            HIPX(hipEventRecord(h->ev[3], st));
-            HIPX(hipEventRecord(h->ev[4], st));
+            for (int i = 4; i < 8; ++i) HIPX(hipEventRecord(h->ev[i], st));
         }
         CountArgs a{};
         a.firsts = firsts;
@@ -306,12 +309,14 @@ int run_pipeline_t(kc_engine* h)
         a.distinct_rep = h->d_ctr + C_DISTINCT;
         a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
         a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
+        HIPX(hipEventRecord(h->ev[8], st));
         HIPX(launch_count<NW>(nf, src, fine_off, a, st));
-        HIPX(hipEventRecord(h->ev[5], st));
+        HIPX(hipEventRecord(h->ev[9], st));
+        HIPX(launch_spill_emit<NW>(a, st));
         HIPX(launch_group_records<NW>(false, h->d_recs, h->d_ctr + C_REC, h->rec_cap, root_counts, nullptr, nullptr, st));
         HIPX(launch_rec_scan(root_counts, root_off, root_cursor, st));
         HIPX(launch_group_records<NW>(true, h->d_recs, h->d_ctr + C_REC, h->rec_cap, root_counts, root_cursor, h->d_recs_sorted, st));
-        HIPX(hipEventRecord(h->ev[6], st));
+        HIPX(hipEventRecord(h->ev[10], st));
         HIPX(hipStreamSynchronize(st));
 
         unsigned long long ctr[C_NCTR];
@@ -349,12 +354,14 @@ int run_pipeline_t(kc_engine* h)
         h->res_firsts = firsts;
         h->root_off.assign(257, 0);
         HIPX(hipMemcpy(h->root_off.data(), root_off, sizeof(unsigned long long) * 257, hipMemcpyDeviceToHost));
+        // P0, P1, P2a, P2b, P3 kernels; P4 = record grouping (three kernels); total
+        const int pairs[6][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {9, 10}};
         float ms = 0;
         for (int p = 0; p < 6; ++p) {
-            HIPX(hipEventElapsedTime(&ms, h->ev[p], h->ev[p + 1]));
+            HIPX(hipEventElapsedTime(&ms, h->ev[pairs[p][0]], h->ev[pairs[p][1]]));
             h->phase_ms[p] = ms;
         }
-        HIPX(hipEventElapsedTime(&ms, h->ev[0], h->ev[6]));
+        HIPX(hipEventElapsedTime(&ms, h->ev[0], h->ev[10]));
         h->phase_ms[6] = ms;
         h->have_results = true;
         return KC_OK;

@@ -75,6 +75,8 @@ template <int NW>
 hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
                         hipStream_t st);
 template <int NW>
+hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st);
+template <int NW>
 hipError_t launch_group_records(bool scatter, const uint8_t* recs, const unsigned long long* n_ptr, uint64_t cap,
                                 unsigned long long* root_counts, unsigned long long* root_cursor, uint8_t* out,
                                 hipStream_t st);

@@ -759,6 +759,12 @@ hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long
 {
     hipLaunchKernelGGL((k_count<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
     KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NW>
+hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st)
+{
     const uint64_t blocks = (args.spill_cap + 255) / 256;
     hipLaunchKernelGGL((k_spill_emit<NW>), dim3((uint32_t)blocks), dim3(256), 0, st, args);
     KMC_LAUNCH_CHECK();
@@ -801,6 +807,8 @@ template hipError_t launch_sub<1>(bool, uint64_t, const uint64_t*, const unsigne
 template hipError_t launch_sub<2>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_count<1>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_count<2>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
+template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
 template hipError_t launch_group_records<1>(bool, const uint8_t*, const unsigned long long*, uint64_t, unsigned long long*, unsigned long long*, uint8_t*, hipStream_t);
 template hipError_t launch_group_records<2>(bool, const uint8_t*, const unsigned long long*, uint64_t, unsigned long long*, unsigned long long*, uint8_t*, hipStream_t);
 template hipError_t launch_debug_hash<1>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);

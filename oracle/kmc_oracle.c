@@ -21,7 +21,7 @@
  *                        Tree merged tables (tests/golden).
  *   orc_count_sorted   - Python sorted()/Counter on small inputs, sum == K.
  *   orc_xxh3_128       - reference xxhash.h v0.8.3 via oracle/_ref harness
- *                        (vectors committed in tests/golden/xxh3_vectors.json).
+ *                        (vectors committed in tests/golden/harness_vectors.json).
  *   orc_bloom_*        - no-false-negative and order-independence invariants,
  *                        reference ConcurrentBloomFilter via oracle/_ref harness.
  *   orc_pack_low/unpack- reference ExportReader KAT inputs (k=33, k=64),
@@ -639,7 +639,7 @@ static int cmp_rec(const void* a, const void* b)
 }
 
 ORC_API int orc_histogram_precise(const uint8_t* recs, uint64_t nr, const uint64_t* low, uint64_t nl,
-                                  uint32_t nw, uint32_t minf, uint32_t maxf, int64_t* hist)
+                                  uint32_t nw, uint32_t minf, uint32_t maxf, int64_t* hist, int query_only)
 {
     const uint64_t rb = orc_record_bytes(nw);
     const uint32_t tw = nw + 1;
@@ -653,7 +653,7 @@ ORC_API int orc_histogram_precise(const uint8_t* recs, uint64_t nr, const uint64
         uint32_t c;
         memcpy(&c, r + nw * 8, 4);
         if ((uint64_t)c + 1 < minf) continue;                 /* :161-165 */
-        if (c >= minf && c <= maxf) hist[c - minf] += 1;      /* :167-170 */
+        if (!query_only && c >= minf && c <= maxf) hist[c - minf] += 1;  /* :167-170 */
         memcpy(table + ins * tw, r, nw * 8);
         table[ins * tw + nw] = c;
         ins++;

@@ -137,11 +137,53 @@ def run_case(name, cfg, scale, seed, k, max_freq, O) -> dict:
     }
 
 
+def harness_vectors() -> dict:
+    """Input/output vectors of the reference's own GetKmer, XXH3_128bits and
+    BloomFilter code (oracle/_ref/libref_harness.so)."""
+    H = ctypes.CDLL(str(REF / "libref_harness.so"))
+    u64, vp, u32 = ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint32
+    H.ref_extract.restype = u64
+    H.ref_extract.argtypes = [ctypes.c_char_p, u64, u32, vp, u64]
+    H.ref_xxh3_128.argtypes = [vp, u64, vp, vp]
+    H.ref_bloom_classify.restype = u64
+    H.ref_bloom_classify.argtypes = [vp, u64, u32, u64, vp, vp]
+    rng = np.random.default_rng(2024)
+    out = {"xxh3": [], "extract": [], "bloom": []}
+    for nw in (1, 2):
+        for i in range(64):
+            key = rng.integers(0, 2**63, size=nw, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, size=nw, dtype=np.uint64)
+            if i < 2:
+                key[:] = np.uint64(0) if i == 0 else np.uint64(2**64 - 1)
+            lo, hi = u64(), u64()
+            H.ref_xxh3_128(key.ctypes.data, 8 * nw, ctypes.byref(lo), ctypes.byref(hi))
+            out["xxh3"].append({"key": [int(x) for x in key], "lo": lo.value, "hi": hi.value})
+    alphabet = np.frombuffer(b"ACGTacgtuUNn\n\rX", dtype=np.uint8)
+    for k in (1, 3, 15, 16, 17, 31, 32, 33, 47, 63, 64, 65, 128):
+        p = np.array([.22, .22, .22, .22, .02, .02, .02, .02, .005, .005, .01, .005, .005, .003, .002])
+        seq = bytes(rng.choice(alphabet, size=1500, p=p / p.sum()))
+        nw = 1 if k <= 32 else (2 if k <= 64 else 4)
+        buf = np.zeros(len(seq) * nw, dtype=np.uint64)
+        n = H.ref_extract(seq, len(seq), k, buf.ctypes.data, len(seq))
+        out["extract"].append({"k": k, "seq": seq.decode("latin-1"), "keys": [str(int(x)) for x in buf[: n * nw]]})
+    for nw in (1, 2):
+        keys = rng.integers(0, 2**40, size=800 * nw, dtype=np.uint64)
+        keys = np.concatenate([keys, keys[: 300 * nw]])
+        filt = np.zeros(256, dtype=np.uint64)
+        first = np.zeros(keys.size // nw, dtype=np.uint8)
+        H.ref_bloom_classify(keys.ctypes.data, keys.size // nw, nw, 256, filt.ctypes.data, first.ctypes.data)
+        out["bloom"].append({"nw": nw, "cap_words": 256, "seed": 2024, "keys": [str(int(x)) for x in keys],
+                             "filter": [str(int(x)) for x in filt], "first": first.tolist()})
+    return out
+
+
 def main() -> None:
     if not (REF / "Tree").exists():
         sys.exit("oracle/_ref/Tree missing: run `make -C oracle` in the build container")
     O = oracle()
     only = set(sys.argv[1:])
+    if not only or "vectors" in only:
+        (OUT.parent / "harness_vectors.json").write_text(json.dumps(harness_vectors()) + "\n")
+        print("harness vectors written", flush=True)
     data = json.loads(OUT.read_text()) if OUT.exists() else {}
     for case in CASES:
         if only and case[0] not in only:

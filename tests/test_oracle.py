@@ -1,0 +1,311 @@
+"""CPU tests that pin the oracle (oracle/kmc_oracle.c) before it is trusted:
+the reference's own known-answer tests, vectors produced by the reference's
+own code (tests/golden), closed forms and brute force on tiny inputs."""
+from __future__ import annotations
+
+import ctypes
+import json
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from tests import common as C
+
+O = C.oracle()
+
+
+def py_key(window: str, k: int) -> tuple:
+    """Independent restatement of the canonical key of one window (all bases
+    valid): min over pack(reverse(w)) and pack(complement(w)), MSB-first,
+    left aligned in N words (src/GetKmer.h:58-82, src/kmer.h:12-21)."""
+    code = {"A": 0, "C": 1, "G": 2, "T": 3, "U": 3}
+    w = [code[c] for c in window.upper()]
+    nw = C.nwords(k)
+
+    def pack(bases):
+        v = 0
+        for b in bases:
+            v = (v << 2) | b
+        v <<= 64 * nw - 2 * k
+        return tuple((v >> (64 * (nw - 1 - i))) & (2**64 - 1) for i in range(nw))
+
+    return min(pack(w[::-1]), pack([3 - b for b in w]))
+
+
+def brute_keys(seq: str, k: int) -> list:
+    out, run = [], 0
+    for i, c in enumerate(seq):
+        if c.upper() in "ACGTU":
+            run += 1
+            if run >= k:
+                out.append(py_key(seq[i - k + 1: i + 1], k))
+        else:
+            run = 0
+    return out
+
+
+# ------------------------------------------------------------------ extraction
+
+@pytest.mark.parametrize("k", [1, 2, 7, 16, 31, 32, 33, 45, 63, 64, 65, 100])
+def test_extract_matches_brute_force(k):
+    rng = np.random.default_rng(k)
+    seq = "".join(rng.choice(list("ACGTacgtuUN\nX"), size=600,
+                             p=[.2, .2, .2, .2, .03, .03, .03, .03, .02, .02, .02, .01, .01]))
+    got = [tuple(int(x) for x in row) for row in C.extract(seq.encode(), k)]
+    assert got == brute_keys(seq, k)
+
+
+def test_extract_closed_form_count_and_invariants():
+    rng = np.random.default_rng(7)
+    seq = bytes(rng.choice(np.frombuffer(b"ACGTN\n", np.uint8), size=200_000, p=[.24, .24, .24, .24, .02, .02]))
+    for k in (21, 31, 63):
+        keys = C.extract(seq, k)
+        # K = sum over runs of valid bases of max(0, r - k + 1)
+        runs = [len(r) for r in seq.replace(b"N", b"\n").split(b"\n")]
+        assert keys.shape[0] == sum(max(0, r - k + 1) for r in runs) == O.orc_count_kmers(seq, len(seq), k)
+        # the unused low bits of the last word are zero
+        used = 2 * k - 64 * (C.nwords(k) - 1)
+        if used < 64:
+            assert (keys[:, -1] & np.uint64((1 << (64 - used)) - 1) == 0).all()
+    # strand symmetry: a window and its reverse complement share a key
+    comp = bytes.maketrans(b"ACGT", b"TGCA")
+    s = bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), size=5000))
+    rc = s.translate(comp)[::-1]
+    for k in (11, 31, 63):
+        a = C.extract(s, k)
+        b = C.extract(rc, k)[::-1]
+        np.testing.assert_array_equal(a, b)
+
+
+def test_extract_matches_reference_vectors():
+    vec = json.loads((C.GOLDEN / "harness_vectors.json").read_text())
+    for case in vec["extract"]:
+        k = case["k"]
+        got = C.extract(case["seq"].encode("latin-1"), k).ravel()
+        np.testing.assert_array_equal(got, np.array([int(x) for x in case["keys"]], dtype=np.uint64))
+
+
+def test_extract_matches_reference_harness_live():
+    H = C.ref_harness()
+    if H is None:
+        pytest.skip("oracle/_ref not built here (reference sources absent)")
+    rng = np.random.default_rng(3)
+    seq = bytes(rng.choice(np.frombuffer(b"ACGTacgtuUN\n\r", np.uint8), size=100_000))
+    for k in (3, 31, 32, 33, 63, 64, 127):
+        nw = C.nwords(k)
+        buf = np.zeros(len(seq) * nw, dtype=np.uint64)
+        n = H.ref_extract(seq, len(seq), k, buf.ctypes.data, len(seq))
+        np.testing.assert_array_equal(C.extract(seq, k).ravel(), buf[: n * nw])
+
+
+# ------------------------------------------------------------------ counting
+
+def test_count_sorted_matches_counter():
+    rng = np.random.default_rng(11)
+    for nw in (1, 2):
+        keys = rng.integers(0, 50, size=(5000, nw), dtype=np.uint64) * np.uint64(2**40)
+        k, c = C.count_table(keys)
+        exp = Counter(map(tuple, keys.tolist()))
+        assert [tuple(r) for r in k.tolist()] == sorted(exp)
+        assert c.tolist() == [exp[key] for key in sorted(exp)]
+        assert int(c.sum()) == 5000
+
+
+# ------------------------------------------------------------------ xxh3 / Bloom
+
+def test_xxh3_matches_reference_vectors():
+    vec = json.loads((C.GOLDEN / "harness_vectors.json").read_text())
+    for e in vec["xxh3"]:
+        key = np.array(e["key"], dtype=np.uint64)
+        lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+        assert O.orc_xxh3_128(key.ctypes.data, key.size, ctypes.byref(lo), ctypes.byref(hi)) == 0
+        assert (lo.value, hi.value) == (e["lo"], e["hi"])
+
+
+def test_bloom_matches_reference_vectors():
+    vec = json.loads((C.GOLDEN / "harness_vectors.json").read_text())
+    for e in vec["bloom"]:
+        nw, cap = e["nw"], e["cap_words"]
+        keys = np.array([int(x) for x in e["keys"]], dtype=np.uint64)
+        n = keys.size // nw
+        filt = np.zeros(cap, dtype=np.uint64)
+        first = np.zeros(n, dtype=np.uint8)
+        O.orc_bloom_classify(keys.ctypes.data, n, nw, cap, filt.ctypes.data, first.ctypes.data)
+        np.testing.assert_array_equal(filt, np.array([int(x) for x in e["filter"]], dtype=np.uint64))
+        np.testing.assert_array_equal(first, np.array(e["first"], dtype=np.uint8))
+
+
+def test_bloom_invariants():
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 2**62, size=(20000, 1), dtype=np.uint64)
+    seq = np.concatenate([keys, keys[:5000], keys[:100]])
+    cap = 4096
+    f1 = np.zeros(cap, dtype=np.uint64)
+    first = np.zeros(seq.shape[0], dtype=np.uint8)
+    nf = O.orc_bloom_classify(seq.ctypes.data, seq.shape[0], 1, cap, f1.ctypes.data, first.ctypes.data)
+    # no false negatives: a repeated key is never first again
+    assert not first[20000:].any()
+    # firsts never exceed distinct keys
+    assert nf <= 20000
+    # the final filter is the OR of all masks: independent of insertion order
+    f2 = np.zeros(cap, dtype=np.uint64)
+    perm = np.ascontiguousarray(seq[rng.permutation(seq.shape[0])])
+    O.orc_bloom_classify(perm.ctypes.data, perm.shape[0], 1, cap, f2.ctypes.data, None)
+    np.testing.assert_array_equal(f1, f2)
+    # each mask has between one and three bits
+    for i in range(50):
+        w, m = ctypes.c_uint64(), ctypes.c_uint64()
+        kk = np.ascontiguousarray(keys[i])
+        O.orc_bloom_probe(kk.ctypes.data, 1, cap, ctypes.byref(w), ctypes.byref(m))
+        assert 1 <= bin(m.value).count("1") <= 3 and w.value < cap
+
+
+# ------------------------------------------------------------------ low.bin
+
+def masked(key: list[int], k: int) -> list[int]:
+    """Key as the reader returns it: bits beyond 2k cleared (writer mask)."""
+    nw = C.nwords(k)
+    full, tail_bits = k // 32, 2 * (k % 32)
+    out = list(key)
+    if tail_bits:
+        out[full] &= ((1 << tail_bits) - 1) << (64 - tail_bits)
+    for i in range(full + (1 if tail_bits else 0), nw):
+        out[i] = 0
+    return out
+
+
+@pytest.mark.parametrize("k,keys", [
+    # tests/test_export_reader.cpp:175-181 (k=33) and :241-246 (k=64)
+    (33, [[0x0123456789abcdef, 0xc0000000000000ff], [0xfedcba9876543210, 0x8000000000001234],
+          [0x1111222233334444, 0x40000000ffffffff]]),
+    (64, [[0x0123456789abcdef, 0xfedcba9876543210], [0xaaaaaaaa55555555, 0x123456789abcdef0]]),
+    (33, []),
+])
+def test_low_bin_pack_round_trip(k, keys):
+    nw = C.nwords(k)
+    pb = O.orc_packed_bytes(k)
+    arr = np.array(keys, dtype=np.uint64).reshape(-1, nw)
+    buf = (ctypes.c_uint8 * max(1, pb * len(keys)))()
+    O.orc_pack_low(arr.ctypes.data, len(keys), k, nw, buf)
+    assert pb == {33: 9, 64: 16}[k]
+    back = np.zeros(len(keys) * nw, dtype=np.uint64)
+    O.orc_unpack_low(buf, len(keys), k, nw, back.ctypes.data)
+    assert back.reshape(-1, nw).tolist() == [masked(x, k) for x in keys]
+    if keys and k == 33:
+        # the 9th byte of a k=33 record is the top byte of word 1 (src/ExportWriter.h:538-543)
+        assert bytes(buf)[8] == keys[0][1] >> 56
+
+
+def test_low_bin_cross_buffer_case():
+    # tests/test_export_reader.cpp:318-330: 60000 k=33 keys
+    k, n = 33, 60000
+    i = np.arange(n, dtype=np.uint64)
+    w0 = np.uint64(0x0123456789abcdef) ^ (i * np.uint64(0x9e3779b97f4a7c15))
+    w1 = np.uint64(0xc000000000000000) | (i & np.uint64(0x00ffffffffffffff))
+    arr = np.ascontiguousarray(np.stack([w0, w1], axis=1))
+    buf = (ctypes.c_uint8 * (O.orc_packed_bytes(k) * n))()
+    O.orc_pack_low(arr.ctypes.data, n, k, 2, buf)
+    back = np.zeros(n * 2, dtype=np.uint64).reshape(n, 2)
+    O.orc_unpack_low(buf, n, k, 2, back.ctypes.data)
+    np.testing.assert_array_equal(back[:, 0], w0)
+    np.testing.assert_array_equal(back[:, 1], w1 & np.uint64(0xC000000000000000))
+
+
+# ------------------------------------------------------------------ histogram
+
+def records(nw, items):
+    dt = np.dtype([("key", "<u8", (nw,)), ("count", "<u4"), ("pad", "<u4")])
+    r = np.zeros(len(items), dtype=dt)
+    for i, (key, c) in enumerate(items):
+        r["key"][i] = key
+        r["count"][i] = c
+    return r
+
+
+def precise(recs, low, nw, minf, maxf, query_only=0):
+    low = np.ascontiguousarray(np.array(low, dtype=np.uint64).reshape(-1, nw))
+    h = np.zeros(maxf - minf + 1, dtype=np.int64)
+    rb = recs.view(np.uint8) if len(recs) else np.zeros(1, np.uint8)
+    O.orc_histogram_precise(rb.ctypes.data, len(recs), low.ctypes.data, low.shape[0], nw, minf, maxf,
+                            h.ctypes.data, query_only)
+    return h.tolist()
+
+
+def test_histogram_reference_kats():
+    a = [0x0123456789abcdef, 0xc000000000000000]
+    b = [0x1111111111111111, 0x8000000000000000]
+    c = [0x2222222222222222, 0x4000000000000000]
+    e = [0x3333333333333333, 0x0000000000000000]
+    # tests/test_histogram_tool.cpp:211-235
+    assert precise(records(2, [(a, 2), (c, 1), (e, 4)]), [a, b, c, e], 2, 1, 4) == [1, 1, 1, 0]
+    g = [0x4444444444444444, 0xc000000000000000]
+    h = [0x5555555555555555, 0x8000000000000000]
+    i = [0x6666666666666666, 0x4000000000000000]
+    j = [0x7777777777777777, 0x0000000000000000]
+    # :237-260 (min filter 3..5)
+    assert precise(records(2, [(g, 1), (h, 2), (i, 5)]), [g, h, i, j], 2, 3, 5) == [1, 0, 0]
+    # :262-303 (N=1 k=32 and N=4 k=65)
+    assert precise(records(1, [([0xaaaaaaaa55555555], 1)]), [[0xaaaaaaaa55555555]], 1, 1, 3) == [0, 1, 0]
+    k4 = [0xaaaaaaaa55555555, 0x0123456789abcdef, 0xc000000000000000, 0]
+    assert precise(records(4, [(k4, 1)]), [k4], 4, 1, 3) == [0, 1, 0]
+    # tests/test_low_frequency_query_thread_pool.cpp:86-139 (query phase only)
+    two = [0x0123456789abcdef, 0xc000000000000000]
+    four = [0x1111222233334444, 0x8000000000000000]
+    miss = [0x9999aaaabbbbcccc, 0x4000000000000000]
+    assert precise(records(2, [(two, 2), (four, 4)]), [two, four, miss], 2, 1, 4, query_only=1) == [1, -1, 1, -1]
+
+
+def test_histogram_equals_closed_form_of_merged_table():
+    rng = np.random.default_rng(2)
+    keys = rng.integers(0, 3000, size=(40000, 1), dtype=np.uint64) << np.uint64(20)
+    k, c = C.count_table(keys)
+    # an ideal split: every key once in low.bin, c-1 in its record when c >= 2;
+    # a few keys as Bloom false positives: absent from low.bin, full count in the record
+    fp = rng.random(k.shape[0]) < 0.05
+    low = k[~fp]
+    items = [(k[i].tolist(), int(c[i] - (0 if fp[i] else 1))) for i in range(k.shape[0]) if fp[i] or c[i] >= 2]
+    for minf, maxf in ((1, 100), (2, 9), (5, 5)):
+        assert precise(records(1, items), low, 1, minf, maxf) == C.histogram(c, minf, maxf).tolist()
+
+
+# ------------------------------------------------------------------ Stage 0
+
+def test_lpt_and_bloom_capacity():
+    counts = np.zeros(256, dtype=np.uint64)
+    counts[:4] = [9000, 7000, 5000, 3000]
+    avg = O.orc_clamp_prefix_counts(counts.ctypes.data)
+    assert counts[4] == 1000 and counts[0] == 9000
+    assert avg == (24000 + 252 * 1000) // 256
+    owners = np.zeros(256, dtype=np.uint8)
+    loads = np.zeros(3, dtype=np.uint64)
+    O.orc_lpt(counts.ctypes.data, 3, owners.ctypes.data, loads.ctypes.data)
+    # largest first: 9000 -> 0, 7000 -> 1, 5000 -> 2, 3000 -> 2 (5000 < 7000 < 9000) ...
+    assert owners[:4].tolist() == [0, 1, 2, 2]
+    assert int(loads.sum()) == int(counts.sum())
+    # LPT bound: max load <= mean + max item
+    assert loads.max() <= loads.mean() + 9000
+    caps = np.zeros(256, dtype=np.uint64)
+    O.orc_bloom_capacity(counts.ctypes.data, 3 * 10**9, caps.ctypes.data)
+    # prefix 0: 1e9 k-mers * 9000/276000 share * 4.81 / 64 = 2.45M words -> 2^22
+    assert caps[0] == 1 << 22
+    assert caps.min() >= 1 << 15
+    assert all(int(x) & (int(x) - 1) == 0 for x in caps)
+
+
+# ------------------------------------------------------------------ fixtures
+
+SMALL = [n for n, v in C.reference_runs().items() if v["K"] <= 13_000_000]
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_oracle_matches_reference_runs(name):
+    import kmertreecounter_amd as K
+    fx = C.reference_runs()[name]
+    seq = K.generate(fx["workload"], scale=fx["scale"], seed=fx["seed"], fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, fx["k"]))
+    assert int(counts.sum()) == fx["K"]
+    assert keys.shape[0] == fx["D"]
+    assert C.table_sha(keys, counts) == fx["sha256"]
+    h = C.histogram(counts, 1, fx["max_freq"])
+    assert {str(i + 1): int(v) for i, v in enumerate(h) if v} == fx["histogram_nonzero"]
