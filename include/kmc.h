@@ -66,16 +66,20 @@ typedef enum kc_partition {
     KC_PARTITION_LEARNED = 1 /* learned CDF model -> balanced key-range buckets */
 } kc_partition;
 
+#define KC_SUB_BITS_NONE 0xFFFFFFFFu
+
 typedef struct kc_config {
     uint32_t k;               /* k-mer length, 1..63 */
     uint32_t partition;       /* kc_partition */
     uint32_t min_count;       /* root record filter on the repeat count, inclusive */
     uint32_t max_count;       /*   (append_export_record, src/NewKmerTree.h:1159-1168); 0 = UINT32_MAX */
     uint64_t expected_bytes;  /* hint: total sequence bytes to be pushed (0 = grow on demand) */
-    uint32_t sub_bits;        /* fine fan-out bits below the root prefix (0 = automatic, max 8) */
+    uint32_t sub_bits;        /* fine fan-out bits below the root prefix: 0 = automatic, 1..8,
+                                 or KC_SUB_BITS_NONE for no sub-partition */
     uint32_t device;          /* HIP device ordinal */
     uint32_t classifier_num;  /* LPT owner count reported by kc_plan (reference: (n-2-n/8)/2) */
-    uint32_t reserved;
+    uint32_t pass_keys;       /* keys per counting pass of a bucket (0 = default; larger buckets take
+                                 several hash-range passes) */
 } kc_config;
 
 typedef struct kc_stats {

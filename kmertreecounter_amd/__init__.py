@@ -26,6 +26,7 @@ TREE_PATH = PKG / "kmc_tree"
 
 KC_PARTITION_PREFIX = 0
 KC_PARTITION_LEARNED = 1
+KC_SUB_BITS_NONE = 0xFFFFFFFF
 
 
 class KmcError(RuntimeError):
@@ -38,7 +39,7 @@ class Config(ctypes.Structure):
         ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
         ("expected_bytes", ctypes.c_uint64), ("sub_bits", ctypes.c_uint32),
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("pass_keys", ctypes.c_uint32),
     ]
 
 
@@ -201,11 +202,12 @@ class Engine:
     """One counting job on one GPU (a kc_engine handle)."""
 
     def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
-                 expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3):
+                 expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
+                 pass_keys: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
-        cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, 0)
+        cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

@@ -21,7 +21,6 @@ constexpr uint64_t kStageBytes = 64ULL << 20;   // pinned staging buffer for pus
 constexpr uint64_t kEmitChunk = 32ULL << 20;    // pinned bounce buffer for emits
 constexpr uint64_t kFineTarget = 32768;         // keys per fine bucket the fan-out aims for
 
-enum Ctr { C_FIRST = 0, C_REC, C_SPILLED, C_ERR, C_MAXB, C_DISTINCT, C_NCTR = 8 };
 
 }  // namespace
 
@@ -48,18 +47,22 @@ struct kc_engine {
     bool planned = false;
     PlanResult plan;
 
-    // work buffers
+    // work buffers: A/B hold keys (K_cap x N words), R the per-bucket record
+    // slices (K_cap x N words of bytes); the final firsts and records reuse them
     uint64_t key_cap = 0;  // keys
     uint64_t* d_A = nullptr;
     uint64_t* d_B = nullptr;
-    unsigned long long* d_small = nullptr;  // coarse counts/cursors/offsets/tile starts
+    uint8_t* d_R = nullptr;
+    unsigned long long* d_small = nullptr;  // coarse counts/cursors/offsets/tile starts, root offsets
     unsigned long long* d_fcounts = nullptr;
     unsigned long long* d_fcursor = nullptr;
     unsigned long long* d_fine_off = nullptr;
+    unsigned long long* d_foff = nullptr;
+    unsigned long long* d_roff = nullptr;
+    uint32_t* d_bucket_u32 = nullptr;       // nfirst, nrec, ndist per fine bucket
     uint32_t nf_alloc = 0;
-    uint8_t* d_recs = nullptr;
-    uint8_t* d_recs_sorted = nullptr;
-    uint64_t rec_cap = 0;
+    uint8_t* d_ovf = nullptr;               // overflow records
+    uint64_t ovf_cap = 0;
     unsigned long long* d_spill_keys = nullptr;
     uint32_t* d_spill_cnt = nullptr;
     uint64_t spill_cap = 0;
@@ -75,6 +78,7 @@ struct kc_engine {
     uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0;
     uint32_t nf_last = 0;
     const uint64_t* res_firsts = nullptr;
+    const uint8_t* res_recs = nullptr;
     std::vector<unsigned long long> root_off;
     double phase_ms[7] = {};
     char* h_bounce = nullptr;
@@ -175,42 +179,49 @@ void free_work(kc_engine* h)
 {
     hipFree(h->d_A); h->d_A = nullptr;
     hipFree(h->d_B); h->d_B = nullptr;
+    hipFree(h->d_R); h->d_R = nullptr;
     h->key_cap = 0;
 }
+
+// small arrays in d_small (unsigned long long units)
+constexpr uint64_t kSmCounts = 0, kSmCursor = 8 * 256, kSmCoarseOff = 16 * 256, kSmTileStart = kSmCoarseOff + 257,
+                   kSmRootOvf = kSmTileStart + 257, kSmRootOff = kSmRootOvf + 256, kSmOvfCursor = kSmRootOff + 257,
+                   kSmTotal = kSmOvfCursor + 256;
 
 int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 {
     const uint64_t kb = 8ULL * h->nw;
     if (h->key_cap < padded_len) {
         free_work(h);
-        if (hipMalloc(&h->d_A, padded_len * kb) != hipSuccess || hipMalloc(&h->d_B, padded_len * kb) != hipSuccess) {
+        if (hipMalloc(&h->d_A, padded_len * kb) != hipSuccess || hipMalloc(&h->d_B, padded_len * kb) != hipSuccess ||
+            hipMalloc(&h->d_R, padded_len * kb) != hipSuccess) {
             free_work(h);
             return h->fail(KC_ERR_NOMEM, "device allocation for the key buffers failed");
         }
         h->key_cap = padded_len;
     }
     if (!h->d_small) {
-        // counts[8*256] cursor[8*256] coarse_off[257] tile_start[257] root_counts[256] root_off[257] root_cursor[256]
-        HIPX(hipMalloc(&h->d_small, sizeof(unsigned long long) * (8 * 256 * 2 + 257 * 2 + 256 + 257 + 256 + 64)));
-        HIPX(hipMalloc(&h->d_ctr, sizeof(unsigned long long) * C_NCTR));
+        HIPX(hipMalloc(&h->d_small, sizeof(unsigned long long) * kSmTotal));
+        HIPX(hipMalloc(&h->d_ctr, sizeof(unsigned long long) * kCtrCount));
     }
     if (h->nf_alloc < nf) {
         hipFree(h->d_fcounts); hipFree(h->d_fcursor); hipFree(h->d_fine_off);
+        hipFree(h->d_foff); hipFree(h->d_roff); hipFree(h->d_bucket_u32);
         HIPX(hipMalloc(&h->d_fcounts, sizeof(unsigned long long) * kXcds * nf));
         HIPX(hipMalloc(&h->d_fcursor, sizeof(unsigned long long) * kXcds * nf));
         HIPX(hipMalloc(&h->d_fine_off, sizeof(unsigned long long) * (nf + 1)));
+        HIPX(hipMalloc(&h->d_foff, sizeof(unsigned long long) * (nf + 1)));
+        HIPX(hipMalloc(&h->d_roff, sizeof(unsigned long long) * (nf + 1)));
+        HIPX(hipMalloc(&h->d_bucket_u32, sizeof(uint32_t) * 3 * nf));
         h->nf_alloc = nf;
     }
-    const uint64_t want_rec = std::max<uint64_t>(1 << 16, padded_len / 8);
-    if (h->rec_cap < want_rec) {
-        hipFree(h->d_recs); hipFree(h->d_recs_sorted);
-        h->d_recs = h->d_recs_sorted = nullptr;
-        if (hipMalloc(&h->d_recs, want_rec * rec_bytes(h->nw)) != hipSuccess ||
-            hipMalloc(&h->d_recs_sorted, want_rec * rec_bytes(h->nw)) != hipSuccess)
-            return h->fail(KC_ERR_NOMEM, "device allocation for the record buffers failed");
-        h->rec_cap = want_rec;
+    if (h->ovf_cap == 0) {
+        const uint64_t cap = 1ULL << 20;
+        if (hipMalloc(&h->d_ovf, cap * rec_bytes(h->nw)) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the overflow records failed");
+        h->ovf_cap = cap;
     }
-    const uint64_t want_spill = std::min<uint64_t>(1ULL << 28, std::max<uint64_t>(1ULL << 16, pow2_at_least(padded_len / 64)));
+    const uint64_t want_spill = std::min<uint64_t>(1ULL << 26, std::max<uint64_t>(1ULL << 16, pow2_at_least(padded_len / 512)));
     if (h->spill_cap < want_spill) {
         hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
         if (hipMalloc(&h->d_spill_keys, want_spill * 8ULL * h->nw) != hipSuccess ||
@@ -224,6 +235,7 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 
 uint32_t choose_sbits(const kc_engine* h, uint64_t padded_len)
 {
+    if (h->cfg.sub_bits == KC_SUB_BITS_NONE) return 0;
     if (h->cfg.sub_bits) return std::min<uint32_t>(8, h->cfg.sub_bits);
     const double per_coarse = (double)padded_len / 256.0;
     uint32_t sb = 0;
@@ -249,18 +261,18 @@ int run_pipeline_t(kc_engine* h)
     rt.coarse_bounds = h->d_bounds;
     rt.fine_bounds = h->d_bounds ? h->d_bounds + 255 : nullptr;
 
-    unsigned long long* counts = h->d_small;
-    unsigned long long* cursor = counts + 8 * 256;
-    unsigned long long* coarse_off = cursor + 8 * 256;
-    unsigned long long* tile_start = coarse_off + 257;
-    unsigned long long* root_counts = tile_start + 257;
-    unsigned long long* root_off = root_counts + 256;
-    unsigned long long* root_cursor = root_off + 257;
+    unsigned long long* counts = h->d_small + kSmCounts;
+    unsigned long long* cursor = h->d_small + kSmCursor;
+    unsigned long long* coarse_off = h->d_small + kSmCoarseOff;
+    unsigned long long* tile_start = h->d_small + kSmTileStart;
+    uint32_t* d_nfirst = h->d_bucket_u32;
+    uint32_t* d_nrec = d_nfirst + nf;
+    uint32_t* d_ndist = d_nrec + nf;
 
     for (int attempt = 0; attempt < 6; ++attempt) {
         if (padded > h->seq_len) HIPX(hipMemsetAsync(h->d_seq + h->seq_len, '\n', padded - h->seq_len, st));
-        HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * (8 * 256 * 2 + 257 * 2 + 256 + 257 + 256), st));
-        HIPX(hipMemsetAsync(h->d_ctr, 0, sizeof(unsigned long long) * C_NCTR, st));
+        HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * kSmTotal, st));
+        HIPX(hipMemsetAsync(h->d_ctr, 0, sizeof(unsigned long long) * kCtrCount, st));
         if (h->spill_dirty) {
             HIPX(hipMemsetAsync(h->d_spill_keys, 0xFF, h->spill_cap * 8ULL * NW, st));
             HIPX(hipMemsetAsync(h->d_spill_cnt, 0, h->spill_cap * 4ULL, st));
@@ -276,8 +288,8 @@ int run_pipeline_t(kc_engine* h)
         HIPX(hipEventRecord(h->ev[2], st));
         HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
         HIPX(hipEventRecord(h->ev[3], st));
-        const uint64_t* src = h->d_A;
-        uint64_t* firsts = h->d_B;
+        uint64_t* src = h->d_A;        // keys grouped by fine bucket (P3 input)
+        uint64_t* slices = h->d_B;     // per-bucket first-occurrence slices
         const unsigned long long* fine_off = coarse_off;
         if (sbits) {
             const uint64_t max_tiles = padded / sub_tile(NW) + 256;
@@ -289,44 +301,60 @@ int run_pipeline_t(kc_engine* h)
             HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
             HIPX(hipEventRecord(h->ev[7], st));
             src = h->d_B;
-            firsts = h->d_A;
+            slices = h->d_A;
             fine_off = h->d_fine_off;
         } else {
             for (int i = 4; i < 8; ++i) HIPX(hipEventRecord(h->ev[i], st));
         }
         CountArgs a{};
-        a.firsts = firsts;
-        a.first_count = h->d_ctr + C_FIRST;
-        a.recs = h->d_recs;
-        a.rec_count = h->d_ctr + C_REC;
-        a.rec_cap = h->rec_cap;
+        a.firsts = slices;
+        a.rec_tmp = h->d_R;
+        a.nfirst = d_nfirst;
+        a.nrec = d_nrec;
+        a.ndist = d_ndist;
+        a.ovf = h->d_ovf;
+        a.ovf_count = h->d_ctr + kCtrOvf;
+        a.ovf_cap = h->ovf_cap;
         a.spill_keys = h->d_spill_keys;
         a.spill_cnt = h->d_spill_cnt;
         a.spill_cap = h->spill_cap;
-        a.spilled = h->d_ctr + C_SPILLED;
-        a.error_flags = h->d_ctr + C_ERR;
-        a.max_bucket = h->d_ctr + C_MAXB;
-        a.distinct_rep = h->d_ctr + C_DISTINCT;
+        a.spilled = h->d_ctr + kCtrSpilled;
+        a.spill_distinct = h->d_ctr + kCtrSpillDistinct;
+        a.error_flags = h->d_ctr + kCtrErr;
         a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
         a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
+        a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys : CountCfg<NW>::kPassKeys;
         HIPX(hipEventRecord(h->ev[8], st));
         HIPX(launch_count<NW>(nf, src, fine_off, a, st));
         HIPX(hipEventRecord(h->ev[9], st));
         HIPX(launch_spill_emit<NW>(a, st));
-        HIPX(launch_group_records<NW>(false, h->d_recs, h->d_ctr + C_REC, h->rec_cap, root_counts, nullptr, nullptr, st));
-        HIPX(launch_rec_scan(root_counts, root_off, root_cursor, st));
-        HIPX(launch_group_records<NW>(true, h->d_recs, h->d_ctr + C_REC, h->rec_cap, root_counts, root_cursor, h->d_recs_sorted, st));
+        SegArgs sa{};
+        sa.nf = nf;
+        sa.sbits = sbits;
+        sa.fine_off = fine_off;
+        sa.nfirst = d_nfirst;
+        sa.nrec = d_nrec;
+        sa.ndist = d_ndist;
+        sa.foff = h->d_foff;
+        sa.roff = h->d_roff;
+        sa.root_ovf = h->d_small + kSmRootOvf;
+        sa.root_off = h->d_small + kSmRootOff;
+        sa.ovf_cursor = h->d_small + kSmOvfCursor;
+        sa.ctr = h->d_ctr;
+        sa.rec_out_cap = (h->key_cap * 8ULL * NW) / rec_bytes(NW);
+        // final firsts into the P3 key input (free now), final records into the slices buffer
+        HIPX(launch_finalize<NW>(sa, a, slices, src, h->d_R, reinterpret_cast<uint8_t*>(slices), st));
         HIPX(hipEventRecord(h->ev[10], st));
         HIPX(hipStreamSynchronize(st));
 
-        unsigned long long ctr[C_NCTR];
+        unsigned long long ctr[kCtrCount];
         HIPX(hipMemcpy(ctr, h->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
         unsigned long long coff256 = 0;
         HIPX(hipMemcpy(&coff256, coarse_off + 256, sizeof(coff256), hipMemcpyDeviceToHost));
-        h->spill_dirty = ctr[C_SPILLED] != 0;
-        if (ctr[C_ERR] & kErrSpillFull) {
+        h->spill_dirty = ctr[kCtrSpilled] != 0;
+        if (ctr[kCtrErr] & kErrSpillFull) {
             if (h->spill_cap >= (1ULL << 31)) return h->fail(KC_ERR_CAPACITY, "spill table cannot grow further");
-            const uint64_t ncap = h->spill_cap * 4;
+            const uint64_t ncap = std::min<uint64_t>(1ULL << 31, std::max<uint64_t>(h->spill_cap * 4, pow2_at_least(ctr[kCtrSpilled] * 4)));
             hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
             h->d_spill_keys = nullptr; h->d_spill_cnt = nullptr; h->spill_cap = 0;
             if (hipMalloc(&h->d_spill_keys, ncap * 8ULL * NW) != hipSuccess || hipMalloc(&h->d_spill_cnt, ncap * 4ULL) != hipSuccess)
@@ -335,26 +363,29 @@ int run_pipeline_t(kc_engine* h)
             h->spill_dirty = true;
             continue;
         }
-        if (ctr[C_ERR] & kErrRecFull) {
-            const uint64_t ncap = std::max<uint64_t>(h->rec_cap * 2, ctr[C_REC] + 1024);
-            hipFree(h->d_recs); hipFree(h->d_recs_sorted);
-            h->d_recs = h->d_recs_sorted = nullptr; h->rec_cap = 0;
-            if (hipMalloc(&h->d_recs, ncap * rec_bytes(NW)) != hipSuccess || hipMalloc(&h->d_recs_sorted, ncap * rec_bytes(NW)) != hipSuccess)
-                return h->fail(KC_ERR_NOMEM, "record buffer growth failed");
-            h->rec_cap = ncap;
+        if (ctr[kCtrErr] & kErrRecFull) {
+            const uint64_t ncap = std::max<uint64_t>(h->ovf_cap * 4, ctr[kCtrOvf] + 1024);
+            hipFree(h->d_ovf);
+            h->d_ovf = nullptr; h->ovf_cap = 0;
+            if (hipMalloc(&h->d_ovf, ncap * rec_bytes(NW)) != hipSuccess)
+                return h->fail(KC_ERR_NOMEM, "overflow record buffer growth failed");
+            h->ovf_cap = ncap;
             continue;
         }
+        if (ctr[kCtrErr] & kErrRecOut)
+            return h->fail(KC_ERR_CAPACITY, "root records exceed the record buffer (too many overflow records)");
         h->K = coff256;
-        h->n_first = ctr[C_FIRST];
-        h->n_rec = ctr[C_REC];
-        h->n_spilled = ctr[C_SPILLED];
-        h->max_bucket = ctr[C_MAXB];
-        h->n_distinct_rep = ctr[C_DISTINCT];
+        h->n_first = ctr[kCtrFirst];
+        h->n_rec = ctr[kCtrRec];
+        h->n_spilled = ctr[kCtrSpilled];
+        h->max_bucket = ctr[kCtrMaxBucket];
+        h->n_distinct_rep = ctr[kCtrDistinct] + ctr[kCtrSpillDistinct];
         h->nf_last = nf;
-        h->res_firsts = firsts;
+        h->res_firsts = src;
+        h->res_recs = reinterpret_cast<const uint8_t*>(slices);
         h->root_off.assign(257, 0);
-        HIPX(hipMemcpy(h->root_off.data(), root_off, sizeof(unsigned long long) * 257, hipMemcpyDeviceToHost));
-        // P0, P1, P2a, P2b, P3 kernels; P4 = record grouping (three kernels); total
+        HIPX(hipMemcpy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 257, hipMemcpyDeviceToHost));
+        // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total
         const int pairs[6][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {9, 10}};
         float ms = 0;
         for (int p = 0; p < 6; ++p) {
@@ -396,7 +427,7 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     *out = nullptr;
     const uint32_t nw = kc_words_for_k(cfg->k);
     if (!nw) return KC_ERR_ARG;
-    if (cfg->partition > 1 || cfg->sub_bits > 8) return KC_ERR_ARG;
+    if (cfg->partition > 1 || (cfg->sub_bits > 8 && cfg->sub_bits != KC_SUB_BITS_NONE)) return KC_ERR_ARG;
     if (cfg->max_count && cfg->min_count > cfg->max_count) return KC_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_ERR_NODEV;
@@ -436,8 +467,10 @@ int kc_destroy(kc_engine* h)
     hipFree(h->d_fcounts);
     hipFree(h->d_fcursor);
     hipFree(h->d_fine_off);
-    hipFree(h->d_recs);
-    hipFree(h->d_recs_sorted);
+    hipFree(h->d_foff);
+    hipFree(h->d_roff);
+    hipFree(h->d_bucket_u32);
+    hipFree(h->d_ovf);
     hipFree(h->d_spill_keys);
     hipFree(h->d_spill_cnt);
     hipFree(h->d_bounds);
@@ -556,7 +589,7 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     const uint64_t b = h->root_off[root_id], e = h->root_off[root_id + 1];
     for (uint64_t i = b; i < e; i += per) {
         const uint64_t n = std::min(per, e - i);
-        HIPX(hipMemcpy(h->h_bounce, h->d_recs_sorted + i * rb, n * rb, hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(h->h_bounce, h->res_recs + i * rb, n * rb, hipMemcpyDeviceToHost));
         int rc = cb(user, h->h_bounce, n);
         if (rc) return h->fail(rc, "records callback returned an error");
     }

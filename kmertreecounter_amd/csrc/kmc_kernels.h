@@ -11,7 +11,8 @@ namespace kmc {
 constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, workgroup i -> XCD i % 8
 
 constexpr unsigned long long kErrSpillFull = 1ULL;
-constexpr unsigned long long kErrRecFull = 2ULL;
+constexpr unsigned long long kErrRecFull = 2ULL;  // overflow record list full
+constexpr unsigned long long kErrRecOut = 4ULL;   // records exceed the final record buffer
 
 // P0/P1: one 16-byte word of sequence per thread, kThreads*16 positions per tile.
 template <int NW> struct ExtractCfg;
@@ -23,41 +24,67 @@ template <int NW> struct SubCfg;
 template <> struct SubCfg<1> { static constexpr int kThreads = 512; static constexpr int kTile = 8192; };
 template <> struct SubCfg<2> { static constexpr int kThreads = 256; static constexpr int kTile = 4096; };
 
-// P3: one workgroup per fine bucket.  LDS: Bloom words + count table + firsts buffer.
+// P3: one workgroup per fine bucket (1 per CU).  LDS: Bloom words + count
+// table; buckets above kPassKeys keys are processed in hash-range passes.
 template <int NW> struct CountCfg;
 template <> struct CountCfg<1> {
-    static constexpr int kThreads = 512;
-    static constexpr uint32_t kBloomWords = 2048;
-    static constexpr uint32_t kTableSlots = 4096;
-    static constexpr uint32_t kFirstBuf = 1024;
+    static constexpr int kThreads = 1024;
+    static constexpr uint32_t kBloomWords = 4096;
+    static constexpr uint32_t kTableSlots = 8192;
+    static constexpr uint32_t kPassKeys = 98304;
     static constexpr uint32_t kProbe = 64;
 };
 template <> struct CountCfg<2> {
-    static constexpr int kThreads = 512;
-    static constexpr uint32_t kBloomWords = 2048;
-    static constexpr uint32_t kTableSlots = 2048;
-    static constexpr uint32_t kFirstBuf = 1024;
+    static constexpr int kThreads = 1024;
+    static constexpr uint32_t kBloomWords = 4096;
+    static constexpr uint32_t kTableSlots = 4096;
+    static constexpr uint32_t kPassKeys = 49152;
     static constexpr uint32_t kProbe = 64;
 };
+
+// probes into the global spill table before a key is declared unplaceable
+constexpr uint32_t kSpillProbe = 1024;
 
 // ExportRecord<N> size (src/definition.h:159-163): N key words + u32 count, padded to 8
 template <int NW> struct RecBytes { static constexpr uint32_t value = 8 * NW + 8; };
 
+// device counters (d_ctr)
+enum : int { kCtrFirst = 0, kCtrRec, kCtrSpilled, kCtrErr, kCtrMaxBucket, kCtrDistinct, kCtrOvf, kCtrSpillDistinct, kCtrCount };
+
 struct CountArgs {
-    uint64_t* firsts;                 // first-occurrence keys, appended
-    unsigned long long* first_count;
-    uint8_t* recs;                    // ExportRecord<N> array, appended
-    unsigned long long* rec_count;
-    uint64_t rec_cap;
-    unsigned long long* spill_keys;   // NW x spill_cap words (structure of arrays)
+    uint64_t* firsts;                   // bucket f: keys at [fine_off[f], fine_off[f+1])
+    uint8_t* rec_tmp;                   // bucket f: records at byte fine_off[f]*8*NW, n_f*8*NW bytes
+    uint32_t* nfirst;                   // [nf]
+    uint32_t* nrec;                     // [nf] records kept in the bucket's slice
+    uint32_t* ndist;                    // [nf] distinct repeated keys in LDS tables (before the filter)
+    uint8_t* ovf;                       // overflow records (slice full, spilled keys)
+    unsigned long long* ovf_count;
+    uint64_t ovf_cap;
+    unsigned long long* spill_keys;     // NW x spill_cap words (structure of arrays)
     uint32_t* spill_cnt;
-    uint64_t spill_cap;               // power of two
-    unsigned long long* spilled;      // distinct keys claimed in the spill table
+    uint64_t spill_cap;                 // power of two
+    unsigned long long* spilled;        // distinct keys claimed in the spill table
+    unsigned long long* spill_distinct; // spilled keys found by k_spill_emit
     unsigned long long* error_flags;
-    unsigned long long* max_bucket;
-    unsigned long long* distinct_rep; // distinct repeated keys (before the min/max filter)
     uint32_t min_count;
     uint32_t max_count;
+    uint64_t pass_keys;
+};
+
+struct SegArgs {
+    uint32_t nf;
+    uint32_t sbits;
+    const unsigned long long* fine_off;
+    const uint32_t* nfirst;
+    const uint32_t* nrec;
+    const uint32_t* ndist;
+    unsigned long long* foff;           // [nf+1]
+    unsigned long long* roff;           // [nf+1]
+    unsigned long long* root_ovf;       // [256]
+    unsigned long long* root_off;       // [257]
+    unsigned long long* ovf_cursor;     // [256]
+    unsigned long long* ctr;
+    uint64_t rec_out_cap;               // records the final record buffer holds
 };
 
 template <int NW>
@@ -77,11 +104,8 @@ hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long
 template <int NW>
 hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st);
 template <int NW>
-hipError_t launch_group_records(bool scatter, const uint8_t* recs, const unsigned long long* n_ptr, uint64_t cap,
-                                unsigned long long* root_counts, unsigned long long* root_cursor, uint8_t* out,
-                                hipStream_t st);
-hipError_t launch_rec_scan(const unsigned long long* root_counts, unsigned long long* root_off,
-                           unsigned long long* root_cursor, hipStream_t st);
+hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_t* firsts_in, uint64_t* firsts_out,
+                           const uint8_t* rec_in, uint8_t* rec_out, hipStream_t st);
 template <int NW>
 hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t st);
 

@@ -9,11 +9,13 @@
 //   S1  k_scan_fine
 //   P2b k_sub<scatter>      A -> B grouped by fine bucket
 //   P3  k_count             one workgroup per fine bucket: LDS Bloom filter
-//                           (first occurrences -> firsts list) and LDS
-//                           open-addressing count table for repeats, with a
-//                           global spill table behind it; emits root records
+//                           (first occurrences -> firsts) and LDS open-
+//                           addressing count table for repeats (hash-range
+//                           passes for large buckets, a global spill table
+//                           behind it); writes into the bucket's own slices
 //   P3b k_spill_emit        records of keys that lived in the spill table
-//   P4  k_rec_hist/scatter  records grouped by root id (key word 0 >> 56)
+//   P4  k_seg_scan/k_compact per-bucket outputs made contiguous: firsts in one
+//                           array, records grouped by root id (word 0 >> 56)
 // Workgroup-to-XCD: workgroup i runs on XCD i % 8, so write cursors are kept
 // per XCD and each XCD fills its own sub-range of a bucket (partial cache lines
 // merge in that XCD's L2 instead of crossing XCDs).
@@ -449,6 +451,22 @@ __device__ __forceinline__ void write_record(uint8_t* recs, uint64_t i, const Ke
     p[NW] = (uint64_t)count;  // count in the low 4 bytes, zero padding above
 }
 
+// Overflow list append (records that did not fit their bucket's region, and
+// records of spilled keys).  Rare path: one global atomic per record.
+template <int NW>
+__device__ __forceinline__ void append_overflow(const CountArgs& a, const Key<NW>& key, uint32_t count)
+{
+    const unsigned long long pos = atomicAdd(a.ovf_count, 1ULL);
+    if (pos < a.ovf_cap) write_record<NW>(a.ovf, pos, key, count);
+    else atomicOr(a.error_flags, (unsigned long long)kErrRecFull);
+}
+
+// One workgroup per fine bucket f, keys B[fine_off[f], fine_off[f+1]).
+// A bucket larger than pass_keys is processed in P hash-range passes (pass =
+// top hash bits), each with a fresh LDS Bloom filter and count table, so the
+// table holds the repeated keys of one pass only.  First occurrences are
+// written to the bucket's own slice of `firsts`; root records to the bucket's
+// own slice of `rec_tmp` (n_f*8*NW bytes); no global atomics on the hot path.
 template <int NW>
 __global__ void __launch_bounds__(CountCfg<NW>::kThreads)
 k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ fine_off, CountArgs args)
@@ -456,238 +474,278 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
     constexpr int T = CountCfg<NW>::kThreads;
     constexpr uint32_t BW = CountCfg<NW>::kBloomWords;
     constexpr uint32_t TS = CountCfg<NW>::kTableSlots;
-    constexpr uint32_t FB = CountCfg<NW>::kFirstBuf;
     __shared__ unsigned long long s_bloom[BW];
     __shared__ unsigned long long s_tkey[NW * TS];
     __shared__ uint32_t s_tcnt[TS];
-    __shared__ Key<NW> s_first[FB];
     __shared__ uint32_t s_nfirst;
-    __shared__ unsigned long long s_fbase;
-    __shared__ uint32_t s_rec;
-    __shared__ unsigned long long s_rbase;
-    __shared__ uint32_t s_distinct;
+    __shared__ uint32_t s_nrec;
+    __shared__ uint32_t s_dist;
+    __shared__ uint32_t s_abort;
 
     const int t = threadIdx.x;
+    const uint32_t lane = t & (kWave - 1);
     const uint32_t f = blockIdx.x;
     const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
-    for (uint32_t i = t; i < BW; i += T) s_bloom[i] = 0ULL;
-    for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
-    for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
-    if (t == 0) { s_nfirst = 0; s_rec = 0; s_distinct = 0; }
-    __syncthreads();
-    if (t == 0 && hi > lo) atomicMax(args.max_bucket, (unsigned long long)(hi - lo));
-
     const unsigned long long n = hi - lo;
+    const uint32_t passes = n > args.pass_keys ? (uint32_t)((n + args.pass_keys - 1) / args.pass_keys) : 1u;
+    const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
+    uint8_t* rec_region = args.rec_tmp + lo * 8ULL * NW;
+    if (t == 0) { s_nfirst = 0; s_nrec = 0; s_dist = 0; }
     const unsigned long long iters = (n + T - 1) / T;
-    for (unsigned long long it = 0; it < iters; ++it) {
-        const unsigned long long i = lo + it * T + t;
-        const bool active = i < hi;
-        Key<NW> key;
-        bool first = false;
-        if (active) {
-            key = load_key<NW>(B, i);
-            uint64_t hl, hh;
-            xxh3_128(key, hl, hh);
-            const uint32_t word = (uint32_t)(hl & (BW - 1));
-            const uint64_t mask = bloom_mask(hl, hh);
-            // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
-            unsigned long long old = s_bloom[word];
-            if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
-            first = (old & mask) != mask;
-            if (!first) {
-                bool claimed;
-                const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
-                                                         (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
-                if (s >= 0) {
-                    atomicAdd(&s_tcnt[s], 1u);
-                } else {
-                    // spill to the global table
-                    const uint64_t gs = (hh >> 32) ^ (hl >> 17);
-                    const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap,
-                                                             args.spill_cap - 1, gs, (uint32_t)args.spill_cap,
-                                                             key, claimed);
-                    if (g >= 0) {
-                        atomicAdd(&args.spill_cnt[g], 1u);
-                        if (claimed) atomicAdd(args.spilled, 1ULL);
-                    } else {
-                        atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        for (uint32_t i = t; i < BW; i += T) s_bloom[i] = 0ULL;
+        for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
+        for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
+        // a full spill table voids the run (the host regrows it and runs again):
+        // skip the remaining work
+        if (t == 0) s_abort = (__atomic_load_n(args.error_flags, __ATOMIC_RELAXED) & kErrSpillFull) ? 1u : 0u;
+        __syncthreads();
+        if (s_abort) break;
+        for (unsigned long long it = 0; it < iters; ++it) {
+            const unsigned long long i = lo + it * T + t;
+            bool first = false;
+            Key<NW> key;
+            if (i < hi) {
+                key = load_key<NW>(B, i);
+                uint64_t hl, hh;
+                xxh3_128(key, hl, hh);
+                const uint32_t kp = passes > 1 ? (uint32_t)(((hh >> 48) * passes) >> 16) : 0u;
+                if (kp == pass) {
+                    const uint32_t word = (uint32_t)(hl & (BW - 1));
+                    const uint64_t mask = bloom_mask(hl, hh);
+                    // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
+                    unsigned long long old = s_bloom[word];
+                    if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
+                    first = (old & mask) != mask;
+                    if (!first) {
+                        bool claimed;
+                        const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
+                                                                 (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
+                        if (s >= 0) {
+                            atomicAdd(&s_tcnt[s], 1u);
+                        } else {
+                            const uint64_t gs = (hh >> 32) ^ (hl >> 17);
+                            const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
+                            const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap,
+                                                                     args.spill_cap - 1, gs, glimit, key, claimed);
+                            if (g >= 0) {
+                                atomicAdd(&args.spill_cnt[g], 1u);
+                                if (claimed) atomicAdd(args.spilled, 1ULL);
+                            } else {
+                                atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+                            }
+                        }
                     }
                 }
             }
+            // first occurrences: one LDS atomic per wave, consecutive slots per wave
+            const uint64_t bal = __ballot(first);
+            uint32_t wbase = 0;
+            if (lane == 0 && bal) wbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bal));
+            wbase = __shfl(wbase, 0, kWave);
+            if (first) store_key<NW>(args.firsts, lo + wbase + lanes_below(bal), key);
         }
-        // append firsts through the LDS buffer
-        const uint64_t bal = __ballot(first);
-        uint32_t wbase = 0;
-        if ((threadIdx.x & (kWave - 1)) == 0 && bal) wbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bal));
-        wbase = __shfl(wbase, 0, kWave);
-        if (first) s_first[wbase + lanes_below(bal)] = key;
         __syncthreads();
-        const uint32_t nfb = s_nfirst;
-        if (nfb + T > FB || it + 1 == iters) {
-            if (t == 0 && nfb) s_fbase = atomicAdd(args.first_count, (unsigned long long)nfb);
-            __syncthreads();
-            for (uint32_t j = t; j < nfb; j += T) store_key<NW>(args.firsts, s_fbase + j, s_first[j]);
-            __syncthreads();
-            if (t == 0) s_nfirst = 0;
-            __syncthreads();
-        }
-    }
-    __syncthreads();
-
-    // root records (append_export_record filter on the repeat count,
-    // src/NewKmerTree.h:1159-1168)
-    uint32_t mine = 0, distinct = 0;
-    for (uint32_t s = t; s < TS; s += T) {
-        const uint32_t cnt = s_tcnt[s];
-        if (cnt) {
-            distinct++;
-            if (cnt >= args.min_count && cnt <= args.max_count) mine++;
-        }
-    }
-    if (distinct) atomicAdd(&s_distinct, distinct);
-    const uint32_t mbase = mine ? atomicAdd(&s_rec, mine) : 0u;
-    __syncthreads();
-    if (t == 0) {
-        const uint32_t nr = s_rec;
-        s_rbase = nr ? atomicAdd(args.rec_count, (unsigned long long)nr) : 0ULL;
-        if (s_distinct) atomicAdd(args.distinct_rep, (unsigned long long)s_distinct);
-    }
-    __syncthreads();
-    uint64_t pos = s_rbase + mbase;
-    for (uint32_t s = t; s < TS; s += T) {
-        const uint32_t cnt = s_tcnt[s];
-        if (cnt && cnt >= args.min_count && cnt <= args.max_count) {
-            if (pos < args.rec_cap) {
+        // root records of this pass (append_export_record filter on the repeat
+        // count, src/NewKmerTree.h:1159-1168)
+        uint32_t dist = 0;
+        for (uint32_t s0 = 0; s0 < TS; s0 += T) {
+            const uint32_t s = s0 + t;
+            const uint32_t cnt = s_tcnt[s];
+            dist += cnt ? 1u : 0u;
+            const bool keep = cnt && cnt >= args.min_count && cnt <= args.max_count;
+            const uint64_t bal = __ballot(keep);
+            uint32_t wbase = 0;
+            if (lane == 0 && bal) wbase = atomicAdd(&s_nrec, (uint32_t)__popcll(bal));
+            wbase = __shfl(wbase, 0, kWave);
+            if (keep) {
                 Key<NW> key;
 #pragma unroll
                 for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
-                write_record<NW>(args.recs, pos, key, cnt);
-            } else {
-                atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
+                const uint64_t pos = wbase + lanes_below(bal);
+                if (pos < rec_cap) write_record<NW>(rec_region, pos, key, cnt);
+                else append_overflow<NW>(args, key, cnt);
             }
-            pos++;
         }
+        if (dist) atomicAdd(&s_dist, dist);
+        __syncthreads();
+    }
+    if (t == 0) {
+        args.nfirst[f] = s_nfirst;
+        args.nrec[f] = (uint32_t)(s_nrec < rec_cap ? s_nrec : rec_cap);
+        args.ndist[f] = s_dist;
     }
 }
 
-// P3b: records of spilled keys (slots with a claimed final word)
+// Records of keys that lived in the global spill table go to the overflow
+// list.  Fixed grid, grid-stride; returns at once when nothing spilled.
 template <int NW>
 __global__ void __launch_bounds__(256)
 k_spill_emit(CountArgs args)
 {
-    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = false;
-    uint32_t cnt = 0;
-    Key<NW> key;
-    if (s < args.spill_cap && *args.spilled) {
-        cnt = args.spill_cnt[s];
-        if (cnt) {
+    __shared__ uint32_t s_n;
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_dist;
+    if (*args.spilled == 0) return;
+    for (uint64_t base = (uint64_t)blockIdx.x * 256; base < args.spill_cap; base += (uint64_t)gridDim.x * 256) {
+        if (threadIdx.x == 0) { s_n = 0; s_dist = 0; }
+        __syncthreads();
+        const uint64_t s = base + threadIdx.x;
+        uint32_t cnt = 0;
+        Key<NW> key;
+        bool keep = false;
+        if (s < args.spill_cap) {
+            cnt = args.spill_cnt[s];
+            if (cnt) {
 #pragma unroll
-            for (int w = 0; w < NW; ++w) key.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
-            keep = cnt >= args.min_count && cnt <= args.max_count;
-            atomicAdd(args.distinct_rep, 1ULL);
+                for (int w = 0; w < NW; ++w) key.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
+                keep = cnt >= args.min_count && cnt <= args.max_count;
+                atomicAdd(&s_dist, 1u);
+            }
         }
-    }
-    const uint64_t bal = __ballot(keep);
-    unsigned long long wbase = 0;
-    if ((threadIdx.x & (kWave - 1)) == 0 && bal) wbase = atomicAdd(args.rec_count, (unsigned long long)__popcll(bal));
-    wbase = __shfl(wbase, 0, kWave);
-    if (keep) {
-        const uint64_t pos = wbase + lanes_below(bal);
-        if (pos < args.rec_cap) write_record<NW>(args.recs, pos, key, cnt);
-        else atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
+        const uint32_t my = keep ? atomicAdd(&s_n, 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_base = s_n ? atomicAdd(args.ovf_count, (unsigned long long)s_n) : 0ULL;
+            if (s_dist) atomicAdd(args.spill_distinct, (unsigned long long)s_dist);
+        }
+        __syncthreads();
+        if (keep) {
+            const uint64_t pos = s_base + my;
+            if (pos < args.ovf_cap) write_record<NW>(args.ovf, pos, key, cnt);
+            else atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
+        }
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------ P4
 
-// Records grouped by root id (word 0 >> 56).  Grid-stride over tiles of
-// kRecTile records; count pass adds per-root totals, scatter pass reserves a
-// run per (tile, root) and writes it from LDS.
-constexpr int kRecThreads = 256;
-constexpr int kRecPer = 8;
-constexpr int kRecTile = kRecThreads * kRecPer;
-
-template <int NW, bool SCATTER>
-__global__ void __launch_bounds__(kRecThreads)
-k_rec_group(const uint8_t* __restrict__ recs, const unsigned long long* __restrict__ n_ptr, uint64_t cap,
-            unsigned long long* __restrict__ root_counts, unsigned long long* __restrict__ root_cursor,
-            uint8_t* __restrict__ out)
+// overflow records: per-root histogram (single workgroup, the list is short)
+template <int NW>
+__global__ void __launch_bounds__(256)
+k_ovf_hist(const uint8_t* __restrict__ ovf, const unsigned long long* __restrict__ n_ptr, uint64_t cap,
+           unsigned long long* __restrict__ root_ovf)
 {
-    constexpr uint32_t RW = RecBytes<NW>::value / 8;  // u64 words per record
-    __shared__ uint32_t s_hist[256];
-    __shared__ uint32_t s_off[256];
-    __shared__ unsigned long long s_gbase[256];
-    __shared__ uint32_t s_scan[kRecThreads / kWave + 1];
-    __shared__ uint64_t s_rec[SCATTER ? kRecTile * RW : 1];
-    __shared__ uint8_t s_bkt[SCATTER ? kRecTile : 1];
-
-    const int t = threadIdx.x;
+    __shared__ uint32_t s_h[256];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
     const uint64_t n = *n_ptr < cap ? *n_ptr : cap;
-    const uint64_t ntiles = (n + kRecTile - 1) / kRecTile;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        s_hist[t] = 0;
-        __syncthreads();
-        const uint64_t base = tile * kRecTile;
-        uint64_t r[kRecPer][RW];
-        uint32_t root[kRecPer], rank[kRecPer];
-#pragma unroll
-        for (int j = 0; j < kRecPer; ++j) {
-            const uint64_t i = base + t + (uint64_t)j * kRecThreads;
-            if (i < n) {
-                const uint64_t* src = reinterpret_cast<const uint64_t*>(recs + i * (RW * 8));
-#pragma unroll
-                for (uint32_t w = 0; w < RW; ++w) r[j][w] = src[w];
-                root[j] = (uint32_t)(r[j][0] >> 56);
-                rank[j] = atomicAdd(&s_hist[root[j]], 1u);
-            }
+    for (uint64_t i = threadIdx.x; i < n; i += 256) {
+        const uint64_t w0 = *reinterpret_cast<const uint64_t*>(ovf + i * RecBytes<NW>::value);
+        atomicAdd(&s_h[w0 >> 56], 1u);
+    }
+    __syncthreads();
+    root_ovf[threadIdx.x] = s_h[threadIdx.x];
+}
+
+// Exclusive scans of the per-bucket outputs (one workgroup of 1024 threads):
+// foff/roff over buckets, per-root record offsets (records of root r: the
+// in-bucket records of its buckets, then its overflow records), totals.
+__global__ void __launch_bounds__(1024)
+k_seg_scan(SegArgs a)
+{
+    __shared__ unsigned long long s_scan[17];
+    __shared__ unsigned long long s_rin[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t nf = a.nf;
+    const uint32_t per = (nf + 1023) / 1024;
+    const uint32_t f0 = t * per;
+    unsigned long long sf = 0, sr = 0, sd = 0, mx = 0;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t f = f0 + j;
+        if (f < nf) {
+            sf += a.nfirst[f];
+            sr += a.nrec[f];
+            sd += a.ndist[f];
+            const unsigned long long nb = a.fine_off[f + 1] - a.fine_off[f];
+            mx = nb > mx ? nb : mx;
         }
-        __syncthreads();
-        if constexpr (!SCATTER) {
-            const uint32_t h = s_hist[t];
-            if (h) atomicAdd(&root_counts[t], (unsigned long long)h);
-            __syncthreads();
-        } else {
-            const uint32_t h = s_hist[t];
-            uint32_t total;
-            const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
-            s_off[t] = off;
-            if (h) s_gbase[t] = atomicAdd(&root_cursor[t], (unsigned long long)h);
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kRecPer; ++j) {
-                const uint64_t i = base + t + (uint64_t)j * kRecThreads;
-                if (i < n) {
-                    const uint32_t slot = s_off[root[j]] + rank[j];
-#pragma unroll
-                    for (uint32_t w = 0; w < RW; ++w) s_rec[slot * RW + w] = r[j][w];
-                    s_bkt[slot] = (uint8_t)root[j];
-                }
-            }
-            __syncthreads();
-            for (uint32_t slot = t; slot < total; slot += kRecThreads) {
-                const uint32_t b = s_bkt[slot];
-                const uint64_t dst = s_gbase[b] + (slot - s_off[b]);
-                uint64_t* d = reinterpret_cast<uint64_t*>(out + dst * (RW * 8));
-#pragma unroll
-                for (uint32_t w = 0; w < RW; ++w) d[w] = s_rec[slot * RW + w];
-            }
-            __syncthreads();
+    }
+    unsigned long long tf, tr;
+    unsigned long long bf = block_exclusive_scan<unsigned long long>(sf, s_scan, &tf);
+    unsigned long long br = block_exclusive_scan<unsigned long long>(sr, s_scan, &tr);
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t f = f0 + j;
+        if (f < nf) {
+            a.foff[f] = bf;
+            a.roff[f] = br;
+            bf += a.nfirst[f];
+            br += a.nrec[f];
         }
+    }
+    if (t == 1023) { a.foff[nf] = tf; a.roff[nf] = tr; }
+    if (sd) atomicAdd(&a.ctr[kCtrDistinct], sd);
+    if (mx) atomicMax(&a.ctr[kCtrMaxBucket], mx);
+    __syncthreads();
+    // root r owns buckets [r << sbits, (r+1) << sbits)
+    unsigned long long tot = 0;
+    if (t < 256) {
+        const unsigned long long rin = a.roff[(t + 1) << a.sbits] - a.roff[t << a.sbits];
+        s_rin[t] = rin;
+        tot = rin + a.root_ovf[t];
+    }
+    unsigned long long all;
+    const unsigned long long roff_root = block_exclusive_scan<unsigned long long>(tot, s_scan, &all);
+    if (t < 256) {
+        a.root_off[t] = roff_root;
+        a.ovf_cursor[t] = roff_root + s_rin[t];
+        if (t == 255) a.root_off[256] = all;
+    }
+    if (t == 0) {
+        a.ctr[kCtrFirst] = tf;
+        a.ctr[kCtrRec] = all;
+        if (all > a.rec_out_cap) atomicOr(&a.ctr[kCtrErr], kErrRecOut);
     }
 }
 
+// Segmented copies of bucket f: its firsts to the contiguous firsts array,
+// then (second kernel) its records to their root's range.
+template <int NW>
 __global__ void __launch_bounds__(256)
-k_rec_scan(const unsigned long long* __restrict__ root_counts, unsigned long long* __restrict__ root_off,
-           unsigned long long* __restrict__ root_cursor)
+k_compact_firsts(SegArgs a, const uint64_t* __restrict__ firsts_in, uint64_t* __restrict__ firsts_out)
 {
-    __shared__ unsigned long long s_scan[5];
-    unsigned long long all;
-    const unsigned long long c = root_counts[threadIdx.x];
-    const unsigned long long off = block_exclusive_scan<unsigned long long>(c, s_scan, &all);
-    root_off[threadIdx.x] = off;
-    root_cursor[threadIdx.x] = off;
-    if (threadIdx.x == 255) root_off[256] = all;
+    const uint32_t f = blockIdx.x;
+    const unsigned long long lo = a.fine_off[f];
+    const uint32_t nfst = a.nfirst[f];
+    const unsigned long long fdst = a.foff[f];
+    for (uint32_t j = threadIdx.x; j < nfst; j += 256) store_key<NW>(firsts_out, fdst + j, load_key<NW>(firsts_in, lo + j));
+}
+
+template <int NW>
+__global__ void __launch_bounds__(256)
+k_compact_recs(SegArgs a, const uint8_t* __restrict__ rec_in, uint8_t* __restrict__ rec_out)
+{
+    constexpr uint32_t RW = RecBytes<NW>::value / 8;
+    const uint32_t f = blockIdx.x;
+    const uint32_t nr = a.nrec[f];
+    if (nr == 0 || (a.ctr[kCtrErr] & kErrRecOut)) return;
+    const unsigned long long lo = a.fine_off[f];
+    const uint32_t r = f >> a.sbits;
+    const unsigned long long rdst = a.root_off[r] + (a.roff[f] - a.roff[(unsigned long long)r << a.sbits]);
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(rec_in) + lo * NW;
+    uint64_t* dst = reinterpret_cast<uint64_t*>(rec_out) + rdst * RW;
+    const uint32_t words = nr * RW;
+    for (uint32_t w = threadIdx.x; w < words; w += 256) dst[w] = src[w];
+}
+
+template <int NW>
+__global__ void __launch_bounds__(256)
+k_ovf_scatter(const uint8_t* __restrict__ ovf, const unsigned long long* __restrict__ n_ptr, uint64_t cap,
+              unsigned long long* __restrict__ ovf_cursor, uint8_t* __restrict__ rec_out,
+              const unsigned long long* __restrict__ ctr)
+{
+    constexpr uint32_t RW = RecBytes<NW>::value / 8;
+    if (ctr[kCtrErr] & kErrRecOut) return;
+    const uint64_t n = *n_ptr < cap ? *n_ptr : cap;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t* src = reinterpret_cast<const uint64_t*>(ovf + i * (RW * 8));
+        const unsigned long long pos = atomicAdd(&ovf_cursor[src[0] >> 56], 1ULL);
+        uint64_t* d = reinterpret_cast<uint64_t*>(rec_out + pos * (RW * 8));
+#pragma unroll
+        for (uint32_t w = 0; w < RW; ++w) d[w] = src[w];
+    }
 }
 
 // ------------------------------------------------------------------ debug hooks
@@ -765,29 +823,25 @@ hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long
 template <int NW>
 hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st)
 {
-    const uint64_t blocks = (args.spill_cap + 255) / 256;
-    hipLaunchKernelGGL((k_spill_emit<NW>), dim3((uint32_t)blocks), dim3(256), 0, st, args);
+    hipLaunchKernelGGL((k_spill_emit<NW>), dim3(1024), dim3(256), 0, st, args);
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }
 
 template <int NW>
-hipError_t launch_group_records(bool scatter, const uint8_t* recs, const unsigned long long* n_ptr, uint64_t cap,
-                                unsigned long long* root_counts, unsigned long long* root_cursor, uint8_t* out,
-                                hipStream_t st)
+hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_t* firsts_in, uint64_t* firsts_out,
+                           const uint8_t* rec_in, uint8_t* rec_out, hipStream_t st)
 {
-    if (scatter)
-        hipLaunchKernelGGL((k_rec_group<NW, true>), dim3(2048), dim3(kRecThreads), 0, st, recs, n_ptr, cap, root_counts, root_cursor, out);
-    else
-        hipLaunchKernelGGL((k_rec_group<NW, false>), dim3(2048), dim3(kRecThreads), 0, st, recs, n_ptr, cap, root_counts, root_cursor, out);
+    hipLaunchKernelGGL((k_ovf_hist<NW>), dim3(1), dim3(256), 0, st, ca.ovf, ca.ovf_count, ca.ovf_cap, sa.root_ovf);
     KMC_LAUNCH_CHECK();
-    return hipSuccess;
-}
-
-hipError_t launch_rec_scan(const unsigned long long* root_counts, unsigned long long* root_off,
-                           unsigned long long* root_cursor, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(256), 0, st, root_counts, root_off, root_cursor);
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, st, sa);
+    KMC_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_compact_firsts<NW>), dim3(sa.nf), dim3(256), 0, st, sa, firsts_in, firsts_out);
+    KMC_LAUNCH_CHECK();
+    // records go into the buffer the firsts were just copied out of
+    hipLaunchKernelGGL((k_compact_recs<NW>), dim3(sa.nf), dim3(256), 0, st, sa, rec_in, rec_out);
+    KMC_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_ovf_scatter<NW>), dim3(256), dim3(256), 0, st, ca.ovf, ca.ovf_count, ca.ovf_cap, sa.ovf_cursor, rec_out, sa.ctr);
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -809,8 +863,8 @@ template hipError_t launch_count<1>(uint32_t, const uint64_t*, const unsigned lo
 template hipError_t launch_count<2>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
-template hipError_t launch_group_records<1>(bool, const uint8_t*, const unsigned long long*, uint64_t, unsigned long long*, unsigned long long*, uint8_t*, hipStream_t);
-template hipError_t launch_group_records<2>(bool, const uint8_t*, const unsigned long long*, uint64_t, unsigned long long*, unsigned long long*, uint8_t*, hipStream_t);
+template hipError_t launch_finalize<1>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
+template hipError_t launch_finalize<2>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
 template hipError_t launch_debug_hash<1>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template hipError_t launch_debug_hash<2>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 

@@ -90,12 +90,12 @@ def test_empty_and_degenerate_inputs():
             check_outputs(f, r, s, 31, keys, counts)
 
 
-@pytest.mark.parametrize("sub_bits", [0, 3, 8])
+@pytest.mark.parametrize("sub_bits", [K.KC_SUB_BITS_NONE, 3, 8])
 def test_fanout_does_not_change_table(sub_bits):
     seq = K.generate(1, scale=20, seed=3, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
     f, r, s = run_engine([seq], 31, sub_bits=sub_bits)
-    assert s.fine_buckets == 256 << sub_bits
+    assert s.fine_buckets == 256 << (0 if sub_bits == K.KC_SUB_BITS_NONE else sub_bits)
     check_outputs(f, r, s, 31, keys, counts)
 
 
@@ -110,12 +110,23 @@ def test_block_carry_path_matches():
 
 
 def test_spill_table_path():
-    # sub_bits=0 leaves ~190k keys per fine bucket, more distinct repeats than the
-    # LDS table holds: the global spill table must keep the counts exact
+    # no sub-partition and one pass per bucket leave ~190k keys (~11k distinct repeats)
+    # per bucket, more than the LDS table holds: the global spill table must keep
+    # the counts exact
     seq = K.generate(1, scale=1, seed=1, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
-    f, r, s = run_engine([seq], 31, sub_bits=0)
+    f, r, s = run_engine([seq], 31, sub_bits=K.KC_SUB_BITS_NONE, pass_keys=1 << 30)
+    assert s.fine_buckets == 256
     assert s.spilled_keys > 0
+    check_outputs(f, r, s, 31, keys, counts)
+
+
+@pytest.mark.parametrize("pass_keys", [2000, 50000])
+def test_multi_pass_buckets(pass_keys):
+    # buckets split into hash-range passes give the same table
+    seq = K.generate(1, scale=10, seed=21, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    f, r, s = run_engine([seq], 31, sub_bits=2, pass_keys=pass_keys)
     check_outputs(f, r, s, 31, keys, counts)
 
 
