@@ -106,6 +106,16 @@ typedef struct kc_stats {
     uint64_t prefix_counts[256]; /* Stage-0 root prefix histogram, after the >= 1000 clamp */
     uint64_t bloom_capacity[256];/* reference Bloom sizing (words) from the Stage-0 sample */
     uint8_t prefix_owners[256];  /* LPT owner of each root prefix */
+    /* learned partitioner (kc_plan with KC_PARTITION_LEARNED) */
+    uint64_t learned_train_keys;        /* sorted Stage-0 keys the model was trained on */
+    double learned_train_ms;            /* sample extraction + training + boundary derivation */
+    double learned_loss_first;          /* training loss at the first and last step */
+    double learned_loss_last;
+    double learned_sample_max_over_mean;/* largest of the 4096 learned buckets / mean, on the sample */
+    double prefix_sample_max_over_mean; /* largest of the 4096 12-bit key prefixes / mean, on the sample */
+    double learned_sample_distinct_max_over_mean; /* the same two, counting distinct keys per bucket */
+    double prefix_sample_distinct_max_over_mean;
+    uint64_t extra_passes;              /* counting passes beyond one per bucket, summed over buckets */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;
@@ -142,6 +152,19 @@ KC_API const char* kc_last_error(kc_engine* h);
  * words (the Bloom filter's hash, src/BloomFilter.h:209); out gets n (low64,
  * high64) pairs.  Lets tests pin the device hash against the host one. */
 KC_API int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out);
+
+/* Test hooks of the learned partitioner, no reference counterpart.
+ * kc_debug_gemm_bf16: C[M][N] = A[M][K] . Bt[N][K]^T on the MFMA product
+ *   (A, Bt as bf16 bit patterns; M, N multiples of 64, K of 16).
+ * kc_debug_mlp_grad: loss and gradient (fp32, parameter layout of the model)
+ *   of a batch (nb a multiple of 64) with the device kernels. */
+KC_API int kc_debug_gemm_bf16(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, uint32_t N, uint32_t K);
+KC_API int kc_debug_mlp_grad(const float* params, const uint64_t* keys, const uint32_t* labels, uint32_t nb,
+                             float* grad, float* loss);
+/* kc_debug_learned_fit: train on n sorted key words (steps, lr, batch, seed as
+ * in kc_plan) and return the 4097 bucket boundaries and the loss every 50 steps. */
+KC_API int kc_debug_learned_fit(const uint64_t* sorted, uint64_t n, uint32_t steps, float lr, uint32_t batch,
+                                uint64_t seed, uint64_t* bounds, float* loss_curve, uint32_t loss_cap);
 
 #ifdef __cplusplus
 }

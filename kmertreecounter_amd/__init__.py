@@ -58,6 +58,11 @@ class Stats(ctypes.Structure):
         ("stage0_sample_kmers", ctypes.c_uint64),
         ("prefix_counts", ctypes.c_uint64 * 256), ("bloom_capacity", ctypes.c_uint64 * 256),
         ("prefix_owners", ctypes.c_uint8 * 256),
+        ("learned_train_keys", ctypes.c_uint64), ("learned_train_ms", ctypes.c_double),
+        ("learned_loss_first", ctypes.c_double), ("learned_loss_last", ctypes.c_double),
+        ("learned_sample_max_over_mean", ctypes.c_double), ("prefix_sample_max_over_mean", ctypes.c_double),
+        ("learned_sample_distinct_max_over_mean", ctypes.c_double),
+        ("prefix_sample_distinct_max_over_mean", ctypes.c_double), ("extra_passes", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -93,6 +98,7 @@ EXPORTED = [
     "kc_emit_root", "kc_stats_get", "kc_destroy", "kc_count_resident", "kc_words_for_k",
     "kc_last_error", "kc_debug_xxh3", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
+    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit",
 ]
 
 _lib = None
@@ -127,6 +133,9 @@ def lib() -> ctypes.CDLL:
     L.kc_last_error.argtypes = [vp]
     L.kc_last_error.restype = cp
     L.kc_debug_xxh3.argtypes = [vp, u64, u32, vp]
+    L.kc_debug_gemm_bf16.argtypes = [vp, vp, vp, u32, u32, u32]
+    L.kc_debug_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.kc_debug_learned_fit.argtypes = [vp, u64, u32, ctypes.c_float, u32, u64, vp, vp, u32]
     L.kcs_generate.argtypes = [ctypes.POINTER(SynthSpec), ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(u64),
                                ctypes.POINTER(u64)]
     L.kcs_free.argtypes = [vp]
@@ -306,6 +315,46 @@ def debug_xxh3(keys: np.ndarray, nw: int) -> np.ndarray:
     if rc != 0:
         raise KmcError(f"kc_debug_xxh3 failed: {rc}")
     return out.reshape(n, 2)
+
+
+MLP_NPARAM = 64 * 64 + 64 + 64 * 64 + 64 + 4096 * 64 + 4096
+
+
+def debug_gemm_bf16(a_bits: np.ndarray, bt_bits: np.ndarray) -> np.ndarray:
+    """MFMA product on the GPU: a (M, K) and bt (N, K) as uint16 bf16 bit patterns."""
+    a = np.ascontiguousarray(a_bits, dtype=np.uint16)
+    b = np.ascontiguousarray(bt_bits, dtype=np.uint16)
+    M, K = a.shape
+    N = b.shape[0]
+    c = np.zeros((M, N), dtype=np.float32)
+    rc = lib().kc_debug_gemm_bf16(a.ctypes.data, b.ctypes.data, c.ctypes.data, M, N, K)
+    if rc != 0:
+        raise KmcError(f"kc_debug_gemm_bf16 failed: {rc}")
+    return c
+
+
+def debug_mlp_grad(params: np.ndarray, keys: np.ndarray, labels: np.ndarray) -> tuple[float, np.ndarray]:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    y = np.ascontiguousarray(labels, dtype=np.uint32)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    rc = lib().kc_debug_mlp_grad(p.ctypes.data, k.ctypes.data, y.ctypes.data, k.size, g.ctypes.data, loss.ctypes.data)
+    if rc != 0:
+        raise KmcError(f"kc_debug_mlp_grad failed: {rc}")
+    return float(loss[0]), g
+
+
+def debug_learned_fit(sorted_keys: np.ndarray, steps: int = 10000, lr: float = 1e-2, batch: int = 2048,
+                      seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    k = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+    b = np.zeros(4097, dtype=np.uint64)
+    cap = steps // 50 + 2
+    lc = np.zeros(cap, dtype=np.float32)
+    rc = lib().kc_debug_learned_fit(k.ctypes.data, k.size, steps, lr, batch, seed, b.ctypes.data, lc.ctypes.data, cap)
+    if rc != 0:
+        raise KmcError(f"kc_debug_learned_fit failed: {rc}")
+    return b, lc
 
 
 def tree_run(fastq_path: str, tmp_dir: str, k: int, n_thread: int = 8, min_count: int = 1, max_count: int = 0,

@@ -11,6 +11,7 @@
 
 #include "../../include/kmc.h"
 #include "kmc_kernels.h"
+#include "kmc_learned.h"
 #include "kmc_plan.h"
 
 using namespace kmc;
@@ -68,14 +69,14 @@ struct kc_engine {
     uint64_t spill_cap = 0;
     bool spill_dirty = true;
     unsigned long long* d_ctr = nullptr;
-    uint64_t* d_bounds = nullptr;  // learned: 255 coarse + 256*255 fine boundaries
-    bool bounds_valid = false;
+    uint64_t* d_bounds = nullptr;  // learned: 255 coarse + 256*(2^sbits-1) fine boundaries
+    uint32_t bounds_sbits = ~0u;   // fan-out the uploaded bounds were built for
 
     hipEvent_t ev[12] = {};
 
     // results
     bool have_results = false;
-    uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0;
+    uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0, extra_passes = 0;
     uint32_t nf_last = 0;
     const uint64_t* res_firsts = nullptr;
     const uint8_t* res_recs = nullptr;
@@ -185,8 +186,8 @@ void free_work(kc_engine* h)
 
 // small arrays in d_small (unsigned long long units)
 constexpr uint64_t kSmCounts = 0, kSmCursor = 8 * 256, kSmCoarseOff = 16 * 256, kSmTileStart = kSmCoarseOff + 257,
-                   kSmRootOvf = kSmTileStart + 257, kSmRootOff = kSmRootOvf + 256, kSmOvfCursor = kSmRootOff + 257,
-                   kSmTotal = kSmOvfCursor + 256;
+                   kSmRootOvf = kSmTileStart + 257, kSmRootMain = kSmRootOvf + 256, kSmRootOff = kSmRootMain + 256,
+                   kSmOvfOff = kSmRootOff + 257, kSmOvfCursor = kSmOvfOff + 257, kSmTotal = kSmOvfCursor + 256;
 
 int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 {
@@ -249,14 +250,21 @@ int run_pipeline_t(kc_engine* h)
     hipStream_t st = h->st;
     const uint64_t tile = ext_tile(h->nw);
     const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
-    const uint32_t sbits = (h->cfg.partition == KC_PARTITION_LEARNED && h->bounds_valid) ? h->plan.sbits
-                                                                                           : choose_sbits(h, padded);
+    const uint32_t sbits = choose_sbits(h, padded);
     const uint32_t nf = 256u << sbits;
     int rc = ensure_work(h, padded > 0 ? padded : tile, nf);
     if (rc) return rc;
 
+    const bool learned = h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
+    if (learned && h->bounds_sbits != sbits) {
+        std::vector<uint64_t> rb;
+        routing_bounds(h->plan.learned_b, sbits, rb);
+        if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * 255)));
+        HIPX(hipMemcpy(h->d_bounds, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice));
+        h->bounds_sbits = sbits;
+    }
     Route rt{};
-    rt.mode = (h->cfg.partition == KC_PARTITION_LEARNED && h->bounds_valid) ? 1u : 0u;
+    rt.mode = learned ? 1u : 0u;
     rt.sbits = sbits;
     rt.coarse_bounds = h->d_bounds;
     rt.fine_bounds = h->d_bounds ? h->d_bounds + 255 : nullptr;
@@ -324,6 +332,7 @@ int run_pipeline_t(kc_engine* h)
         a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
         a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
         a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys : CountCfg<NW>::kPassKeys;
+        a.extra_passes = h->d_ctr + kCtrExtraPasses;
         HIPX(hipEventRecord(h->ev[8], st));
         HIPX(launch_count<NW>(nf, src, fine_off, a, st));
         HIPX(hipEventRecord(h->ev[9], st));
@@ -338,7 +347,9 @@ int run_pipeline_t(kc_engine* h)
         sa.foff = h->d_foff;
         sa.roff = h->d_roff;
         sa.root_ovf = h->d_small + kSmRootOvf;
+        sa.root_main = h->d_small + kSmRootMain;
         sa.root_off = h->d_small + kSmRootOff;
+        sa.ovf_off = h->d_small + kSmOvfOff;
         sa.ovf_cursor = h->d_small + kSmOvfCursor;
         sa.ctr = h->d_ctr;
         sa.rec_out_cap = (h->key_cap * 8ULL * NW) / rec_bytes(NW);
@@ -380,11 +391,12 @@ int run_pipeline_t(kc_engine* h)
         h->n_spilled = ctr[kCtrSpilled];
         h->max_bucket = ctr[kCtrMaxBucket];
         h->n_distinct_rep = ctr[kCtrDistinct] + ctr[kCtrSpillDistinct];
+        h->extra_passes = ctr[kCtrExtraPasses];
         h->nf_last = nf;
         h->res_firsts = src;
         h->res_recs = reinterpret_cast<const uint8_t*>(slices);
-        h->root_off.assign(257, 0);
-        HIPX(hipMemcpy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 257, hipMemcpyDeviceToHost));
+        h->root_off.assign(2 * 257, 0);
+        HIPX(hipMemcpy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
         // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total
         const int pairs[6][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {9, 10}};
         float ms = 0;
@@ -501,10 +513,10 @@ int kc_plan(kc_engine* h)
     HIPX(hipSetDevice(h->device));
     int rc = stage0_plan(h->k, h->cfg, h->sample, h->st, h->plan, h->err);
     if (rc) return rc;
-    if (h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.bounds.empty()) {
-        if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * 255)));
-        HIPX(hipMemcpy(h->d_bounds, h->plan.bounds.data(), sizeof(uint64_t) * h->plan.bounds.size(), hipMemcpyHostToDevice));
-        h->bounds_valid = true;
+    if (h->cfg.partition == KC_PARTITION_LEARNED) {
+        rc = learned_plan(h->k, h->sample, h->st, h->plan, h->err);
+        if (rc) return rc;
+        h->bounds_sbits = ~0u;  // routing bounds are rebuilt for the run's fan-out
     }
     h->planned = true;
     return KC_OK;
@@ -586,12 +598,15 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t rb = rec_bytes(h->nw);
     const uint64_t per = kEmitChunk / rb;
-    const uint64_t b = h->root_off[root_id], e = h->root_off[root_id + 1];
-    for (uint64_t i = b; i < e; i += per) {
-        const uint64_t n = std::min(per, e - i);
-        HIPX(hipMemcpy(h->h_bounce, h->res_recs + i * rb, n * rb, hipMemcpyDeviceToHost));
-        int rc = cb(user, h->h_bounce, n);
-        if (rc) return h->fail(rc, "records callback returned an error");
+    // in-bucket records of the root, then its overflow records
+    for (int part = 0; part < 2; ++part) {
+        const uint64_t b = h->root_off[part * 257 + root_id], e = h->root_off[part * 257 + root_id + 1];
+        for (uint64_t i = b; i < e; i += per) {
+            const uint64_t n = std::min(per, e - i);
+            HIPX(hipMemcpy(h->h_bounce, h->res_recs + i * rb, n * rb, hipMemcpyDeviceToHost));
+            int rc = cb(user, h->h_bounce, n);
+            if (rc) return h->fail(rc, "records callback returned an error");
+        }
     }
     return KC_OK;
 }
@@ -610,6 +625,7 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->sequence_bytes = h->seq_len;
     s->fine_buckets = h->nf_last;
     s->max_bucket_keys = h->max_bucket;
+    s->extra_passes = h->extra_passes;
     s->mean_bucket_keys = h->nf_last ? (double)h->K / h->nf_last : 0.0;
     s->ms_extract_count = h->phase_ms[0];
     s->ms_extract_scatter = h->phase_ms[1];
@@ -620,6 +636,16 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->ms_total = h->phase_ms[6];
     s->ms_h2d = h->h2d_ms;
     if (h->planned) {
+        s->learned_train_keys = h->plan.train_keys;
+        s->learned_sample_distinct_max_over_mean = h->plan.sample_distinct_learned;
+        s->prefix_sample_distinct_max_over_mean = h->plan.sample_distinct_prefix;
+        s->learned_train_ms = h->plan.train_ms;
+        if (!h->plan.loss_curve.empty()) {
+            s->learned_loss_first = h->plan.loss_curve.front();
+            s->learned_loss_last = h->plan.loss_curve.back();
+        }
+        s->learned_sample_max_over_mean = h->plan.sample_max_over_mean_learned;
+        s->prefix_sample_max_over_mean = h->plan.sample_max_over_mean_prefix;
         s->stage0_sample_kmers = h->plan.sample_kmers;
         for (int i = 0; i < 256; ++i) {
             s->prefix_counts[i] = h->plan.prefix_counts[i];
@@ -645,6 +671,39 @@ int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out)
     hipFree(dk);
     hipFree(dout);
     return rc;
+}
+
+int kc_debug_gemm_bf16(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, uint32_t N, uint32_t K)
+{
+    if (!A || !Bt || !C) return KC_ERR_ARG;
+    std::string err;
+    const int rc = mlp_debug_gemm(A, Bt, C, M, N, K, err);
+    return rc == -1 ? KC_ERR_ARG : (rc ? KC_ERR_HIP : KC_OK);
+}
+
+int kc_debug_mlp_grad(const float* params, const uint64_t* keys, const uint32_t* labels, uint32_t nb, float* grad,
+                      float* loss)
+{
+    if (!params || !keys || !labels || !grad || !loss) return KC_ERR_ARG;
+    for (uint32_t i = 0; i < nb; ++i)
+        if (labels[i] >= kMlpC) return KC_ERR_ARG;
+    std::string err;
+    const int rc = mlp_debug_loss_grad(params, keys, labels, nb, grad, loss, err);
+    return rc == -1 ? KC_ERR_ARG : (rc ? KC_ERR_HIP : KC_OK);
+}
+
+int kc_debug_learned_fit(const uint64_t* sorted, uint64_t n, uint32_t steps, float lr, uint32_t batch, uint64_t seed,
+                         uint64_t* bounds, float* loss_curve, uint32_t loss_cap)
+{
+    if (!sorted || !bounds || n < 64 || batch == 0 || batch % 64 || steps == 0) return KC_ERR_ARG;
+    MlpTrainOpts o;
+    o.steps = steps;
+    o.lr = lr;
+    o.batch = batch;
+    o.seed = seed;
+    std::string err;
+    const int rc = mlp_debug_fit(sorted, n, o, bounds, loss_curve, loss_curve ? loss_cap : 0, err);
+    return rc ? KC_ERR_HIP : KC_OK;
 }
 
 }  // extern "C"

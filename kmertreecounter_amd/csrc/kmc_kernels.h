@@ -32,6 +32,7 @@ template <> struct CountCfg<1> {
     static constexpr uint32_t kBloomWords = 4096;
     static constexpr uint32_t kTableSlots = 8192;
     static constexpr uint32_t kPassKeys = 98304;
+    static constexpr uint32_t kPassDistinct = 5000;  // target distinct keys per pass (61% table load)
     static constexpr uint32_t kProbe = 64;
 };
 template <> struct CountCfg<2> {
@@ -39,6 +40,7 @@ template <> struct CountCfg<2> {
     static constexpr uint32_t kBloomWords = 4096;
     static constexpr uint32_t kTableSlots = 4096;
     static constexpr uint32_t kPassKeys = 49152;
+    static constexpr uint32_t kPassDistinct = 2500;
     static constexpr uint32_t kProbe = 64;
 };
 
@@ -49,7 +51,8 @@ constexpr uint32_t kSpillProbe = 1024;
 template <int NW> struct RecBytes { static constexpr uint32_t value = 8 * NW + 8; };
 
 // device counters (d_ctr)
-enum : int { kCtrFirst = 0, kCtrRec, kCtrSpilled, kCtrErr, kCtrMaxBucket, kCtrDistinct, kCtrOvf, kCtrSpillDistinct, kCtrCount };
+enum : int { kCtrFirst = 0, kCtrRec, kCtrSpilled, kCtrErr, kCtrMaxBucket, kCtrDistinct, kCtrOvf, kCtrSpillDistinct,
+             kCtrExtraPasses, kCtrCount };
 
 struct CountArgs {
     uint64_t* firsts;                   // bucket f: keys at [fine_off[f], fine_off[f+1])
@@ -69,6 +72,7 @@ struct CountArgs {
     uint32_t min_count;
     uint32_t max_count;
     uint64_t pass_keys;
+    unsigned long long* extra_passes;   // passes beyond the first, summed over buckets
 };
 
 struct SegArgs {
@@ -80,8 +84,10 @@ struct SegArgs {
     const uint32_t* ndist;
     unsigned long long* foff;           // [nf+1]
     unsigned long long* roff;           // [nf+1]
-    unsigned long long* root_ovf;       // [256]
-    unsigned long long* root_off;       // [257]
+    unsigned long long* root_ovf;       // [256] overflow records per root
+    unsigned long long* root_main;      // [256] in-bucket records per root
+    unsigned long long* root_off;       // [257] root ranges of the in-bucket records
+    unsigned long long* ovf_off;        // [257] root ranges of the overflow records (after all in-bucket ones)
     unsigned long long* ovf_cursor;     // [256]
     unsigned long long* ctr;
     uint64_t rec_out_cap;               // records the final record buffer holds

@@ -462,9 +462,12 @@ __device__ __forceinline__ void append_overflow(const CountArgs& a, const Key<NW
 }
 
 // One workgroup per fine bucket f, keys B[fine_off[f], fine_off[f+1]).
-// A bucket larger than pass_keys is processed in P hash-range passes (pass =
-// top hash bits), each with a fresh LDS Bloom filter and count table, so the
-// table holds the repeated keys of one pass only.  First occurrences are
+// A bucket larger than pass_keys is processed in hash-range passes (range of
+// the top 16 hash bits), each with a fresh LDS Bloom filter and count table, so
+// the table holds the repeated keys of one range only.  The first range is
+// sized from the bucket's occurrences, each later one from the distinct keys
+// the previous range held: a bucket dominated by a few very frequent keys
+// finishes in one or two passes.  First occurrences are
 // written to the bucket's own slice of `firsts`; root records to the bucket's
 // own slice of `rec_tmp` (n_f*8*NW bytes); no global atomics on the hot path.
 template <int NW>
@@ -480,28 +483,45 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
     __shared__ uint32_t s_nfirst;
     __shared__ uint32_t s_nrec;
     __shared__ uint32_t s_dist;
+    __shared__ uint32_t s_dpass;
     __shared__ uint32_t s_abort;
+    __shared__ uint32_t s_hstart;
+    __shared__ uint32_t s_hwidth;
+    __shared__ uint32_t s_passes;
 
     const int t = threadIdx.x;
     const uint32_t lane = t & (kWave - 1);
     const uint32_t f = blockIdx.x;
     const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
     const unsigned long long n = hi - lo;
-    const uint32_t passes = n > args.pass_keys ? (uint32_t)((n + args.pass_keys - 1) / args.pass_keys) : 1u;
     const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
     uint8_t* rec_region = args.rec_tmp + lo * 8ULL * NW;
-    if (t == 0) { s_nfirst = 0; s_nrec = 0; s_dist = 0; }
+    if (t == 0) {
+        s_nfirst = 0;
+        s_nrec = 0;
+        s_dist = 0;
+        s_hstart = 0;
+        // first hash range sized by occurrences (a conservative bound on the
+        // distinct keys); later ranges by the distinct density measured so far
+        s_hwidth = n > args.pass_keys ? (uint32_t)max(1ULL, (65536ULL * args.pass_keys) / n) : 65536u;
+        s_passes = 0;
+    }
     const unsigned long long iters = (n + T - 1) / T;
 
-    for (uint32_t pass = 0; pass < passes; ++pass) {
+    for (;;) {
         for (uint32_t i = t; i < BW; i += T) s_bloom[i] = 0ULL;
         for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
         for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
         // a full spill table voids the run (the host regrows it and runs again):
         // skip the remaining work
-        if (t == 0) s_abort = (__atomic_load_n(args.error_flags, __ATOMIC_RELAXED) & kErrSpillFull) ? 1u : 0u;
+        if (t == 0) {
+            s_abort = (__atomic_load_n(args.error_flags, __ATOMIC_RELAXED) & kErrSpillFull) ? 1u : 0u;
+            s_dpass = 0;
+        }
         __syncthreads();
         if (s_abort) break;
+        const uint32_t hstart = s_hstart;
+        const uint32_t hend = min(65536u, hstart + s_hwidth);
         for (unsigned long long it = 0; it < iters; ++it) {
             const unsigned long long i = lo + it * T + t;
             bool first = false;
@@ -510,8 +530,8 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                 key = load_key<NW>(B, i);
                 uint64_t hl, hh;
                 xxh3_128(key, hl, hh);
-                const uint32_t kp = passes > 1 ? (uint32_t)(((hh >> 48) * passes) >> 16) : 0u;
-                if (kp == pass) {
+                const uint32_t hb = (uint32_t)(hh >> 48);
+                if (hb >= hstart && hb < hend) {
                     const uint32_t word = (uint32_t)(hl & (BW - 1));
                     const uint64_t mask = bloom_mask(hl, hh);
                     // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
@@ -568,13 +588,26 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                 else append_overflow<NW>(args, key, cnt);
             }
         }
-        if (dist) atomicAdd(&s_dist, dist);
+        if (dist) atomicAdd(&s_dpass, dist);
         __syncthreads();
+        if (t == 0) {
+            const uint32_t d = s_dpass;
+            s_dist += d;
+            s_passes++;
+            s_hstart = hend;
+            // next range: about kPassDistinct distinct keys at the density just seen
+            const uint32_t width = hend - hstart;
+            uint64_t next = d ? ((uint64_t)width * CountCfg<NW>::kPassDistinct) / d : 65536ULL;
+            s_hwidth = (uint32_t)(next < 1 ? 1 : (next > 65536 ? 65536 : next));
+        }
+        __syncthreads();
+        if (s_hstart >= 65536u) break;
     }
     if (t == 0) {
         args.nfirst[f] = s_nfirst;
         args.nrec[f] = (uint32_t)(s_nrec < rec_cap ? s_nrec : rec_cap);
         args.ndist[f] = s_dist;
+        if (s_passes > 1) atomicAdd(args.extra_passes, (unsigned long long)(s_passes - 1));
     }
 }
 
@@ -641,13 +674,12 @@ k_ovf_hist(const uint8_t* __restrict__ ovf, const unsigned long long* __restrict
 }
 
 // Exclusive scans of the per-bucket outputs (one workgroup of 1024 threads):
-// foff/roff over buckets, per-root record offsets (records of root r: the
-// in-bucket records of its buckets, then its overflow records), totals.
+// foff/roff over buckets (bucket order is key-range order), the overflow
+// records' root ranges after all in-bucket records, totals.
 __global__ void __launch_bounds__(1024)
 k_seg_scan(SegArgs a)
 {
     __shared__ unsigned long long s_scan[17];
-    __shared__ unsigned long long s_rin[256];
     const uint32_t t = threadIdx.x;
     const uint32_t nf = a.nf;
     const uint32_t per = (nf + 1023) / 1024;
@@ -678,30 +710,24 @@ k_seg_scan(SegArgs a)
     if (t == 1023) { a.foff[nf] = tf; a.roff[nf] = tr; }
     if (sd) atomicAdd(&a.ctr[kCtrDistinct], sd);
     if (mx) atomicMax(&a.ctr[kCtrMaxBucket], mx);
-    __syncthreads();
-    // root r owns buckets [r << sbits, (r+1) << sbits)
-    unsigned long long tot = 0;
-    if (t < 256) {
-        const unsigned long long rin = a.roff[(t + 1) << a.sbits] - a.roff[t << a.sbits];
-        s_rin[t] = rin;
-        tot = rin + a.root_ovf[t];
-    }
+    const unsigned long long o = t < 256 ? a.root_ovf[t] : 0ULL;
     unsigned long long all;
-    const unsigned long long roff_root = block_exclusive_scan<unsigned long long>(tot, s_scan, &all);
+    const unsigned long long ooff = block_exclusive_scan<unsigned long long>(o, s_scan, &all);
     if (t < 256) {
-        a.root_off[t] = roff_root;
-        a.ovf_cursor[t] = roff_root + s_rin[t];
-        if (t == 255) a.root_off[256] = all;
+        a.ovf_off[t] = tr + ooff;
+        a.ovf_cursor[t] = tr + ooff;
+        if (t == 255) a.ovf_off[256] = tr + all;
     }
     if (t == 0) {
         a.ctr[kCtrFirst] = tf;
-        a.ctr[kCtrRec] = all;
-        if (all > a.rec_out_cap) atomicOr(&a.ctr[kCtrErr], kErrRecOut);
+        a.ctr[kCtrRec] = tr + all;
+        if (tr + all > a.rec_out_cap) atomicOr(&a.ctr[kCtrErr], kErrRecOut);
     }
 }
 
 // Segmented copies of bucket f: its firsts to the contiguous firsts array,
-// then (second kernel) its records to their root's range.
+// then (second kernel) its records to [roff[f], roff[f] + nrec[f]), grouped
+// by root when the bucket's key range spans several roots.
 template <int NW>
 __global__ void __launch_bounds__(256)
 k_compact_firsts(SegArgs a, const uint64_t* __restrict__ firsts_in, uint64_t* __restrict__ firsts_out)
@@ -718,16 +744,58 @@ __global__ void __launch_bounds__(256)
 k_compact_recs(SegArgs a, const uint8_t* __restrict__ rec_in, uint8_t* __restrict__ rec_out)
 {
     constexpr uint32_t RW = RecBytes<NW>::value / 8;
+    __shared__ uint32_t s_min, s_max;
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_scan[5];
+    const uint32_t t = threadIdx.x;
     const uint32_t f = blockIdx.x;
     const uint32_t nr = a.nrec[f];
     if (nr == 0 || (a.ctr[kCtrErr] & kErrRecOut)) return;
-    const unsigned long long lo = a.fine_off[f];
-    const uint32_t r = f >> a.sbits;
-    const unsigned long long rdst = a.root_off[r] + (a.roff[f] - a.roff[(unsigned long long)r << a.sbits]);
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(rec_in) + lo * NW;
-    uint64_t* dst = reinterpret_cast<uint64_t*>(rec_out) + rdst * RW;
-    const uint32_t words = nr * RW;
-    for (uint32_t w = threadIdx.x; w < words; w += 256) dst[w] = src[w];
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(rec_in) + a.fine_off[f] * NW;
+    uint64_t* dst = reinterpret_cast<uint64_t*>(rec_out) + a.roff[f] * RW;
+    if (t == 0) { s_min = 255; s_max = 0; }
+    s_off[t] = 0;
+    __syncthreads();
+    uint32_t lmin = 255, lmax = 0;
+    for (uint32_t i = t; i < nr; i += 256) {
+        const uint32_t r = (uint32_t)(src[(size_t)i * RW] >> 56);
+        lmin = r < lmin ? r : lmin;
+        lmax = r > lmax ? r : lmax;
+    }
+    atomicMin(&s_min, lmin);
+    atomicMax(&s_max, lmax);
+    __syncthreads();
+    const uint32_t rmin = s_min, rmax = s_max;
+    if (rmin == rmax) {
+        for (uint32_t w = t; w < nr * RW; w += 256) dst[w] = src[w];
+        if (t == 0) atomicAdd(&a.root_main[rmin], (unsigned long long)nr);
+        return;
+    }
+    for (uint32_t i = t; i < nr; i += 256) atomicAdd(&s_off[(uint32_t)(src[(size_t)i * RW] >> 56) - rmin], 1u);
+    __syncthreads();
+    const uint32_t c = s_off[t];
+    uint32_t tot;
+    const uint32_t off = block_exclusive_scan<uint32_t>(c, s_scan, &tot);
+    s_off[t] = off;
+    if (c) atomicAdd(&a.root_main[rmin + t], (unsigned long long)c);
+    __syncthreads();
+    for (uint32_t i = t; i < nr; i += 256) {
+        const uint32_t r = (uint32_t)(src[(size_t)i * RW] >> 56) - rmin;
+        const uint32_t pos = atomicAdd(&s_off[r], 1u);
+#pragma unroll
+        for (uint32_t w = 0; w < RW; ++w) dst[(size_t)pos * RW + w] = src[(size_t)i * RW + w];
+    }
+}
+
+// root ranges of the in-bucket records
+__global__ void __launch_bounds__(256)
+k_root_scan(SegArgs a)
+{
+    __shared__ unsigned long long s_scan[5];
+    unsigned long long all;
+    const unsigned long long off = block_exclusive_scan<unsigned long long>(a.root_main[threadIdx.x], s_scan, &all);
+    a.root_off[threadIdx.x] = off;
+    if (threadIdx.x == 255) a.root_off[256] = all;
 }
 
 template <int NW>
@@ -840,6 +908,8 @@ hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_
     KMC_LAUNCH_CHECK();
     // records go into the buffer the firsts were just copied out of
     hipLaunchKernelGGL((k_compact_recs<NW>), dim3(sa.nf), dim3(256), 0, st, sa, rec_in, rec_out);
+    KMC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_root_scan, dim3(1), dim3(256), 0, st, sa);
     KMC_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_ovf_scatter<NW>), dim3(256), dim3(256), 0, st, ca.ovf, ca.ovf_count, ca.ovf_cap, sa.ovf_cursor, rec_out, sa.ctr);
     KMC_LAUNCH_CHECK();

@@ -1,0 +1,65 @@
+// kmc_learned.h -- learned CDF partitioner (model, training, boundaries).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace kmc {
+
+// 64 inputs (key bits) -> 64 -> 64 -> 4096 buckets
+constexpr uint32_t kMlpIn = 64;
+constexpr uint32_t kMlpH = 64;
+constexpr uint32_t kMlpC = 4096;
+constexpr uint32_t kMlpOffB1 = kMlpH * kMlpIn;
+constexpr uint32_t kMlpOffW2 = kMlpOffB1 + kMlpH;
+constexpr uint32_t kMlpOffB2 = kMlpOffW2 + kMlpH * kMlpH;
+constexpr uint32_t kMlpOffW3 = kMlpOffB2 + kMlpH;
+constexpr uint32_t kMlpOffB3 = kMlpOffW3 + kMlpC * kMlpH;
+constexpr uint32_t kMlpNParam = kMlpOffB3 + kMlpC;
+
+// fp32 master parameter i -> its bf16 copies (W1, W2 row major; W3 as [C][H] and [H][C])
+__device__ __forceinline__ void write_bf16_copy(uint32_t i, float v, __bf16* w1b, __bf16* w2b, __bf16* w3b, __bf16* w3tb)
+{
+    if (i < kMlpOffB1) {
+        w1b[i] = (__bf16)v;
+    } else if (i >= kMlpOffW2 && i < kMlpOffB2) {
+        w2b[i - kMlpOffW2] = (__bf16)v;
+    } else if (i >= kMlpOffW3 && i < kMlpOffB3) {
+        const uint32_t j = i - kMlpOffW3, c = j / kMlpH, h = j % kMlpH;
+        w3b[j] = (__bf16)v;
+        w3tb[h * kMlpC + c] = (__bf16)v;
+    }
+}
+
+struct MlpTrainOpts {
+    uint32_t steps = 10000;
+    uint32_t batch = 2048;  // multiple of 64
+    float lr = 1e-2f;       // Adam, cosine decay to zero
+    uint64_t seed = 1;
+};
+
+void mlp_init_params(uint64_t seed, std::vector<float>& p);
+int mlp_train(const uint64_t* d_sorted, uint64_t n, const MlpTrainOpts& o, hipStream_t st, std::vector<float>& params,
+              std::vector<float>& loss_curve, std::string& err);
+int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, hipStream_t st, std::vector<uint16_t>& pred,
+                std::string& err);
+int mlp_debug_loss_grad(const float* params, const uint64_t* keys, const uint32_t* labels, uint32_t nb, float* grad,
+                        float* loss, std::string& err);
+int mlp_debug_fit(const uint64_t* sorted, uint64_t n, const MlpTrainOpts& o, uint64_t* bounds, float* loss_curve,
+                  uint32_t loss_cap, std::string& err);
+int mlp_debug_gemm(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, uint32_t N, uint32_t K, std::string& err);
+
+// Monotone bucket boundaries from predictions on sorted keys (isotonic fit of
+// prediction against key order): b[0] = 0, b[j] = first key whose fitted
+// prediction reaches j (j = 1..4095), b[4096] = UINT64_MAX.  Bucket j is
+// [b[j], b[j+1]).
+void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std::vector<uint16_t>& pred,
+                                 std::vector<uint64_t>& b);
+// Coarse (255) then per-coarse fine (256 x (2^sbits - 1)) routing boundaries
+// from the 4097 learned boundaries; fine ones interpolate inside a learned bucket.
+void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<uint64_t>& out);
+
+}  // namespace kmc

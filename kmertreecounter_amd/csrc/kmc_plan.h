@@ -18,10 +18,16 @@ struct PlanResult {
     uint64_t bloom_capacity[256] = {};
     uint8_t owners[256] = {};
     uint32_t sbits = 0;
-    // learned partition: 255 coarse boundaries followed by 256 x 255 fine
-    // boundaries on key word 0 (empty when the plan is a prefix plan)
-    std::vector<uint64_t> bounds;
-    double train_loss = 0.0;
+    // learned partition: 4097 bucket boundaries on key word 0 (bucket j is
+    // [b[j], b[j+1])); empty for a prefix plan
+    std::vector<uint64_t> learned_b;
+    std::vector<float> loss_curve;      // training loss every 50 steps
+    uint64_t train_keys = 0;            // sorted sample keys the model saw
+    double train_ms = 0.0;
+    double sample_max_over_mean_learned = 0.0;  // over the 4096 learned buckets, on the sample
+    double sample_max_over_mean_prefix = 0.0;   // over the 4096 12-bit prefixes, on the sample
+    double sample_distinct_learned = 0.0;       // the same, counting distinct keys per bucket
+    double sample_distinct_prefix = 0.0;
 };
 
 // Reference Stage-0 steps restated for the GPU host (src/main.cpp:173-189).
@@ -31,5 +37,7 @@ void bloom_capacity(const uint64_t* counts256, uint64_t est_file_size, uint64_t*
 
 int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sample, hipStream_t st,
                 PlanResult& out, std::string& err);
+// Train the CDF model on the sample's keys and fill out.learned_b (and its stats).
+int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, PlanResult& out, std::string& err);
 
 }  // namespace kmc

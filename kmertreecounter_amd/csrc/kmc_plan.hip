@@ -7,7 +7,10 @@
 #include <cstring>
 #include <numeric>
 
+#include <chrono>
+
 #include "kmc_kernels.h"
+#include "kmc_learned.h"
 
 namespace kmc {
 
@@ -90,6 +93,111 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
     clamp_prefix_counts(out.prefix_counts);
     lpt_assign(out.prefix_counts, cfg.classifier_num ? cfg.classifier_num : 1, out.owners);
     bloom_capacity(out.prefix_counts, cfg.expected_bytes ? cfg.expected_bytes : sample.size(), out.bloom_capacity);
+    return KC_OK;
+}
+
+namespace {
+
+// max/mean of bucket sizes, buckets given by sorted lower bounds over sorted keys
+double imbalance(const std::vector<uint64_t>& sorted, const std::vector<uint64_t>& lower_bounds)
+{
+    const size_t nb = lower_bounds.size();
+    std::vector<uint64_t> pos(nb + 1);
+    for (size_t j = 0; j < nb; ++j)
+        pos[j] = std::lower_bound(sorted.begin(), sorted.end(), lower_bounds[j]) - sorted.begin();
+    pos[nb] = sorted.size();
+    uint64_t mx = 0;
+    for (size_t j = 0; j < nb; ++j) mx = std::max<uint64_t>(mx, pos[j + 1] - pos[j]);
+    const double mean = (double)sorted.size() / (double)nb;
+    return mean > 0 ? (double)mx / mean : 0.0;
+}
+
+}  // namespace
+
+int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, PlanResult& out, std::string& err)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t nw = k <= 32 ? 1u : 2u;
+    const uint64_t tile = (nw == 1 ? ExtractCfg<1>::kThreads : ExtractCfg<2>::kThreads) * 16ULL;
+    const uint64_t padded = ((sample.size() + tile - 1) / tile) * tile;
+    if (padded == 0) { err = "learned plan: empty Stage-0 sample"; return KC_ERR_STATE; }
+    // sample keys on the device (root-grouped, the order does not matter here)
+    uint8_t* d_seq = nullptr;
+    unsigned long long* d_small = nullptr;
+    uint64_t* d_keys = nullptr;
+    auto cleanup = [&]() { hipFree(d_seq); hipFree(d_small); hipFree(d_keys); };
+    if (hipMalloc(&d_seq, padded + 256) != hipSuccess || hipMalloc(&d_small, sizeof(unsigned long long) * (16 * 256 + 600)) != hipSuccess ||
+        hipMalloc(&d_keys, padded * 8ULL * nw) != hipSuccess) {
+        cleanup();
+        err = "learned plan: device allocation failed";
+        return KC_ERR_NOMEM;
+    }
+    unsigned long long* counts = d_small;
+    unsigned long long* cursor = d_small + 8 * 256;
+    unsigned long long* coff = d_small + 16 * 256;
+    unsigned long long* tstart = coff + 257;
+    Route rt{};
+    hipError_t e = hipMemcpyAsync(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_small, 0, sizeof(unsigned long long) * (16 * 256 + 600), st);
+    if (e == hipSuccess)
+        e = nw == 1 ? launch_extract<1>(false, d_seq, padded, k, rt, counts, cursor, nullptr, st)
+                    : launch_extract<2>(false, d_seq, padded, k, rt, counts, cursor, nullptr, st);
+    if (e == hipSuccess) e = launch_scan_coarse(counts, coff, cursor, tstart, 4096, st);
+    if (e == hipSuccess)
+        e = nw == 1 ? launch_extract<1>(true, d_seq, padded, k, rt, counts, cursor, d_keys, st)
+                    : launch_extract<2>(true, d_seq, padded, k, rt, counts, cursor, d_keys, st);
+    unsigned long long nkeys = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nkeys, coff + 256, sizeof(nkeys), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    std::vector<uint64_t> all;
+    if (e == hipSuccess && nkeys) {
+        all.resize(nkeys * nw);
+        e = hipMemcpy(all.data(), d_keys, nkeys * 8ULL * nw, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) {
+        cleanup();
+        err = std::string("learned plan: ") + hipGetErrorString(e);
+        return KC_ERR_HIP;
+    }
+    // word 0 of every key, at most 4M of them (evenly strided), sorted
+    const uint64_t cap = 4ULL << 20;
+    const uint64_t stride = nkeys > cap ? (nkeys + cap - 1) / cap : 1;
+    std::vector<uint64_t> sorted;
+    sorted.reserve(nkeys / stride + 1);
+    for (uint64_t i = 0; i < nkeys; i += stride) sorted.push_back(all[i * nw]);
+    std::vector<uint64_t>().swap(all);
+    std::sort(sorted.begin(), sorted.end());
+    if (sorted.size() < 64) {
+        cleanup();
+        err = "learned plan: too few sample keys";
+        return KC_ERR_STATE;
+    }
+    e = hipMemcpy(d_keys, sorted.data(), sorted.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        cleanup();
+        err = std::string("learned plan: ") + hipGetErrorString(e);
+        return KC_ERR_HIP;
+    }
+    MlpTrainOpts o;
+    std::vector<float> params;
+    int rc = mlp_train(d_keys, sorted.size(), o, st, params, out.loss_curve, err);
+    std::vector<uint16_t> pred;
+    if (!rc) rc = mlp_predict(params.data(), d_keys, sorted.size(), st, pred, err);
+    cleanup();
+    if (rc) return KC_ERR_HIP;
+    boundaries_from_predictions(sorted, pred, out.learned_b);
+    out.train_keys = sorted.size();
+    std::vector<uint64_t> lb(out.learned_b.begin(), out.learned_b.begin() + kMlpC);
+    out.sample_max_over_mean_learned = imbalance(sorted, lb);
+    std::vector<uint64_t> pb(kMlpC);
+    for (uint32_t j = 0; j < kMlpC; ++j) pb[j] = (uint64_t)j << 52;
+    out.sample_max_over_mean_prefix = imbalance(sorted, pb);
+    std::vector<uint64_t> uniq(sorted);
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    out.sample_distinct_learned = imbalance(uniq, lb);
+    out.sample_distinct_prefix = imbalance(uniq, pb);
+    out.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return KC_OK;
 }
 

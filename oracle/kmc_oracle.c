@@ -35,6 +35,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 
 #define ORC_API __attribute__((visibility("default")))
 
@@ -757,4 +758,92 @@ ORC_API void orc_bloom_capacity(const uint64_t* counts256, uint64_t est_file_siz
         uint64_t c = bit_ceil64(est);
         caps256[i] = c > (1ULL << 15) ? c : (1ULL << 15);
     }
+}
+
+/* ------------------------------------------------------------------------- */
+/* A16 (learned partitioner, no reference code): the MLP that maps a key to   */
+/* one of 4096 CDF buckets.  fp64 restatement of the loss and its gradient,   */
+/* for checking the device training kernels.                                   */
+/*   input x_j = +1 if bit (63-j) of key word 0 is set else -1 (64 inputs)    */
+/*   h1 = relu(W1 x + b1) (64), h2 = relu(W2 h1 + b2) (64),                   */
+/*   z = W3 h2 + b3 (4096), loss = mean over the batch of -log softmax(z)[y]   */
+/* params layout (floats): W1[64][64] b1[64] W2[64][64] b2[64] W3[4096][64]   */
+/* b3[4096]; 274560 values.  Gradients use the same layout.                   */
+/* ------------------------------------------------------------------------- */
+
+#define MLP_IN 64
+#define MLP_H 64
+#define MLP_C 4096
+#define MLP_OFF_B1 (MLP_H * MLP_IN)
+#define MLP_OFF_W2 (MLP_OFF_B1 + MLP_H)
+#define MLP_OFF_B2 (MLP_OFF_W2 + MLP_H * MLP_H)
+#define MLP_OFF_W3 (MLP_OFF_B2 + MLP_H)
+#define MLP_OFF_B3 (MLP_OFF_W3 + MLP_C * MLP_H)
+#define MLP_NPARAM (MLP_OFF_B3 + MLP_C)
+
+ORC_API uint32_t orc_mlp_nparam(void) { return MLP_NPARAM; }
+
+ORC_API double orc_mlp_loss_grad(const double* p, const uint64_t* keys, const uint32_t* labels, uint32_t nb,
+                                 double* grad)
+{
+    if (grad) memset(grad, 0, sizeof(double) * MLP_NPARAM);
+    const double* W1 = p;
+    const double* b1 = p + MLP_OFF_B1;
+    const double* W2 = p + MLP_OFF_W2;
+    const double* b2 = p + MLP_OFF_B2;
+    const double* W3 = p + MLP_OFF_W3;
+    const double* b3 = p + MLP_OFF_B3;
+    double* z = (double*)malloc(sizeof(double) * MLP_C);
+    double loss = 0.0;
+    for (uint32_t s = 0; s < nb; ++s) {
+        double x[MLP_IN], a1[MLP_H], h1[MLP_H], a2[MLP_H], h2[MLP_H];
+        for (int j = 0; j < MLP_IN; ++j) x[j] = ((keys[s] >> (63 - j)) & 1ULL) ? 1.0 : -1.0;
+        for (int i = 0; i < MLP_H; ++i) {
+            double acc = b1[i];
+            for (int j = 0; j < MLP_IN; ++j) acc += W1[i * MLP_IN + j] * x[j];
+            a1[i] = acc;
+            h1[i] = acc > 0 ? acc : 0;
+        }
+        for (int i = 0; i < MLP_H; ++i) {
+            double acc = b2[i];
+            for (int j = 0; j < MLP_H; ++j) acc += W2[i * MLP_H + j] * h1[j];
+            a2[i] = acc;
+            h2[i] = acc > 0 ? acc : 0;
+        }
+        double mx = -1e300;
+        for (int c = 0; c < MLP_C; ++c) {
+            double acc = b3[c];
+            for (int j = 0; j < MLP_H; ++j) acc += W3[c * MLP_H + j] * h2[j];
+            z[c] = acc;
+            if (acc > mx) mx = acc;
+        }
+        double se = 0;
+        for (int c = 0; c < MLP_C; ++c) se += exp(z[c] - mx);
+        loss += log(se) + mx - z[labels[s]];
+        if (!grad) continue;
+        double dh2[MLP_H] = {0}, da2[MLP_H], dh1[MLP_H] = {0}, da1[MLP_H];
+        for (int c = 0; c < MLP_C; ++c) {
+            const double dz = (exp(z[c] - mx) / se - (c == (int)labels[s] ? 1.0 : 0.0)) / nb;
+            grad[MLP_OFF_B3 + c] += dz;
+            for (int j = 0; j < MLP_H; ++j) {
+                grad[MLP_OFF_W3 + c * MLP_H + j] += dz * h2[j];
+                dh2[j] += dz * W3[c * MLP_H + j];
+            }
+        }
+        for (int i = 0; i < MLP_H; ++i) da2[i] = a2[i] > 0 ? dh2[i] : 0;
+        for (int i = 0; i < MLP_H; ++i) {
+            grad[MLP_OFF_B2 + i] += da2[i];
+            for (int j = 0; j < MLP_H; ++j) {
+                grad[MLP_OFF_W2 + i * MLP_H + j] += da2[i] * h1[j];
+                dh1[j] += da2[i] * W2[i * MLP_H + j];
+            }
+        }
+        for (int i = 0; i < MLP_H; ++i) da1[i] = a1[i] > 0 ? dh1[i] : 0;
+        for (int i = 0; i < MLP_H; ++i) {
+            grad[MLP_OFF_B1 + i] += da1[i];
+            for (int j = 0; j < MLP_IN; ++j) grad[i * MLP_IN + j] += da1[i] * x[j];
+        }
+    }
+    free(z);
+    return loss / nb;
 }

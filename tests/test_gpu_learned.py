@@ -1,0 +1,98 @@
+"""GPU tests of the learned partitioner: the MFMA product's lane layout, the
+device loss/gradient against the oracle's fp64 gradient, and the learned
+partition's balance and exactness."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import kmertreecounter_amd as K
+from tests import common as C
+from tests.test_oracle import mlp_oracle_loss_grad
+
+pytestmark = pytest.mark.gpu
+
+
+def to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    """float32 -> bf16 bit patterns, round to nearest even."""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16_bits(h: np.ndarray) -> np.ndarray:
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("m,n,kk", [(64, 64, 16), (128, 192, 64), (64, 4096, 64)])
+def test_mfma_gemm_layout(m, n, kk):
+    rng = np.random.default_rng(m + n + kk)
+    a = to_bf16_bits(rng.normal(size=(m, kk)).astype(np.float32))
+    b = to_bf16_bits(rng.normal(size=(n, kk)).astype(np.float32))
+    got = K.debug_gemm_bf16(a, b)
+    ref = from_bf16_bits(a).astype(np.float64) @ from_bf16_bits(b).astype(np.float64).T
+    # bf16 inputs are exact in fp32; only the fp32 accumulation order differs
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-4)
+
+
+def test_mlp_gradient_matches_oracle():
+    rng = np.random.default_rng(7)
+    n = K.MLP_NPARAM
+    p = (rng.normal(0, 0.2, size=n)).astype(np.float32)
+    # the device multiplies with bf16 weights: give the oracle the same rounded weights
+    pr = p.copy()
+    for lo, hi in ((0, 4096), (4160, 8256), (8320, 270464)):
+        pr[lo:hi] = from_bf16_bits(to_bf16_bits(p[lo:hi]))
+    keys = rng.integers(0, 2**62, size=128, dtype=np.uint64) << np.uint64(2)
+    labels = rng.integers(0, 4096, size=128).astype(np.uint32)
+    loss, g = K.debug_mlp_grad(p, keys, labels)
+    rloss, rg = mlp_oracle_loss_grad(pr.astype(np.float64), keys, labels)
+    assert abs(loss - rloss) <= 2e-3 * rloss
+    # tolerance: h2 and dz are rounded to bf16 for the MFMA products (relative
+    # error <= 2^-8 each), accumulation in fp32
+    for name, lo, hi in (("W1", 0, 4096), ("b1", 4096, 4160), ("W2", 4160, 8256), ("b2", 8256, 8320),
+                         ("W3", 8320, 270464), ("b3", 270464, 274560)):
+        d = np.linalg.norm(g[lo:hi] - rg[lo:hi])
+        r = np.linalg.norm(rg[lo:hi])
+        assert d <= 3e-2 * r + 1e-7, (name, d, r)
+
+
+def learned_engine_run(seq: bytes, k: int, partition: int, sample_frac: float = 0.01, **kw):
+    e = K.Engine(k, partition=partition, expected_bytes=len(seq), **kw)
+    e.sample_block(seq[: max(1, int(len(seq) * sample_frac))])
+    e.plan()
+    e.push_block(seq)
+    e.finish()
+    return e
+
+
+def test_learned_partition_balances_skewed_input_and_stays_exact():
+    # workload 3 shape: 65% GC plus microsatellite tracts, the prefix partition's worst case
+    seq = K.generate(3, scale=100, seed=31, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    res = {}
+    for part in (K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED):
+        e = learned_engine_run(seq, 31, part, sample_frac=0.05)
+        f, r, s = e.firsts(), e.records(), e.stats()
+        mk, mc = C.merge_outputs(f, r, 1)
+        np.testing.assert_array_equal(mk, keys)
+        np.testing.assert_array_equal(mc, counts)
+        for rid in range(256):
+            rr = e.root(rid)
+            if len(rr):
+                assert (rr["key"][:, 0] >> np.uint64(56) == rid).all()
+        res[part] = s
+        e.close()
+    sl = res[K.KC_PARTITION_LEARNED]
+    sp = res[K.KC_PARTITION_PREFIX]
+    print(f"learned: loss {sl.learned_loss_first:.3f} -> {sl.learned_loss_last:.3f}, sample max/mean "
+          f"{sl.learned_sample_max_over_mean:.2f} (distinct {sl.learned_sample_distinct_max_over_mean:.2f}) vs prefix "
+          f"{sl.prefix_sample_max_over_mean:.2f} (distinct {sl.prefix_sample_distinct_max_over_mean:.2f}); fine buckets "
+          f"max/mean learned {sl.max_bucket_keys / sl.mean_bucket_keys:.2f} prefix {sp.max_bucket_keys / sp.mean_bucket_keys:.2f}; "
+          f"extra passes learned {sl.extra_passes} prefix {sp.extra_passes}")
+    assert sl.learned_loss_last < 0.5 * sl.learned_loss_first
+    # what the trained network gains over the prefix partition: the heaviest
+    # single keys bound max/mean by occurrences for any key-range partition, so
+    # balance is also compared over distinct keys
+    assert sl.learned_sample_max_over_mean <= sl.prefix_sample_max_over_mean
+    assert sl.learned_sample_distinct_max_over_mean < 1.5
+    assert sl.learned_sample_distinct_max_over_mean < 0.5 * sl.prefix_sample_distinct_max_over_mean

@@ -309,3 +309,42 @@ def test_oracle_matches_reference_runs(name):
     assert C.table_sha(keys, counts) == fx["sha256"]
     h = C.histogram(counts, 1, fx["max_freq"])
     assert {str(i + 1): int(v) for i, v in enumerate(h) if v} == fx["histogram_nonzero"]
+
+
+# ------------------------------------------------------------------ learned partitioner (MLP)
+
+def mlp_oracle_loss_grad(params: np.ndarray, keys: np.ndarray, labels: np.ndarray, grad: bool = True):
+    O.orc_mlp_loss_grad.restype = ctypes.c_double
+    O.orc_mlp_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    y = np.ascontiguousarray(labels, dtype=np.uint32)
+    g = np.zeros(p.size, dtype=np.float64) if grad else None
+    loss = O.orc_mlp_loss_grad(p.ctypes.data, k.ctypes.data, y.ctypes.data, k.size, g.ctypes.data if grad else None)
+    return loss, g
+
+
+def test_mlp_oracle_zero_weights_and_finite_differences():
+    O.orc_mlp_nparam.restype = ctypes.c_uint32
+    n = O.orc_mlp_nparam()
+    assert n == 274560
+    rng = np.random.default_rng(0)
+    keys = rng.integers(0, 2**63, size=8, dtype=np.uint64)
+    labels = rng.integers(0, 4096, size=8).astype(np.uint32)
+    # all-zero weights: uniform softmax over 4096 buckets
+    loss0, _ = mlp_oracle_loss_grad(np.zeros(n), keys, labels, grad=False)
+    assert abs(loss0 - np.log(4096)) < 1e-12
+    # random weights: central differences on parameters from every block
+    p = rng.normal(0, 0.3, size=n)
+    loss, g = mlp_oracle_loss_grad(p, keys, labels)
+    blocks = [(0, 4096), (4096, 4160), (4160, 8256), (8256, 8320), (8320, 270464), (270464, 274560)]
+    h = 1e-6
+    for lo, hi in blocks:
+        for i in rng.integers(lo, hi, size=6):
+            q = p.copy()
+            q[i] += h
+            lp, _ = mlp_oracle_loss_grad(q, keys, labels, grad=False)
+            q[i] -= 2 * h
+            lm, _ = mlp_oracle_loss_grad(q, keys, labels, grad=False)
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (lo, i, fd, g[i])
