@@ -1,17 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the k-mer counting path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2] [--partition prefix|learned]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2] [--partition both|prefix|learned]
 
 A step is one pass of the whole device pipeline (extraction, partition,
 Bloom split, repeat counting, record grouping) over the resident synthetic
 input of the workload; the sequence is generated on the host and copied to HBM
-before timing.  Workload 2 is the at-scale plain path: a 100 Mbp genome at
+before timing.  By default both partition methods run on the same input: the
+fixed 256-prefix split, then the learned CDF partition (its network trained on
+the GPU from the first 1% of the reads); `value` is the learned path's
+k-mers/s over the wall clock of the timed steps, and `learned_over_prefix`
+compares the two.  Workload 2 is the at-scale plain path: a 100 Mbp genome at
 45% GC with 10% of it in 40 families x 50 copies of 5 kbp repeats, 20M reads
 of 150 bp, 0.5% substitutions, 0.1% N, k=31 (about 2.3 billion k-mers).
 
 Prints one JSON line (see the keys below).  The device time of each kernel
-comes from HIP events the engine records on its own stream around each launch.
+comes from HIP events the engine records on its own stream around each launch
+(pipeline.phase_ms); roofline.achieved is the dominant kernel's algorithmic
+bytes over that time.
 cpu_baseline times the reference `Tree` binary (oracle/_ref, compiled from the
 reference sources) on a bounded sample of the same workload on the host cores;
 where that binary is absent it times the oracle's C restatement instead.
@@ -77,16 +83,20 @@ def phase_ms(st) -> dict:
     }
 
 
-KERNEL_NAMES = {
-    "extract_count": "k_extract<1,false>", "extract_scatter": "k_extract<1,true>",
-    "sub_count": "k_sub<1,false>", "sub_scatter": "k_sub<1,true>", "count": "k_count<1>",
-    "group_records": "k_rec_group (x2) + k_rec_scan",
-}
+PHASES = ("extract_count", "extract_scatter", "sub_count", "sub_scatter", "count", "group_records")
 
 
-def traffic_from_profiles(kernel: str, workload: int) -> float | None:
+def kernel_names(nw: int) -> dict:
+    return {
+        "extract_count": f"k_extract<{nw}, false>", "extract_scatter": f"k_extract<{nw}, true>",
+        "sub_count": f"k_sub<{nw}, false>", "sub_scatter": f"k_sub<{nw}, true>", "count": f"k_count<{nw}>",
+        "group_records": f"k_spill_emit<{nw}> + k_seg_scan + k_compact_firsts<{nw}> + k_compact_recs<{nw}> + k_root_scan",
+    }
+
+
+def traffic_from_profiles(kernel: str, workload: int, partition: str) -> float | None:
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
-    p = ROOT / "profiles" / f"traffic_w{workload}.json"
+    p = ROOT / "profiles" / f"traffic_w{workload}_{partition}.json"
     if not p.exists():
         return None
     try:
@@ -163,6 +173,74 @@ def cpu_baseline_port(workload: int, k: int) -> dict:
             "sample": f"workload {workload} at 1/20 scale ({n} k-mers), oracle extract + sort + run-length, {dt:.2f} s"}
 
 
+def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, counter: str) -> dict:
+    """Resident run of one partition method: plan (learned only), copy, warm up, time `steps` passes."""
+    part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
+    cnt = K.KC_COUNTER_BLOOM if counter == "bloom" else K.KC_COUNTER_EXACT
+    eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt)
+    plan_ms = 0.0
+    if part == K.KC_PARTITION_LEARNED:
+        # Stage 0 sample: the first 1% of the reads; the network trains on the GPU
+        t0 = time.time()
+        eng.sample_block(seq[: len(seq) // 100])
+        eng.plan()
+        plan_ms = (time.time() - t0) * 1e3
+    t0 = time.time()
+    eng.push_block(seq)
+    eng.finish()  # H2D copy + first full pass (also the first warmup)
+    st = eng.stats()
+    log(f"[{partition}] resident in {time.time() - t0:.1f} s; K={st.total_kmers} firsts={st.firsts} "
+        f"records={st.records} spilled={st.spilled_keys} fine_buckets={st.fine_buckets} "
+        f"max/mean={st.max_bucket_keys / max(st.mean_bucket_keys, 1):.2f} extra_passes={st.extra_passes}")
+    for _ in range(max(0, warmup - 1)):
+        eng.count_resident()
+    per_phase = {name: 0.0 for name in PHASES}
+    dev = []
+    t_wall = time.perf_counter()
+    for _ in range(steps):
+        eng.count_resident()  # synchronous: returns after the device pipeline and its readback
+        st = eng.stats()
+        dev.append(st.ms_total)
+        for name, v in phase_ms(st).items():
+            per_phase[name] += v / steps
+    wall_ms = (time.perf_counter() - t_wall) * 1e3 / steps
+    log(f"[{partition}] {steps} steps: {wall_ms:.3f} ms/step wall, device " + ", ".join(f"{v:.3f}" for v in dev))
+    r = {
+        "partition": partition,
+        "ms_per_step": wall_ms,
+        "device_ms_per_step": sum(dev) / len(dev),
+        "kmers_per_s": st.total_kmers / (wall_ms * 1e-3),
+        "stats": st,
+        "per_phase": per_phase,
+        "plan_ms": plan_ms,
+        "max_over_mean_bucket": st.max_bucket_keys / max(st.mean_bucket_keys, 1e-9),
+    }
+    if part == K.KC_PARTITION_LEARNED:
+        r["learned"] = {
+            "plan_ms": plan_ms,
+            "train_keys": st.learned_train_keys,
+            "train_ms": st.learned_train_ms,
+            "loss_first": st.learned_loss_first,
+            "loss_last": st.learned_loss_last,
+            "sample_max_over_mean": {"learned": st.learned_sample_max_over_mean,
+                                     "prefix": st.prefix_sample_max_over_mean},
+            "sample_distinct_max_over_mean": {"learned": st.learned_sample_distinct_max_over_mean,
+                                              "prefix": st.prefix_sample_distinct_max_over_mean},
+        }
+    eng.close()
+    return r
+
+
+def summary(r: dict) -> dict:
+    st = r["stats"]
+    return {
+        "kmers_per_s": r["kmers_per_s"], "ms_per_step": r["ms_per_step"],
+        "device_ms_per_step": r["device_ms_per_step"], "fine_buckets": st.fine_buckets,
+        "max_over_mean_bucket": r["max_over_mean_bucket"], "extra_passes": st.extra_passes,
+        "spilled_keys": st.spilled_keys, "phase_ms": r["per_phase"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +248,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS))
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--partition", choices=["prefix", "learned"], default="prefix")
+    ap.add_argument("--partition", choices=["prefix", "learned", "both"], default="both")
+    ap.add_argument("--counter", choices=["bloom", "exact"], default="bloom")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -184,42 +263,21 @@ def main() -> int:
     threads = min(16, os.cpu_count() or 1)
     t0 = time.time()
     seq = K.generate(args.workload, scale=args.scale, seed=args.seed, fmt="seq", threads=threads)
-    log(f"generated {len(seq) / 1e9:.2f} GB of sequence in {time.time() - t0:.1f} s")
-    part = K.KC_PARTITION_LEARNED if args.partition == "learned" else K.KC_PARTITION_PREFIX
-    eng = K.Engine(k, partition=part, expected_bytes=len(seq))
-    if part == K.KC_PARTITION_LEARNED:
-        # Stage 0 sample: the first 1% of the reads
-        eng.sample_block(seq[: len(seq) // 100])
-        eng.plan()
-    t0 = time.time()
-    eng.push_block(seq)
     seq_bytes = len(seq)
+    log(f"generated {seq_bytes / 1e9:.2f} GB of sequence in {time.time() - t0:.1f} s")
+    parts = ["prefix", "learned"] if args.partition == "both" else [args.partition]
+    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter) for p in parts}
     del seq
-    eng.finish()  # H2D copy + first full run (also the first warmup)
-    st = eng.stats()
-    log(f"resident in {time.time() - t0:.1f} s; K={st.total_kmers} firsts={st.firsts} records={st.records} "
-        f"spilled={st.spilled_keys} fine_buckets={st.fine_buckets} max/mean={st.max_bucket_keys / max(st.mean_bucket_keys, 1):.2f}")
-    for _ in range(max(0, args.warmup - 1)):
-        eng.count_resident()
-    per_phase = {name: 0.0 for name in KERNEL_NAMES}
-    totals = []
-    t_wall = time.time()
-    for _ in range(args.steps):
-        eng.count_resident()
-        st = eng.stats()
-        totals.append(st.ms_total)
-        for name, v in phase_ms(st).items():
-            per_phase[name] += v / args.steps
-        log(f"step: {st.ms_total:.3f} ms device")
-    wall = (time.time() - t_wall) / args.steps
-    ms = sum(totals) / len(totals)
-    value = st.total_kmers / (ms * 1e-3)
+    main_run = runs["learned"] if "learned" in runs else runs["prefix"]
+    st = main_run["stats"]
+    per_phase = main_run["per_phase"]
+    names = kernel_names(st.key_words)
 
     alg = algorithmic_bytes(st, seq_bytes + (-(seq_bytes) % (512 * 16)))
     dominant = max(per_phase, key=per_phase.get)
     achieved = alg[dominant] / (per_phase[dominant] * 1e-3) / 1e9
     total_alg = sum(alg.values())
-    traffic = traffic_from_profiles(dominant, args.workload)
+    traffic = traffic_from_profiles(dominant, args.workload, main_run["partition"])
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -232,12 +290,12 @@ def main() -> int:
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_run["kmers_per_s"],
         "unit": "k-mers/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": main_run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -245,19 +303,20 @@ def main() -> int:
         "data": "synthetic (deterministic generator, seed %d)" % args.seed,
         "config": {
             "workload": desc + (f" at 1/{args.scale:g} scale" if args.scale != 1 else ""),
-            "partition": args.partition,
+            "partition": main_run["partition"],
+            "counter": args.counter,
             "k": k,
             "kmers": st.total_kmers,
             "distinct_first": st.firsts,
             "root_records": st.records,
             "sequence_bytes": seq_bytes,
             "fine_buckets": st.fine_buckets,
-            "max_over_mean_bucket": st.max_bucket_keys / max(st.mean_bucket_keys, 1e-9),
+            "max_over_mean_bucket": main_run["max_over_mean_bucket"],
             "spilled_keys": st.spilled_keys,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": KERNEL_NAMES[dominant],
+            "kernel": names[dominant],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -265,15 +324,20 @@ def main() -> int:
             "traffic": traffic,
         },
         "pipeline": {
-            "hbm_frac": total_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "hbm_frac": total_alg / (main_run["device_ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "algorithmic_bytes": total_alg,
+            "device_ms_per_step": main_run["device_ms_per_step"],
             "phase_ms": per_phase,
             "phase_gbs": {n: (alg[n] / (per_phase[n] * 1e-3) / 1e9 if per_phase[n] > 0 else None) for n in alg},
-            "host_wall_ms_per_step": wall * 1e3,
+            "kernels": names,
         },
+        "partitions": {p: summary(r) for p, r in runs.items()},
         "cpu_baseline": cpu,
     }
-    eng.close()
+    if "learned" in runs:
+        out["learned"] = runs["learned"]["learned"]
+    if len(runs) == 2:
+        out["learned_over_prefix"] = runs["learned"]["kmers_per_s"] / runs["prefix"]["kmers_per_s"]
     print(json.dumps(out), flush=True)
     return 0
 

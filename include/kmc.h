@@ -68,6 +68,20 @@ typedef enum kc_partition {
 
 #define KC_SUB_BITS_NONE 0xFFFFFFFFu
 
+/* Repeat counting inside a bucket.  Both give the same merged table (the
+ * low.bin keys plus the root records).  BLOOM (the default) runs the
+ * reference's first-occurrence Bloom test (XXH3-128, three bits per key) and
+ * counts only the repeats, so a Bloom false positive moves a key's first
+ * occurrence into its record, as in the reference.  EXACT counts every
+ * occurrence in the bucket's table and splits each key into one low.bin entry
+ * and a root record of the remaining count (the reference's split when its
+ * filter reports no false positive); its table must hold every distinct key
+ * of a pass, not only the repeated ones, so it takes more passes. */
+typedef enum kc_counter {
+    KC_COUNTER_BLOOM = 0,
+    KC_COUNTER_EXACT = 1
+} kc_counter;
+
 typedef struct kc_config {
     uint32_t k;               /* k-mer length, 1..63 */
     uint32_t partition;       /* kc_partition */
@@ -78,8 +92,13 @@ typedef struct kc_config {
                                  or KC_SUB_BITS_NONE for no sub-partition */
     uint32_t device;          /* HIP device ordinal */
     uint32_t classifier_num;  /* LPT owner count reported by kc_plan (reference: (n-2-n/8)/2) */
-    uint32_t pass_keys;       /* keys per counting pass of a bucket (0 = default; larger buckets take
-                                 several hash-range passes) */
+    uint32_t pass_keys;       /* Bloom counter: keys per counting pass of a bucket (0 = default; larger
+                                 buckets take several hash-range passes).  Exact counter: distinct keys
+                                 a pass aims to hold (0 = the table size) */
+    uint32_t counter;         /* kc_counter */
+    uint32_t table_slots;     /* exact counter: count-table slots per bucket and pass (0 = the whole
+                                 LDS table, 10240 for k <= 32 and 6144 above; smaller values narrow
+                                 each pass and are meant for tuning and tests) */
 } kc_config;
 
 typedef struct kc_stats {

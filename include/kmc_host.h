@@ -52,6 +52,8 @@ typedef struct kmc_tree_opts {
     uint32_t partition;     /* kc_partition */
     uint32_t verbose;
     uint64_t sample_bytes;  /* Stage-0 sample size (0 = 256 MiB, the reference's cap) */
+    uint32_t counter;       /* kc_counter */
+    uint32_t reserved;
 } kmc_tree_opts;
 
 KC_API int kmc_tree_run(const kmc_tree_opts* opts, kc_stats* stats, double* wall_ms);

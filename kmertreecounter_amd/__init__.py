@@ -27,6 +27,8 @@ TREE_PATH = PKG / "kmc_tree"
 KC_PARTITION_PREFIX = 0
 KC_PARTITION_LEARNED = 1
 KC_SUB_BITS_NONE = 0xFFFFFFFF
+KC_COUNTER_BLOOM = 0
+KC_COUNTER_EXACT = 1
 
 
 class KmcError(RuntimeError):
@@ -39,7 +41,7 @@ class Config(ctypes.Structure):
         ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
         ("expected_bytes", ctypes.c_uint64), ("sub_bits", ctypes.c_uint32),
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
-        ("pass_keys", ctypes.c_uint32),
+        ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
     ]
 
 
@@ -86,6 +88,7 @@ class TreeOpts(ctypes.Structure):
         ("fastq_path", ctypes.c_char_p), ("tmp_dir", ctypes.c_char_p), ("k", ctypes.c_uint32),
         ("n_thread", ctypes.c_uint32), ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
         ("partition", ctypes.c_uint32), ("verbose", ctypes.c_uint32), ("sample_bytes", ctypes.c_uint64),
+        ("counter", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -212,11 +215,12 @@ class Engine:
 
     def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
-                 pass_keys: int = 0):
+                 pass_keys: int = 0, counter: int = 0, table_slots: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
-        cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys)
+        cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
+                     counter, table_slots)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -358,9 +362,10 @@ def debug_learned_fit(sorted_keys: np.ndarray, steps: int = 10000, lr: float = 1
 
 
 def tree_run(fastq_path: str, tmp_dir: str, k: int, n_thread: int = 8, min_count: int = 1, max_count: int = 0,
-             partition: int = KC_PARTITION_PREFIX, verbose: bool = False, sample_bytes: int = 0) -> tuple[Stats, float]:
+             partition: int = KC_PARTITION_PREFIX, verbose: bool = False, sample_bytes: int = 0,
+             counter: int = KC_COUNTER_BLOOM) -> tuple[Stats, float]:
     o = TreeOpts(str(fastq_path).encode(), str(tmp_dir).encode(), k, n_thread, min_count, max_count, partition,
-                 1 if verbose else 0, sample_bytes)
+                 1 if verbose else 0, sample_bytes, counter, 0)
     s = Stats()
     wall = ctypes.c_double()
     rc = lib().kmc_tree_run(ctypes.byref(o), ctypes.byref(s), ctypes.byref(wall))

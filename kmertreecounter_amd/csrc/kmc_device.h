@@ -167,12 +167,32 @@ __device__ __forceinline__ uint64_t bloom_mask(uint64_t lo, uint64_t hi)
 //           src/definition.h:28), fine = the next sbits bits.
 //   learned: coarse/fine from monotone boundaries on key word 0 produced by the
 //           learned CDF model (bucket = number of boundaries <= key).
+// learned routing tables: coarse table, 256 fine tables, 256 shifts
+constexpr uint32_t kLutCells = 4096;
+constexpr size_t kRouteLutBytes = kLutCells + 256 * (size_t)kLutCells + 256;
+
 struct Route {
     uint32_t mode;   // 0 prefix, 1 learned
     uint32_t sbits;  // 0..8
     const uint64_t* coarse_bounds;  // learned: 255 ascending boundaries
     const uint64_t* fine_bounds;    // learned: 256 x (2^sbits - 1) ascending boundaries
+    // (kernels stage one coarse bucket's fine bounds in LDS padded to 255 with UINT64_MAX)
+    // learned: 4096-entry coarse table, 256 x 4096 fine tables, 256 shifts
+    const uint8_t* lut;
 };
+
+// Number of entries <= x in ascending b[0..255): branchless, eight fixed
+// steps, so the searches of several keys interleave.
+__device__ __forceinline__ uint32_t upper_count255(const uint64_t* b, uint64_t x)
+{
+    uint32_t pos = 0;
+#pragma unroll
+    for (int s = 7; s >= 0; --s) {
+        const uint32_t q = pos + (1u << s);
+        pos = (b[q - 1] <= x) ? q : pos;
+    }
+    return pos;
+}
 
 __device__ __forceinline__ uint32_t upper_count(const uint64_t* b, uint32_t n, uint64_t x)
 {

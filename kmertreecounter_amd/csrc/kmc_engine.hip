@@ -71,6 +71,7 @@ struct kc_engine {
     unsigned long long* d_ctr = nullptr;
     uint64_t* d_bounds = nullptr;  // learned: 255 coarse + 256*(2^sbits-1) fine boundaries
     uint32_t bounds_sbits = ~0u;   // fan-out the uploaded bounds were built for
+    uint8_t* d_lut = nullptr;      // learned: routing lookup tables (kRouteLutBytes)
 
     hipEvent_t ev[12] = {};
 
@@ -261,6 +262,10 @@ int run_pipeline_t(kc_engine* h)
         routing_bounds(h->plan.learned_b, sbits, rb);
         if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * 255)));
         HIPX(hipMemcpy(h->d_bounds, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice));
+        std::vector<uint8_t> lut;
+        routing_luts(rb, sbits, lut);
+        if (!h->d_lut) HIPX(hipMalloc(&h->d_lut, kRouteLutBytes));
+        HIPX(hipMemcpy(h->d_lut, lut.data(), kRouteLutBytes, hipMemcpyHostToDevice));
         h->bounds_sbits = sbits;
     }
     Route rt{};
@@ -268,6 +273,7 @@ int run_pipeline_t(kc_engine* h)
     rt.sbits = sbits;
     rt.coarse_bounds = h->d_bounds;
     rt.fine_bounds = h->d_bounds ? h->d_bounds + 255 : nullptr;
+    rt.lut = h->d_lut;
 
     unsigned long long* counts = h->d_small + kSmCounts;
     unsigned long long* cursor = h->d_small + kSmCursor;
@@ -331,10 +337,13 @@ int run_pipeline_t(kc_engine* h)
         a.error_flags = h->d_ctr + kCtrErr;
         a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
         a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
-        a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys : CountCfg<NW>::kPassKeys;
+        // the exact counter reads 0 as its own default (its table size)
+        a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys
+                                       : (h->cfg.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
         a.extra_passes = h->d_ctr + kCtrExtraPasses;
+        a.table_slots = h->cfg.table_slots;
         HIPX(hipEventRecord(h->ev[8], st));
-        HIPX(launch_count<NW>(nf, src, fine_off, a, st));
+        HIPX(launch_count<NW>(nf, h->cfg.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
         HIPX(hipEventRecord(h->ev[9], st));
         HIPX(launch_spill_emit<NW>(a, st));
         SegArgs sa{};
@@ -440,6 +449,7 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     const uint32_t nw = kc_words_for_k(cfg->k);
     if (!nw) return KC_ERR_ARG;
     if (cfg->partition > 1 || (cfg->sub_bits > 8 && cfg->sub_bits != KC_SUB_BITS_NONE)) return KC_ERR_ARG;
+    if (cfg->counter > KC_COUNTER_EXACT || (cfg->table_slots && cfg->table_slots < 16)) return KC_ERR_ARG;
     if (cfg->max_count && cfg->min_count > cfg->max_count) return KC_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_ERR_NODEV;
@@ -486,6 +496,7 @@ int kc_destroy(kc_engine* h)
     hipFree(h->d_spill_keys);
     hipFree(h->d_spill_cnt);
     hipFree(h->d_bounds);
+    hipFree(h->d_lut);
     for (int i = 0; i < 2; ++i) {
         if (h->h_stage[i]) hipHostFree(h->h_stage[i]);
         if (h->stage_ev[i]) hipEventDestroy(h->stage_ev[i]);

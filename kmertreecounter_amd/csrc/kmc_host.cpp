@@ -301,6 +301,7 @@ int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
     kc_config cfg{};
     cfg.k = o->k;
     cfg.partition = o->partition;
+    cfg.counter = o->counter;
     cfg.min_count = o->min_count ? o->min_count : 1;
     cfg.max_count = o->max_count;
     cfg.expected_bytes = (uint64_t)sb.st_size;

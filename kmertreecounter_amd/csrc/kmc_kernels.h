@@ -27,6 +27,9 @@ template <> struct SubCfg<2> { static constexpr int kThreads = 256; static const
 // P3: one workgroup per fine bucket (1 per CU).  LDS: Bloom words + count
 // table; buckets above kPassKeys keys are processed in hash-range passes.
 template <int NW> struct CountCfg;
+// Bloom counter: 128 KiB of LDS per workgroup (Bloom 32 KiB, table 96 KiB),
+// one 1024-thread workgroup per CU.  (Half the table with two workgroups per
+// CU measured slower: the extra passes cost more than the occupancy gains.)
 template <> struct CountCfg<1> {
     static constexpr int kThreads = 1024;
     static constexpr uint32_t kBloomWords = 4096;
@@ -34,6 +37,8 @@ template <> struct CountCfg<1> {
     static constexpr uint32_t kPassKeys = 98304;
     static constexpr uint32_t kPassDistinct = 5000;  // target distinct keys per pass (61% table load)
     static constexpr uint32_t kProbe = 64;
+    static constexpr int kLoads = 4;  // key loads in flight per thread
+    static constexpr uint32_t kExactSlots = 10240;  // exact counter: 120 KiB of keys + counts
 };
 template <> struct CountCfg<2> {
     static constexpr int kThreads = 1024;
@@ -42,6 +47,8 @@ template <> struct CountCfg<2> {
     static constexpr uint32_t kPassKeys = 49152;
     static constexpr uint32_t kPassDistinct = 2500;
     static constexpr uint32_t kProbe = 64;
+    static constexpr int kLoads = 4;
+    static constexpr uint32_t kExactSlots = 6144;
 };
 
 // probes into the global spill table before a key is declared unplaceable
@@ -73,6 +80,7 @@ struct CountArgs {
     uint32_t max_count;
     uint64_t pass_keys;
     unsigned long long* extra_passes;   // passes beyond the first, summed over buckets
+    uint32_t table_slots;               // exact counter: LDS slots used (0 = all)
 };
 
 struct SegArgs {
@@ -105,7 +113,7 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
                       const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,
                       unsigned long long* cursor, uint64_t* B, hipStream_t st);
 template <int NW>
-hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
+hipError_t launch_count(uint32_t nf, bool bloom, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
                         hipStream_t st);
 template <int NW>
 hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st);

@@ -94,17 +94,28 @@ __device__ __forceinline__ void store_key(uint64_t* p, uint64_t i, const Key<NW>
     }
 }
 
-__device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s_cb, uint64_t w0)
+// Learned coarse bucket = number of the 255 bounds <= w0.  The table entry of
+// w0's 12-bit cell counts the bounds at or below the cell's first key, so only
+// bounds inside the cell remain to be stepped over (usually none).
+__device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s_cb, const uint8_t* s_lut, uint64_t w0)
 {
     if (rt.mode == 0) return (uint32_t)(w0 >> 56);
-    return upper_count(s_cb, 255, w0);
+    uint32_t b = s_lut[w0 >> 52];
+    while (b < 255 && s_cb[b] <= w0) ++b;
+    return b;
 }
 
-__device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_fb, uint64_t w0)
+// Learned fine bucket inside coarse bucket r (keys lo <= w0): the table of r
+// is indexed by (w0 - lo) >> shift, then bounds inside the cell are stepped over.
+__device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_fb, const uint8_t* s_lut, uint64_t lo,
+                                            uint32_t shift, uint64_t w0)
 {
     if (rt.sbits == 0) return 0;
-    if (rt.mode == 0) return (uint32_t)((w0 >> (56 - rt.sbits)) & ((1u << rt.sbits) - 1u));
-    return upper_count(s_fb, (1u << rt.sbits) - 1u, w0);
+    const uint32_t nb = (1u << rt.sbits) - 1u;
+    if (rt.mode == 0) return (uint32_t)((w0 >> (56 - rt.sbits)) & nb);
+    uint32_t f = s_lut[(w0 - lo) >> shift];
+    while (f < nb && s_fb[f] <= w0) ++f;
+    return f;
 }
 
 // ------------------------------------------------------------------ P0 / P1
@@ -160,14 +171,22 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     constexpr int T = ExtractCfg<NW>::kThreads;
     constexpr int H = (NW == 1) ? 2 : 4;
     constexpr int TILE = T * 16;
-    __shared__ uint32_t s_code[T + H];
-    __shared__ uint32_t s_valid[T + H];
+    // The encoded tile and the coarse bounds are dead once every thread has
+    // routed its keys (a barrier later), so they share LDS with the staged
+    // keys: this keeps the scatter variant under half of a CU's LDS.
+    constexpr size_t kCb = 2 * (T + H) * sizeof(uint32_t);
+    constexpr size_t kStage = kCb + 256 * sizeof(uint64_t) + kLutCells;
+    constexpr size_t kKeys = SCATTER ? TILE * sizeof(Key<NW>) : 0;
+    __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
+    uint32_t* s_code = reinterpret_cast<uint32_t*>(s_raw);
+    uint32_t* s_valid = s_code + (T + H);
+    uint64_t* s_cb = reinterpret_cast<uint64_t*>(s_raw + kCb);
+    uint8_t* s_lut = s_raw + kCb + 256 * sizeof(uint64_t);
+    Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_off[256];
     __shared__ unsigned long long s_gbase[256];
     __shared__ uint32_t s_scan[T / kWave + 1];
-    __shared__ uint64_t s_cb[256];
-    __shared__ Key<NW> s_keys[SCATTER ? TILE : 1];
     __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
 
     const int t = threadIdx.x;
@@ -190,7 +209,11 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         s_valid[t] = vd;
     }
     if (t < 256) s_hist[t] = 0;
-    if (rt.mode == 1 && t < 255) s_cb[t] = rt.coarse_bounds[t];
+    if (rt.mode == 1) {
+        if (t < 255) s_cb[t] = rt.coarse_bounds[t];
+        for (int i = t; i < (int)(kLutCells / 16); i += T)
+            reinterpret_cast<uint4*>(s_lut)[i] = reinterpret_cast<const uint4*>(rt.lut)[i];
+    }
     __syncthreads();
 
     uint32_t c[H + 1], v[H + 1];
@@ -212,7 +235,7 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         keys[i] = key;
         if (ok) {
             okm |= 1u << i;
-            const uint32_t b = coarse_of(rt, s_cb, key.w[0]);
+            const uint32_t b = coarse_of(rt, s_cb, s_lut, key.w[0]);
             bk[i] = b;
             rank[i] = atomicAdd(&s_hist[b], 1u);
         }
@@ -330,12 +353,18 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     constexpr int T = SubCfg<NW>::kThreads;
     constexpr int TILE = SubCfg<NW>::kTile;
     constexpr int PER = TILE / T;
+    // fine bounds and the bucket's fine table are dead once every thread has
+    // routed its keys (a barrier later): they share LDS with the staged keys
+    constexpr size_t kStage = 256 * sizeof(uint64_t) + kLutCells;
+    constexpr size_t kKeys = SCATTER ? TILE * sizeof(Key<NW>) : 0;
+    __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
+    uint64_t* s_fb = reinterpret_cast<uint64_t*>(s_raw);
+    uint8_t* s_lut = s_raw + 256 * sizeof(uint64_t);
+    Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_off[256];
     __shared__ unsigned long long s_gbase[256];
     __shared__ uint32_t s_scan[T / kWave + 1];
-    __shared__ uint64_t s_fb[256];
-    __shared__ Key<NW> s_keys[SCATTER ? TILE : 1];
     __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
 
     const int t = threadIdx.x;
@@ -357,18 +386,32 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     const uint32_t nf = 256u << rt.sbits;
 
     if (t < 256) s_hist[t] = 0;
-    if (rt.mode == 1 && t < (int)(nsub - 1)) s_fb[t] = rt.fine_bounds[(size_t)r * (nsub - 1) + t];
+    uint64_t flo = 0;
+    uint32_t shift = 0;
+    if (rt.mode == 1 && rt.sbits) {
+        if (t < 255) s_fb[t] = t < (int)(nsub - 1) ? rt.fine_bounds[(size_t)r * (nsub - 1) + t] : ~0ULL;
+        const uint8_t* tab = rt.lut + kLutCells + (size_t)r * kLutCells;
+        for (int i = t; i < (int)(kLutCells / 16); i += T)
+            reinterpret_cast<uint4*>(s_lut)[i] = reinterpret_cast<const uint4*>(tab)[i];
+        flo = r ? rt.coarse_bounds[r - 1] : 0;
+        shift = rt.lut[kLutCells + 256 * (size_t)kLutCells + r];
+    }
     __syncthreads();
 
     Key<NW> keys[PER];
     uint32_t sb[PER];
     uint32_t rank[PER];
+    // every load issued before the first key is routed
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint32_t idx = t + j * T;
+        if (idx < n) keys[j] = load_key<NW>(A, beg + idx);
+    }
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const uint32_t idx = t + j * T;
         if (idx < n) {
-            keys[j] = load_key<NW>(A, beg + idx);
-            sb[j] = fine_of(rt, s_fb, keys[j].w[0]);
+            sb[j] = fine_of(rt, s_fb, s_lut, flo, shift, keys[j].w[0]);
             rank[j] = atomicAdd(&s_hist[sb[j]], 1u);
         }
     }
@@ -506,7 +549,8 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
         s_hwidth = n > args.pass_keys ? (uint32_t)max(1ULL, (65536ULL * args.pass_keys) / n) : 65536u;
         s_passes = 0;
     }
-    const unsigned long long iters = (n + T - 1) / T;
+    constexpr int kLoads = CountCfg<NW>::kLoads;
+    const unsigned long long iters = (n + (unsigned long long)T * kLoads - 1) / ((unsigned long long)T * kLoads);
 
     for (;;) {
         for (uint32_t i = t; i < BW; i += T) s_bloom[i] = 0ULL;
@@ -523,48 +567,58 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
         const uint32_t hstart = s_hstart;
         const uint32_t hend = min(65536u, hstart + s_hwidth);
         for (unsigned long long it = 0; it < iters; ++it) {
-            const unsigned long long i = lo + it * T + t;
-            bool first = false;
-            Key<NW> key;
-            if (i < hi) {
-                key = load_key<NW>(B, i);
-                uint64_t hl, hh;
-                xxh3_128(key, hl, hh);
-                const uint32_t hb = (uint32_t)(hh >> 48);
-                if (hb >= hstart && hb < hend) {
-                    const uint32_t word = (uint32_t)(hl & (BW - 1));
-                    const uint64_t mask = bloom_mask(hl, hh);
-                    // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
-                    unsigned long long old = s_bloom[word];
-                    if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
-                    first = (old & mask) != mask;
-                    if (!first) {
-                        bool claimed;
-                        const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
-                                                                 (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
-                        if (s >= 0) {
-                            atomicAdd(&s_tcnt[s], 1u);
-                        } else {
-                            const uint64_t gs = (hh >> 32) ^ (hl >> 17);
-                            const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
-                            const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap,
-                                                                     args.spill_cap - 1, gs, glimit, key, claimed);
-                            if (g >= 0) {
-                                atomicAdd(&args.spill_cnt[g], 1u);
-                                if (claimed) atomicAdd(args.spilled, 1ULL);
+            // kLoads keys per thread in flight before any is processed: one
+            // outstanding load per thread cannot cover HBM latency
+            Key<NW> kb[kLoads];
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                if (i < hi) kb[u] = load_key<NW>(B, i);
+            }
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                bool first = false;
+                const Key<NW>& key = kb[u];
+                if (i < hi) {
+                    uint64_t hl, hh;
+                    xxh3_128(key, hl, hh);
+                    const uint32_t hb = (uint32_t)(hh >> 48);
+                    if (hb >= hstart && hb < hend) {
+                        const uint32_t word = (uint32_t)(hl & (BW - 1));
+                        const uint64_t mask = bloom_mask(hl, hh);
+                        // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
+                        unsigned long long old = s_bloom[word];
+                        if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
+                        first = (old & mask) != mask;
+                        if (!first) {
+                            bool claimed;
+                            const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
+                                                                     (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
+                            if (s >= 0) {
+                                atomicAdd(&s_tcnt[s], 1u);
                             } else {
-                                atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+                                const uint64_t gs = (hh >> 32) ^ (hl >> 17);
+                                const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
+                                const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap,
+                                                                         args.spill_cap - 1, gs, glimit, key, claimed);
+                                if (g >= 0) {
+                                    atomicAdd(&args.spill_cnt[g], 1u);
+                                    if (claimed) atomicAdd(args.spilled, 1ULL);
+                                } else {
+                                    atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+                                }
                             }
                         }
                     }
                 }
+                // first occurrences: one LDS atomic per wave, consecutive slots per wave
+                const uint64_t bal = __ballot(first);
+                uint32_t wbase = 0;
+                if (lane == 0 && bal) wbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bal));
+                wbase = __shfl(wbase, 0, kWave);
+                if (first) store_key<NW>(args.firsts, lo + wbase + lanes_below(bal), key);
             }
-            // first occurrences: one LDS atomic per wave, consecutive slots per wave
-            const uint64_t bal = __ballot(first);
-            uint32_t wbase = 0;
-            if (lane == 0 && bal) wbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bal));
-            wbase = __shfl(wbase, 0, kWave);
-            if (first) store_key<NW>(args.firsts, lo + wbase + lanes_below(bal), key);
         }
         __syncthreads();
         // root records of this pass (append_export_record filter on the repeat
@@ -613,6 +667,225 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
 
 // Records of keys that lived in the global spill table go to the overflow
 // list.  Fixed grid, grid-stride; returns at once when nothing spilled.
+// 32-bit mix of a key for the exact counter (multiply-xorshift; the final
+// three steps are the "lowbias32" integer finalizer): bits 31..16 pick the
+// hash range of a pass, bits 15..0 the first table slot.
+template <int NW>
+__device__ __forceinline__ uint32_t count_hash(const Key<NW>& k)
+{
+    uint32_t x = (uint32_t)(k.w[0] >> 32) * 0x9E3779B1u ^ (uint32_t)k.w[0];
+    if constexpr (NW == 2) x = (x ^ (uint32_t)(k.w[1] >> 32)) * 0x85EBCA77u ^ (uint32_t)k.w[1];
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Open addressing over the first ts (any size) of TS LDS slots (word w of
+// slot s at words[w * TS + s]), same claim protocol as table_find_or_claim;
+// returns the slot or -1 after `limit` probes.
+template <int NW, uint32_t TS>
+__device__ __forceinline__ int32_t lds_find_or_claim(unsigned long long* words, uint32_t ts, uint32_t start,
+                                                     uint32_t limit, const Key<NW>& key, bool& claimed)
+{
+    claimed = false;
+    uint32_t s = start;
+    for (uint32_t p = 0; p < limit; ++p) {
+        bool match = true;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (!match) break;
+            unsigned long long* slotw = &words[(uint32_t)w * TS + s];
+            unsigned long long cur = *slotw;
+            if (cur == kEmpty) {
+                cur = atomicCAS(slotw, (unsigned long long)kEmpty, (unsigned long long)key.w[w]);
+                if (cur == kEmpty) {
+                    cur = key.w[w];
+                    if (w == NW - 1) claimed = true;
+                }
+            }
+            match = (cur == key.w[w]);
+        }
+        if (match) return (int32_t)s;
+        if (++s == ts) s = 0;
+    }
+    return -1;
+}
+
+// Exact counter: every occurrence is counted in the bucket's LDS table; a key
+// seen c times leaves one first occurrence (the low.bin key) and, for c >= 2,
+// the root record (key, c - 1) under the min/max filter -- the split the
+// reference produces when its Bloom filter reports no false positive
+// (src/FastqClassifier.h:241-349, src/NewKmerTree.h:1159-1168).
+// A pass takes the keys whose hash range (bits 31..16) lies in
+// [hstart, hend).  The first pass starts with the whole range, later ones with
+// a width sized from the density the previous pass emitted.  hend can shrink
+// while the pass runs: once, when the claimed slots reach `shrink_at`, to the
+// width whose remaining keys are expected to fit; and to a key's range when
+// its probe finds no slot.  Only keys below the final hend are emitted, the
+// rest are counted again from scratch by the next pass.  A key in range hstart itself is in every pass's
+// final range, so only such a key goes to the global spill table (whose
+// records keep their full count, with no first occurrence).
+template <int NW>
+__global__ void __launch_bounds__(CountCfg<NW>::kThreads)
+k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ fine_off, CountArgs args)
+{
+    constexpr int T = CountCfg<NW>::kThreads;
+    constexpr uint32_t TS = CountCfg<NW>::kExactSlots;
+    constexpr int kLoads = CountCfg<NW>::kLoads;
+    __shared__ unsigned long long s_tkey[NW * TS];
+    __shared__ uint32_t s_tcnt[TS];
+    __shared__ uint32_t s_nfirst, s_nrec, s_dist, s_dpass, s_kept, s_claims, s_abort, s_hstart, s_hend, s_width, s_passes;
+
+    const int t = threadIdx.x;
+    const uint32_t lane = t & (kWave - 1);
+    const uint32_t f = blockIdx.x;
+    const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
+    const unsigned long long n = hi - lo;
+    const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
+    uint8_t* rec_region = args.rec_tmp + lo * 8ULL * NW;
+    // slots in use: the whole table unless a smaller one is asked for
+    const uint32_t ts = args.table_slots ? min(args.table_slots, TS) : TS;
+    // distinct keys a pass aims to hold: the table, or fewer when asked
+    const uint32_t cap = (args.pass_keys && args.pass_keys < ts) ? (uint32_t)args.pass_keys : ts;
+    const uint32_t shrink_at = cap / 2 ? cap / 2 : 1u;
+    if (t == 0) {
+        s_nfirst = 0;
+        s_nrec = 0;
+        s_dist = 0;
+        s_hstart = 0;
+        s_width = 65536u;
+        s_passes = 0;
+    }
+    const unsigned long long iters = (n + (unsigned long long)T * kLoads - 1) / ((unsigned long long)T * kLoads);
+
+    for (;;) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            for (uint32_t i = t; i < ts; i += T) s_tkey[(uint32_t)w * TS + i] = kEmpty;
+        for (uint32_t i = t; i < ts; i += T) s_tcnt[i] = 0u;
+        if (t == 0) {
+            s_abort = (__atomic_load_n(args.error_flags, __ATOMIC_RELAXED) & kErrSpillFull) ? 1u : 0u;
+            s_dpass = 0;
+            s_kept = 0;
+            s_claims = 0;
+            s_hend = min(65536u, s_hstart + s_width);
+        }
+        __syncthreads();
+        if (s_abort) break;
+        const uint32_t hstart = s_hstart;
+        for (unsigned long long it = 0; it < iters; ++it) {
+            Key<NW> kb[kLoads];
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                if (i < hi) kb[u] = load_key<NW>(B, i);
+            }
+            // a stale (larger) hend only admits keys that the emission drops
+            const uint32_t hend = __atomic_load_n(&s_hend, __ATOMIC_RELAXED);
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                if (i >= hi) continue;
+                const uint32_t x = count_hash<NW>(kb[u]);
+                const uint32_t hb = x >> 16;
+                if (hb < hstart || hb >= hend) continue;
+                bool claimed;
+                const int32_t s = lds_find_or_claim<NW, TS>(s_tkey, ts, ((x & 0xFFFFu) * ts) >> 16,
+                                                              CountCfg<NW>::kProbe, kb[u], claimed);
+                if (s >= 0) {
+                    atomicAdd(&s_tcnt[s], 1u);
+                    if (claimed) {
+                        const uint32_t c = atomicAdd(&s_claims, 1u) + 1u;
+                        if (c == shrink_at) {
+                            // Keys are met at least as fast as the scan advances, so
+                            // c / phi bounds the range's distinct keys; keep the part
+                            // of the range whose remaining keys fit the free slots.
+                            const double phi = (double)(it + 1) / (double)iters;
+                            const double dest = (double)c / phi;
+                            const double room = 0.9 * cap - c;
+                            const uint32_t cur = __atomic_load_n(&s_hend, __ATOMIC_RELAXED);
+                            if (dest > 0.9 * cap) {
+                                const double w = (double)(cur - hstart) * room / (dest - c);
+                                const uint32_t wi = w < 1.0 ? 1u : (uint32_t)w;
+                                atomicMin(&s_hend, hstart + wi);
+                            }
+                        }
+                    }
+                } else if (hb > hstart) {
+                    atomicMin(&s_hend, hb);  // defer this key's range to a later pass
+                } else {
+                    const uint64_t gs = ((uint64_t)x << 32) ^ kb[u].w[0] ^ (kb[u].w[0] >> 29);
+                    const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
+                    const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap, args.spill_cap - 1,
+                                                             gs, glimit, kb[u], claimed);
+                    if (g >= 0) {
+                        atomicAdd(&args.spill_cnt[g], 1u);
+                        if (claimed) atomicAdd(args.spilled, 1ULL);
+                    } else {
+                        atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t hend = s_hend;
+        uint32_t dist = 0, kept = 0;
+        for (uint32_t s0 = 0; s0 < ts; s0 += T) {
+            const uint32_t s = s0 + t;
+            uint32_t cnt = 0;
+            Key<NW> key;
+            if (s < ts) {
+                cnt = s_tcnt[s];
+                if (cnt) {
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
+                    if ((count_hash<NW>(key) >> 16) >= hend) cnt = 0;  // counted again next pass
+                }
+            }
+            const uint64_t bf = __ballot(cnt != 0);
+            uint32_t fbase = 0;
+            if (lane == 0 && bf) fbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bf));
+            fbase = __shfl(fbase, 0, kWave);
+            if (cnt) store_key<NW>(args.firsts, lo + fbase + lanes_below(bf), key);
+            dist += cnt >= 2 ? 1u : 0u;
+            kept += cnt ? 1u : 0u;
+            const uint32_t r = cnt - 1;
+            const bool keep = cnt >= 2 && r >= args.min_count && r <= args.max_count;
+            const uint64_t bal = __ballot(keep);
+            uint32_t wbase = 0;
+            if (lane == 0 && bal) wbase = atomicAdd(&s_nrec, (uint32_t)__popcll(bal));
+            wbase = __shfl(wbase, 0, kWave);
+            if (keep) {
+                const uint64_t pos = wbase + lanes_below(bal);
+                if (pos < rec_cap) write_record<NW>(rec_region, pos, key, r);
+                else append_overflow<NW>(args, key, r);
+            }
+        }
+        if (dist) atomicAdd(&s_dpass, dist);
+        if (kept) atomicAdd(&s_kept, kept);
+        __syncthreads();
+        if (t == 0) {
+            s_dist += s_dpass;
+            s_passes++;
+            s_hstart = hend;
+            // next range: about 60% of the table at the density this pass emitted
+            const uint64_t w = ((uint64_t)(hend - hstart) * (cap - cap * 2 / 5)) / (s_kept ? s_kept : 1u);
+            s_width = (uint32_t)(w < 1 ? 1 : (w > 65536 ? 65536 : w));
+        }
+        __syncthreads();
+        if (s_hstart >= 65536u) break;
+    }
+    if (t == 0) {
+        args.nfirst[f] = s_nfirst;
+        args.nrec[f] = (uint32_t)(s_nrec < rec_cap ? s_nrec : rec_cap);
+        args.ndist[f] = s_dist;
+        if (s_passes > 1) atomicAdd(args.extra_passes, (unsigned long long)(s_passes - 1));
+    }
+}
+
 template <int NW>
 __global__ void __launch_bounds__(256)
 k_spill_emit(CountArgs args)
@@ -881,9 +1154,13 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
 }
 
 template <int NW>
-hipError_t launch_count(uint32_t nf, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args, hipStream_t st)
+hipError_t launch_count(uint32_t nf, bool bloom, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
+                        hipStream_t st)
 {
-    hipLaunchKernelGGL((k_count<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
+    if (bloom)
+        hipLaunchKernelGGL((k_count<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
+    else
+        hipLaunchKernelGGL((k_count_exact<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -929,8 +1206,8 @@ template hipError_t launch_extract<1>(bool, const uint8_t*, uint64_t, uint32_t, 
 template hipError_t launch_extract<2>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_sub<1>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_sub<2>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
-template hipError_t launch_count<1>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
-template hipError_t launch_count<2>(uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<1>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<2>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
 template hipError_t launch_finalize<1>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "kmc_device.h"
+
 namespace kmc {
 
 // 64 inputs (key bits) -> 64 -> 64 -> 4096 buckets
@@ -61,5 +63,11 @@ void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std:
 // Coarse (255) then per-coarse fine (256 x (2^sbits - 1)) routing boundaries
 // from the 4097 learned boundaries; fine ones interpolate inside a learned bucket.
 void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<uint64_t>& out);
+// Lookup tables that start each routing search next to its answer:
+// [0, 4096): coarse table, entry p = number of coarse bounds <= p << 52;
+// then per coarse bucket r a 4096-entry fine table over r's key range
+// [lo_r, hi_r): entry c = number of r's fine bounds <= lo_r + (c << shift_r);
+// then the 256 shifts.  `rb` is the output of routing_bounds.
+void routing_luts(const std::vector<uint64_t>& rb, uint32_t sbits, std::vector<uint8_t>& lut);
 
 }  // namespace kmc

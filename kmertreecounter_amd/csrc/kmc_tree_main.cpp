@@ -5,7 +5,9 @@
 // Same positional arguments and checks; writes ./tmp/low.bin and
 // ./tmp/root_{0..255}.bin (./tmp/ must exist, as for the reference) for
 // histogram_tool.  The counting runs on the GPU.  KMC_PARTITION=learned picks
-// the learned partitioner (default: the 256 root-prefix partition).
+// the learned partitioner (default: the 256 root-prefix partition);
+// KMC_COUNTER=exact the exact counter (default: the reference's Bloom
+// first-occurrence split).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -54,6 +56,8 @@ int main(int argc, char** argv)
     o.min_count = (uint32_t)min_c;
     o.max_count = (uint32_t)max_c;
     o.partition = (part && std::strcmp(part, "learned") == 0) ? KC_PARTITION_LEARNED : KC_PARTITION_PREFIX;
+    const char* counter = std::getenv("KMC_COUNTER");
+    o.counter = (counter && std::strcmp(counter, "exact") == 0) ? KC_COUNTER_EXACT : KC_COUNTER_BLOOM;
     o.verbose = 1;
     kc_stats st;
     double wall = 0;

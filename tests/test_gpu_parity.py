@@ -26,7 +26,10 @@ def as_rows(keys: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(keys).view(dt).ravel()
 
 
-def check_outputs(firsts, recs, stats, k, exp_keys, exp_counts):
+COUNTERS = pytest.mark.parametrize("counter", [K.KC_COUNTER_EXACT, K.KC_COUNTER_BLOOM], ids=["exact", "bloom"])
+
+
+def check_outputs(firsts, recs, stats, k, exp_keys, exp_counts, exact=False):
     nw = C.nwords(k)
     assert stats.total_kmers == int(exp_counts.sum(dtype=np.uint64))
     keys, counts = C.merge_outputs(firsts, recs, nw)
@@ -35,6 +38,16 @@ def check_outputs(firsts, recs, stats, k, exp_keys, exp_counts):
     np.testing.assert_array_equal(counts, exp_counts)
     assert stats.firsts == firsts.shape[0]
     assert stats.records == len(recs)
+    frows_all = as_rows(firsts.reshape(-1, nw).copy())
+    assert np.unique(frows_all).size == frows_all.size  # a key is a first occurrence at most once
+    if exact:
+        # exact counter: every key is a first occurrence unless it was spilled, and a
+        # spilled key's record holds its whole count
+        rk = as_rows(recs["key"].reshape(-1, nw)) if len(recs) else np.zeros(0, frows_all.dtype)
+        srt = np.sort(frows_all)
+        pos = np.searchsorted(srt, rk)
+        in_f = (pos < srt.size) & (srt[np.minimum(pos, max(srt.size - 1, 0))] == rk) if srt.size else np.zeros(rk.size, bool)
+        assert stats.firsts + int((~in_f).sum()) == exp_keys.shape[0]
     if len(recs):
         assert (recs["pad"] == 0).all()
         assert (recs["count"] >= 1).all()
@@ -67,16 +80,17 @@ def test_device_xxh3_matches_oracle():
         np.testing.assert_array_equal(got, exp)
 
 
+@COUNTERS
 @pytest.mark.parametrize("k", [31, 17, 32, 33, 63, 5])
-def test_small_random_stream(k):
+def test_small_random_stream(k, counter):
     rng = np.random.default_rng(k)
     alphabet = np.frombuffer(b"ACGTacgtuUN\n", dtype=np.uint8)
     p = np.array([.22, .22, .22, .22, .02, .02, .02, .02, .01, .01, .01, .01])
     p /= p.sum()
     seq = bytes(rng.choice(alphabet, size=300_000, p=p))
     keys, counts = C.count_table(C.extract(seq, k))
-    f, r, s = run_engine([seq], k)
-    check_outputs(f, r, s, k, keys, counts)
+    f, r, s = run_engine([seq], k, counter=counter)
+    check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
 
 
 def test_empty_and_degenerate_inputs():
@@ -96,7 +110,7 @@ def test_fanout_does_not_change_table(sub_bits):
     keys, counts = C.count_table(C.extract(seq, 31))
     f, r, s = run_engine([seq], 31, sub_bits=sub_bits)
     assert s.fine_buckets == 256 << (0 if sub_bits == K.KC_SUB_BITS_NONE else sub_bits)
-    check_outputs(f, r, s, 31, keys, counts)
+    check_outputs(f, r, s, 31, keys, counts, exact=True)
 
 
 def test_block_carry_path_matches():
@@ -110,12 +124,12 @@ def test_block_carry_path_matches():
 
 
 def test_spill_table_path():
-    # no sub-partition and one pass per bucket leave ~190k keys (~11k distinct repeats)
-    # per bucket, more than the LDS table holds: the global spill table must keep
-    # the counts exact
+    # Bloom counter, no sub-partition and one pass per bucket: ~190k keys (~11k
+    # distinct repeats) per bucket, more than the LDS table holds, so the global
+    # spill table must keep the counts exact
     seq = K.generate(1, scale=1, seed=1, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
-    f, r, s = run_engine([seq], 31, sub_bits=K.KC_SUB_BITS_NONE, pass_keys=1 << 30)
+    f, r, s = run_engine([seq], 31, sub_bits=K.KC_SUB_BITS_NONE, pass_keys=1 << 30, counter=K.KC_COUNTER_BLOOM)
     assert s.fine_buckets == 256
     assert s.spilled_keys > 0
     check_outputs(f, r, s, 31, keys, counts)
@@ -123,20 +137,35 @@ def test_spill_table_path():
 
 @pytest.mark.parametrize("pass_keys", [2000, 50000])
 def test_multi_pass_buckets(pass_keys):
-    # buckets split into hash-range passes give the same table
+    # Bloom counter: buckets split into hash-range passes give the same table
     seq = K.generate(1, scale=10, seed=21, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
-    f, r, s = run_engine([seq], 31, sub_bits=2, pass_keys=pass_keys)
+    f, r, s = run_engine([seq], 31, sub_bits=2, pass_keys=pass_keys, counter=K.KC_COUNTER_BLOOM)
     check_outputs(f, r, s, 31, keys, counts)
 
 
+@pytest.mark.parametrize("k", [31, 63])
+@pytest.mark.parametrize("table_slots,pass_keys", [(512, 64), (64, 0), (1024, 0), (0, 100)])
+def test_exact_counter_narrowed_passes(k, table_slots, pass_keys):
+    # exact counter with small tables: passes narrow their hash range as the
+    # table fills (and defer a key whose probe finds no slot); keys left out
+    # of a pass's final range are counted again by the next pass
+    seq = K.generate(1, scale=40, seed=22, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, k))
+    # 256 buckets of ~4.7k keys (~1.4k distinct)
+    f, r, s = run_engine([seq], k, sub_bits=K.KC_SUB_BITS_NONE, table_slots=table_slots, pass_keys=pass_keys)
+    assert s.extra_passes > 0
+    check_outputs(f, r, s, k, keys, counts, exact=True)
+
+
+@COUNTERS
 @pytest.mark.parametrize("name", sorted(C.reference_runs().keys()))
-def test_reference_fixture(name):
+def test_reference_fixture(name, counter):
     fx = C.reference_runs()[name]
     k = fx["k"]
     fq = K.generate(fx["workload"], scale=fx["scale"], seed=fx["seed"])
     blocks = K.fastq_blocks(fq, k)
-    f, r, s = run_engine(blocks, k)
+    f, r, s = run_engine(blocks, k, counter=counter)
     assert s.total_kmers == fx["K"]
     keys, counts = C.merge_outputs(f, r, C.nwords(k))
     assert keys.shape[0] == fx["D"]
@@ -146,10 +175,11 @@ def test_reference_fixture(name):
     assert got == fx["histogram_nonzero"]
 
 
-def test_min_max_filter():
+@COUNTERS
+def test_min_max_filter(counter):
     seq = K.generate(1, scale=20, seed=4, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
-    with K.Engine(31, min_count=3, max_count=9) as e:
+    with K.Engine(31, min_count=3, max_count=9, counter=counter) as e:
         e.push_block(seq)
         e.finish()
         f = e.firsts()
