@@ -94,13 +94,14 @@ def kernel_names(nw: int) -> dict:
     }
 
 
-def traffic_from_profiles(kernel: str, workload: int, partition: str) -> float | None:
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+def traffic_from_profiles(phase: str, workload: int, partition: str) -> float | None:
+    """HBM bytes per launch of a phase from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its reading for k_copy16), if present."""
     p = ROOT / "profiles" / f"traffic_w{workload}_{partition}.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get(kernel)
+        return json.loads(p.read_text())["bytes_per_launch"].get(phase)
     except Exception:
         return None
 
@@ -252,6 +253,9 @@ def main() -> int:
     ap.add_argument("--counter", choices=["bloom", "exact"], default="bloom")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
+                    help="after the run, copy this many GiB with k_copy16 (a plain copy whose "
+                         "FETCH_SIZE/WRITE_SIZE readings scale those of the pipeline kernels)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -334,6 +338,10 @@ def main() -> int:
         "partitions": {p: summary(r) for p, r in runs.items()},
         "cpu_baseline": cpu,
     }
+    if args.calibration_copy_gib > 0:
+        n = int(args.calibration_copy_gib * (1 << 30)) // 16 * 16
+        ms = K.debug_copy(n)  # k_copy16: reads n bytes, writes n bytes
+        out["calibration_copy"] = {"bytes": n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
     if "learned" in runs:
         out["learned"] = runs["learned"]["learned"]
     if len(runs) == 2:

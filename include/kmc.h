@@ -172,6 +172,11 @@ KC_API const char* kc_last_error(kc_engine* h);
  * high64) pairs.  Lets tests pin the device hash against the host one. */
 KC_API int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out);
 
+/* kc_debug_copy: one plain device copy of `bytes` (a multiple of 16) with a
+ * 16-byte-per-lane kernel; *ms = its device time.  Reads `bytes` and writes
+ * `bytes` of HBM: the known-size reference for hardware traffic counters. */
+KC_API int kc_debug_copy(uint64_t bytes, uint32_t device, float* ms);
+
 /* Test hooks of the learned partitioner, no reference counterpart.
  * kc_debug_gemm_bf16: C[M][N] = A[M][K] . Bt[N][K]^T on the MFMA product
  *   (A, Bt as bf16 bit patterns; M, N multiples of 64, K of 16).

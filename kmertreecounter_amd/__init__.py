@@ -101,7 +101,7 @@ EXPORTED = [
     "kc_emit_root", "kc_stats_get", "kc_destroy", "kc_count_resident", "kc_words_for_k",
     "kc_last_error", "kc_debug_xxh3", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
-    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit",
+    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy",
 ]
 
 _lib = None
@@ -136,6 +136,7 @@ def lib() -> ctypes.CDLL:
     L.kc_last_error.argtypes = [vp]
     L.kc_last_error.restype = cp
     L.kc_debug_xxh3.argtypes = [vp, u64, u32, vp]
+    L.kc_debug_copy.argtypes = [u64, u32, ctypes.POINTER(ctypes.c_float)]
     L.kc_debug_gemm_bf16.argtypes = [vp, vp, vp, u32, u32, u32]
     L.kc_debug_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
     L.kc_debug_learned_fit.argtypes = [vp, u64, u32, ctypes.c_float, u32, u64, vp, vp, u32]
@@ -309,6 +310,15 @@ def count_sequence(seq: bytes, k: int, **kw) -> tuple[np.ndarray, np.ndarray, St
         e.push_block(seq)
         e.finish()
         return e.firsts(), e.records(), e.stats()
+
+
+def debug_copy(nbytes: int, device: int = 0) -> float:
+    """One plain device copy of nbytes (reads and writes nbytes); returns its device ms."""
+    ms = ctypes.c_float()
+    rc = lib().kc_debug_copy(ctypes.c_uint64(nbytes), ctypes.c_uint32(device), ctypes.byref(ms))
+    if rc != 0:
+        raise KmcError(f"kc_debug_copy failed: {rc}")
+    return ms.value
 
 
 def debug_xxh3(keys: np.ndarray, nw: int) -> np.ndarray:

@@ -684,6 +684,31 @@ int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out)
     return rc;
 }
 
+int kc_debug_copy(uint64_t bytes, uint32_t device, float* ms)
+{
+    if (bytes == 0 || bytes % 16) return KC_ERR_ARG;
+    if (hipSetDevice((int)device) != hipSuccess) return KC_ERR_NODEV;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return KC_ERR_NOMEM;
+    if (hipMalloc(&b, bytes) != hipSuccess) { hipFree(a); return KC_ERR_NOMEM; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = KC_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && hipMemset(a, 1, bytes) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && hipEventRecord(e0, nullptr) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && launch_debug_copy(a, b, bytes / 16, nullptr) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && hipEventRecord(e1, nullptr) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = KC_ERR_HIP;
+    float t = 0.f;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = KC_ERR_HIP;
+    if (ms) *ms = t;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
 int kc_debug_gemm_bf16(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, uint32_t N, uint32_t K)
 {
     if (!A || !Bt || !C) return KC_ERR_ARG;

@@ -122,5 +122,6 @@ hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_
                            const uint8_t* rec_in, uint8_t* rec_out, hipStream_t st);
 template <int NW>
 hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t st);
+hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
 
 }  // namespace kmc

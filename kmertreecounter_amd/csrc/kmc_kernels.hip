@@ -1212,6 +1212,22 @@ template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
 template hipError_t launch_finalize<1>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
 template hipError_t launch_finalize<2>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
+// Plain copy of n16 16-byte words: the traffic reference that HBM counters
+// of the pipeline kernels are scaled by (reads 16*n16 bytes, writes 16*n16).
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, st, reinterpret_cast<const uint4*>(src),
+                       reinterpret_cast<uint4*>(dst), n16);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
 template hipError_t launch_debug_hash<1>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template hipError_t launch_debug_hash<2>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 
