@@ -99,6 +99,9 @@ typedef struct kc_config {
     uint32_t table_slots;     /* exact counter: count-table slots per bucket and pass (0 = the whole
                                  LDS table, 10240 for k <= 32 and 6144 above; smaller values narrow
                                  each pass and are meant for tuning and tests) */
+    uint32_t heavy_keys;      /* a bucket of more keys than this (0 = 1Mi) is counted in chunks by the
+                                 whole GPU instead of one workgroup; its keys are written as root
+                                 records with their whole count (no low.bin entry) */
 } kc_config;
 
 typedef struct kc_stats {
@@ -135,6 +138,7 @@ typedef struct kc_stats {
     double learned_sample_distinct_max_over_mean; /* the same two, counting distinct keys per bucket */
     double prefix_sample_distinct_max_over_mean;
     uint64_t extra_passes;              /* counting passes beyond one per bucket, summed over buckets */
+    uint64_t heavy_buckets;             /* buckets counted in chunks by the whole GPU */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;
@@ -186,9 +190,11 @@ KC_API int kc_debug_gemm_bf16(const uint16_t* A, const uint16_t* Bt, float* C, u
 KC_API int kc_debug_mlp_grad(const float* params, const uint64_t* keys, const uint32_t* labels, uint32_t nb,
                              float* grad, float* loss);
 /* kc_debug_learned_fit: train on n sorted key words (steps, lr, batch, seed as
- * in kc_plan) and return the 4097 bucket boundaries and the loss every 50 steps. */
+ * in kc_plan) and return the 4097 bucket boundaries and the loss every 50 steps.
+ * flags bit 0: train on the distinct keys; bit 1: boundaries from the expected
+ * bucket instead of the most likely one. */
 KC_API int kc_debug_learned_fit(const uint64_t* sorted, uint64_t n, uint32_t steps, float lr, uint32_t batch,
-                                uint64_t seed, uint64_t* bounds, float* loss_curve, uint32_t loss_cap);
+                                uint64_t seed, uint32_t flags, uint64_t* bounds, float* loss_curve, uint32_t loss_cap);
 
 #ifdef __cplusplus
 }

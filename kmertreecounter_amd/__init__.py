@@ -42,6 +42,7 @@ class Config(ctypes.Structure):
         ("expected_bytes", ctypes.c_uint64), ("sub_bits", ctypes.c_uint32),
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
+        ("heavy_keys", ctypes.c_uint32),
     ]
 
 
@@ -65,6 +66,7 @@ class Stats(ctypes.Structure):
         ("learned_sample_max_over_mean", ctypes.c_double), ("prefix_sample_max_over_mean", ctypes.c_double),
         ("learned_sample_distinct_max_over_mean", ctypes.c_double),
         ("prefix_sample_distinct_max_over_mean", ctypes.c_double), ("extra_passes", ctypes.c_uint64),
+        ("heavy_buckets", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -139,7 +141,7 @@ def lib() -> ctypes.CDLL:
     L.kc_debug_copy.argtypes = [u64, u32, ctypes.POINTER(ctypes.c_float)]
     L.kc_debug_gemm_bf16.argtypes = [vp, vp, vp, u32, u32, u32]
     L.kc_debug_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
-    L.kc_debug_learned_fit.argtypes = [vp, u64, u32, ctypes.c_float, u32, u64, vp, vp, u32]
+    L.kc_debug_learned_fit.argtypes = [vp, u64, u32, ctypes.c_float, u32, u64, u32, vp, vp, u32]
     L.kcs_generate.argtypes = [ctypes.POINTER(SynthSpec), ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(u64),
                                ctypes.POINTER(u64)]
     L.kcs_free.argtypes = [vp]
@@ -216,12 +218,12 @@ class Engine:
 
     def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
-                 pass_keys: int = 0, counter: int = 0, table_slots: int = 0):
+                 pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
-                     counter, table_slots)
+                     counter, table_slots, heavy_keys)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -359,13 +361,15 @@ def debug_mlp_grad(params: np.ndarray, keys: np.ndarray, labels: np.ndarray) -> 
     return float(loss[0]), g
 
 
-def debug_learned_fit(sorted_keys: np.ndarray, steps: int = 10000, lr: float = 1e-2, batch: int = 2048,
-                      seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+def debug_learned_fit(sorted_keys: np.ndarray, steps: int = 8000, lr: float = 1e-2, batch: int = 2048,
+                      seed: int = 1, distinct: bool = True, expect: bool = False) -> tuple[np.ndarray, np.ndarray]:
     k = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
     b = np.zeros(4097, dtype=np.uint64)
     cap = steps // 50 + 2
     lc = np.zeros(cap, dtype=np.float32)
-    rc = lib().kc_debug_learned_fit(k.ctypes.data, k.size, steps, lr, batch, seed, b.ctypes.data, lc.ctypes.data, cap)
+    flags = (1 if distinct else 0) | (2 if expect else 0)
+    rc = lib().kc_debug_learned_fit(k.ctypes.data, k.size, steps, lr, batch, seed, flags, b.ctypes.data,
+                                    lc.ctypes.data, cap)
     if rc != 0:
         raise KmcError(f"kc_debug_learned_fit failed: {rc}")
     return b, lc

@@ -78,6 +78,10 @@ struct kc_engine {
     // results
     bool have_results = false;
     uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0, extra_passes = 0;
+    uint64_t heavy_buckets = 0;
+    unsigned long long* d_heavy = nullptr;  // heavy-bucket chunk list, heavy_cap x (lo, hi)
+    uint64_t heavy_cap = 0;
+    uint32_t cu_count = 0;
     uint32_t nf_last = 0;
     const uint64_t* res_firsts = nullptr;
     const uint8_t* res_recs = nullptr;
@@ -190,6 +194,14 @@ constexpr uint64_t kSmCounts = 0, kSmCursor = 8 * 256, kSmCoarseOff = 16 * 256, 
                    kSmRootOvf = kSmTileStart + 257, kSmRootMain = kSmRootOvf + 256, kSmRootOff = kSmRootMain + 256,
                    kSmOvfOff = kSmRootOff + 257, kSmOvfCursor = kSmOvfOff + 257, kSmTotal = kSmOvfCursor + 256;
 
+// Buckets above heavy_keys keys are counted in chunks by the whole GPU.
+uint64_t heavy_keys(const kc_engine* h) { return h->cfg.heavy_keys ? h->cfg.heavy_keys : (1ULL << 20); }
+uint32_t heavy_chunk_keys(const kc_engine* h)
+{
+    const uint64_t c = heavy_keys(h) / 4;
+    return (uint32_t)std::min<uint64_t>(1ULL << 18, std::max<uint64_t>(1024, c));
+}
+
 int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 {
     const uint64_t kb = 8ULL * h->nw;
@@ -222,6 +234,15 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
         if (hipMalloc(&h->d_ovf, cap * rec_bytes(h->nw)) != hipSuccess)
             return h->fail(KC_ERR_NOMEM, "device allocation for the overflow records failed");
         h->ovf_cap = cap;
+    }
+    // every listed chunk but the last of a bucket is full: at most K/chunk + nf chunks
+    const uint64_t want_heavy = padded_len / heavy_chunk_keys(h) + nf + 1;
+    if (h->heavy_cap < want_heavy) {
+        hipFree(h->d_heavy);
+        h->d_heavy = nullptr;
+        h->heavy_cap = 0;
+        HIPX(hipMalloc(&h->d_heavy, sizeof(unsigned long long) * 2 * want_heavy));
+        h->heavy_cap = want_heavy;
     }
     const uint64_t want_spill = std::min<uint64_t>(1ULL << 26, std::max<uint64_t>(1ULL << 16, pow2_at_least(padded_len / 512)));
     if (h->spill_cap < want_spill) {
@@ -342,8 +363,16 @@ int run_pipeline_t(kc_engine* h)
                                        : (h->cfg.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
         a.extra_passes = h->d_ctr + kCtrExtraPasses;
         a.table_slots = h->cfg.table_slots;
+        a.heavy_keys = heavy_keys(h);
+        a.heavy_chunk = heavy_chunk_keys(h);
+        a.heavy_list = h->d_heavy;
+        a.heavy_cap = h->heavy_cap;
+        a.heavy_count = h->d_ctr + kCtrHeavyChunks;
+        a.heavy_next = h->d_ctr + kCtrHeavyNext;
+        a.heavy_buckets = h->d_ctr + kCtrHeavyBuckets;
         HIPX(hipEventRecord(h->ev[8], st));
         HIPX(launch_count<NW>(nf, h->cfg.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
+        HIPX(launch_heavy<NW>(h->cu_count, src, a, st));
         HIPX(hipEventRecord(h->ev[9], st));
         HIPX(launch_spill_emit<NW>(a, st));
         SegArgs sa{};
@@ -401,6 +430,7 @@ int run_pipeline_t(kc_engine* h)
         h->max_bucket = ctr[kCtrMaxBucket];
         h->n_distinct_rep = ctr[kCtrDistinct] + ctr[kCtrSpillDistinct];
         h->extra_passes = ctr[kCtrExtraPasses];
+        h->heavy_buckets = ctr[kCtrHeavyBuckets];
         h->nf_last = nf;
         h->res_firsts = src;
         h->res_recs = reinterpret_cast<const uint8_t*>(slices);
@@ -462,6 +492,10 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     h->device = (int)cfg->device;
     auto bail = [&](int code) { kc_destroy(h); return code; };
     if (hipSetDevice(h->device) != hipSuccess) return bail(KC_ERR_HIP);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || cus <= 0)
+        return bail(KC_ERR_HIP);
+    h->cu_count = (uint32_t)cus;  // k_heavy: one persistent workgroup per CU
     if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return bail(KC_ERR_HIP);
     for (int i = 0; i < 2; ++i) {
         if (hipHostMalloc((void**)&h->h_stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
@@ -497,6 +531,7 @@ int kc_destroy(kc_engine* h)
     hipFree(h->d_spill_cnt);
     hipFree(h->d_bounds);
     hipFree(h->d_lut);
+    hipFree(h->d_heavy);
     for (int i = 0; i < 2; ++i) {
         if (h->h_stage[i]) hipHostFree(h->h_stage[i]);
         if (h->stage_ev[i]) hipEventDestroy(h->stage_ev[i]);
@@ -637,6 +672,7 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->fine_buckets = h->nf_last;
     s->max_bucket_keys = h->max_bucket;
     s->extra_passes = h->extra_passes;
+    s->heavy_buckets = h->heavy_buckets;
     s->mean_bucket_keys = h->nf_last ? (double)h->K / h->nf_last : 0.0;
     s->ms_extract_count = h->phase_ms[0];
     s->ms_extract_scatter = h->phase_ms[1];
@@ -729,14 +765,16 @@ int kc_debug_mlp_grad(const float* params, const uint64_t* keys, const uint32_t*
 }
 
 int kc_debug_learned_fit(const uint64_t* sorted, uint64_t n, uint32_t steps, float lr, uint32_t batch, uint64_t seed,
-                         uint64_t* bounds, float* loss_curve, uint32_t loss_cap)
+                         uint32_t flags, uint64_t* bounds, float* loss_curve, uint32_t loss_cap)
 {
-    if (!sorted || !bounds || n < 64 || batch == 0 || batch % 64 || steps == 0) return KC_ERR_ARG;
+    if (!sorted || !bounds || n < 64 || batch == 0 || batch % 64 || steps == 0 || flags > 3) return KC_ERR_ARG;
     MlpTrainOpts o;
     o.steps = steps;
     o.lr = lr;
     o.batch = batch;
     o.seed = seed;
+    o.distinct = (flags & 1u) != 0;
+    o.expect = (flags & 2u) != 0;
     std::string err;
     const int rc = mlp_debug_fit(sorted, n, o, bounds, loss_curve, loss_curve ? loss_cap : 0, err);
     return rc ? KC_ERR_HIP : KC_OK;

@@ -59,7 +59,7 @@ template <int NW> struct RecBytes { static constexpr uint32_t value = 8 * NW + 8
 
 // device counters (d_ctr)
 enum : int { kCtrFirst = 0, kCtrRec, kCtrSpilled, kCtrErr, kCtrMaxBucket, kCtrDistinct, kCtrOvf, kCtrSpillDistinct,
-             kCtrExtraPasses, kCtrCount };
+             kCtrExtraPasses, kCtrHeavyChunks, kCtrHeavyNext, kCtrHeavyBuckets, kCtrCount };
 
 struct CountArgs {
     uint64_t* firsts;                   // bucket f: keys at [fine_off[f], fine_off[f+1])
@@ -81,6 +81,16 @@ struct CountArgs {
     uint64_t pass_keys;
     unsigned long long* extra_passes;   // passes beyond the first, summed over buckets
     uint32_t table_slots;               // exact counter: LDS slots used (0 = all)
+    // Buckets of more than heavy_keys keys are not counted by one workgroup:
+    // their workgroup lists heavy_chunk-key chunks of them and k_heavy counts
+    // the chunks across the whole GPU into the global table.
+    uint64_t heavy_keys;                // 0 = never
+    uint32_t heavy_chunk;
+    unsigned long long* heavy_list;     // [heavy_cap] x (lo, hi) key ranges
+    uint64_t heavy_cap;
+    unsigned long long* heavy_count;    // chunks listed
+    unsigned long long* heavy_next;     // chunks taken by k_heavy
+    unsigned long long* heavy_buckets;  // buckets handed over
 };
 
 struct SegArgs {
@@ -115,6 +125,8 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
 template <int NW>
 hipError_t launch_count(uint32_t nf, bool bloom, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
                         hipStream_t st);
+template <int NW>
+hipError_t launch_heavy(uint32_t workgroups, const uint64_t* B, const CountArgs& args, hipStream_t st);
 template <int NW>
 hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st);
 template <int NW>

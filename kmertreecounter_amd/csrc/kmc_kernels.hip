@@ -452,6 +452,32 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
 
 // ------------------------------------------------------------------ P3
 
+// Runs of equal keys in consecutive lanes of a wave (the lanes read consecutive
+// keys of a bucket, and a microsatellite's repeated key stays clustered through
+// both scatters).  Returns this lane's run length if it heads a run (the run
+// stops at an invalid lane), 0 if it continues one or is invalid.
+template <int NW>
+__device__ __forceinline__ uint32_t run_head_length(const Key<NW>& key, bool valid)
+{
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    bool same = valid && lane > 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const unsigned long long prev = __shfl_up((unsigned long long)key.w[w], 1, kWave);
+        same = same && prev == key.w[w];
+    }
+    const uint64_t pvalid = __ballot(valid);
+    const bool prev_valid = lane > 0 && ((pvalid >> (lane > 0 ? lane - 1 : 0)) & 1ULL);
+    same = same && prev_valid;
+    const bool head = !same;  // invalid lanes head empty runs, so no run crosses them
+    const uint64_t heads = __ballot(head);
+    if (!valid || !head) return 0;
+    const uint64_t above = lane == 63 ? 0ULL : (heads >> (lane + 1));
+    return above ? (uint32_t)__ffsll((long long)above) : 64u - lane;
+}
+
+
+
 // Two-word-safe open addressing insert.  Slot s holds key words w[0..NW) that
 // only ever change from kEmpty to a value (each word claimed by CAS in order),
 // so every thread inserting the same key walks the same permanent slot states
@@ -504,6 +530,63 @@ __device__ __forceinline__ void append_overflow(const CountArgs& a, const Key<NW
     else atomicOr(a.error_flags, (unsigned long long)kErrRecFull);
 }
 
+// First probe slot of a key in the global (spill) table.  Every kernel that
+// inserts into that table uses this one function, so a key always walks the
+// same probe sequence.
+template <int NW>
+__device__ __forceinline__ uint64_t spill_start(const Key<NW>& k)
+{
+    uint64_t z = k.w[0];
+    if constexpr (NW == 2) z ^= (k.w[1] >> 7) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;  // SplitMix64 finalizer
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Adds `add` occurrences of key to the global table.
+template <int NW>
+__device__ __forceinline__ void global_add(const CountArgs& args, const Key<NW>& key, uint32_t add)
+{
+    bool claimed;
+    const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
+    const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap, args.spill_cap - 1, spill_start<NW>(key),
+                                             glimit, key, claimed);
+    if (g >= 0) {
+        atomicAdd(&args.spill_cnt[g], add);
+        if (claimed) atomicAdd(args.spilled, 1ULL);
+    } else {
+        atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+    }
+}
+
+// A bucket of more than args.heavy_keys keys is handed to k_heavy: thread 0
+// lists its chunks; the bucket's own slices stay empty.  Returns true when the
+// workgroup is done with the bucket.
+__device__ __forceinline__ bool hand_over_heavy(const CountArgs& args, uint32_t f, unsigned long long lo,
+                                                unsigned long long hi)
+{
+    const unsigned long long n = hi - lo;
+    if (!args.heavy_keys || n <= args.heavy_keys) return false;
+    if (threadIdx.x == 0) {
+        const unsigned long long ch = args.heavy_chunk;
+        const unsigned long long nch = (n + ch - 1) / ch;
+        const unsigned long long base = atomicAdd(args.heavy_count, nch);
+        for (unsigned long long c = 0; c < nch; ++c) {
+            if (base + c >= args.heavy_cap) {  // sized so that this cannot happen
+                atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
+                break;
+            }
+            args.heavy_list[2 * (base + c)] = lo + c * ch;
+            args.heavy_list[2 * (base + c) + 1] = min(hi, lo + (c + 1) * ch);
+        }
+        args.nfirst[f] = 0;
+        args.nrec[f] = 0;
+        args.ndist[f] = 0;
+        atomicAdd(args.heavy_buckets, 1ULL);
+    }
+    return true;
+}
+
 // One workgroup per fine bucket f, keys B[fine_off[f], fine_off[f+1]).
 // A bucket larger than pass_keys is processed in hash-range passes (range of
 // the top 16 hash bits), each with a fresh LDS Bloom filter and count table, so
@@ -539,6 +622,7 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
     const unsigned long long n = hi - lo;
     const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
     uint8_t* rec_region = args.rec_tmp + lo * 8ULL * NW;
+    if (hand_over_heavy(args, f, lo, hi)) return;
     if (t == 0) {
         s_nfirst = 0;
         s_nrec = 0;
@@ -580,7 +664,10 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                 const unsigned long long i = lo + (it * kLoads + u) * T + t;
                 bool first = false;
                 const Key<NW>& key = kb[u];
-                if (i < hi) {
+                // a run of r equal keys is one occurrence followed by r - 1
+                // repeats: the head lane tests the filter once and counts the rest
+                const uint32_t run = run_head_length<NW>(key, i < hi);
+                if (run) {
                     uint64_t hl, hh;
                     xxh3_128(key, hl, hh);
                     const uint32_t hb = (uint32_t)(hh >> 48);
@@ -591,24 +678,13 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                         unsigned long long old = s_bloom[word];
                         if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
                         first = (old & mask) != mask;
-                        if (!first) {
+                        const uint32_t add = run - (first ? 1u : 0u);
+                        if (add) {
                             bool claimed;
                             const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
                                                                      (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
-                            if (s >= 0) {
-                                atomicAdd(&s_tcnt[s], 1u);
-                            } else {
-                                const uint64_t gs = (hh >> 32) ^ (hl >> 17);
-                                const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
-                                const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap,
-                                                                         args.spill_cap - 1, gs, glimit, key, claimed);
-                                if (g >= 0) {
-                                    atomicAdd(&args.spill_cnt[g], 1u);
-                                    if (claimed) atomicAdd(args.spilled, 1ULL);
-                                } else {
-                                    atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
-                                }
-                            }
+                            if (s >= 0) atomicAdd(&s_tcnt[s], add);
+                            else global_add<NW>(args, key, add);
                         }
                     }
                 }
@@ -746,6 +822,7 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
     const unsigned long long n = hi - lo;
     const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
     uint8_t* rec_region = args.rec_tmp + lo * 8ULL * NW;
+    if (hand_over_heavy(args, f, lo, hi)) return;
     // slots in use: the whole table unless a smaller one is asked for
     const uint32_t ts = args.table_slots ? min(args.table_slots, TS) : TS;
     // distinct keys a pass aims to hold: the table, or fewer when asked
@@ -788,7 +865,9 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
 #pragma unroll
             for (int u = 0; u < kLoads; ++u) {
                 const unsigned long long i = lo + (it * kLoads + u) * T + t;
-                if (i >= hi) continue;
+                // the head of a run of equal keys counts the whole run
+                const uint32_t run = run_head_length<NW>(kb[u], i < hi);
+                if (!run) continue;
                 const uint32_t x = count_hash<NW>(kb[u]);
                 const uint32_t hb = x >> 16;
                 if (hb < hstart || hb >= hend) continue;
@@ -796,7 +875,7 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
                 const int32_t s = lds_find_or_claim<NW, TS>(s_tkey, ts, ((x & 0xFFFFu) * ts) >> 16,
                                                               CountCfg<NW>::kProbe, kb[u], claimed);
                 if (s >= 0) {
-                    atomicAdd(&s_tcnt[s], 1u);
+                    atomicAdd(&s_tcnt[s], run);
                     if (claimed) {
                         const uint32_t c = atomicAdd(&s_claims, 1u) + 1u;
                         if (c == shrink_at) {
@@ -817,16 +896,7 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
                 } else if (hb > hstart) {
                     atomicMin(&s_hend, hb);  // defer this key's range to a later pass
                 } else {
-                    const uint64_t gs = ((uint64_t)x << 32) ^ kb[u].w[0] ^ (kb[u].w[0] >> 29);
-                    const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
-                    const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap, args.spill_cap - 1,
-                                                             gs, glimit, kb[u], claimed);
-                    if (g >= 0) {
-                        atomicAdd(&args.spill_cnt[g], 1u);
-                        if (claimed) atomicAdd(args.spilled, 1ULL);
-                    } else {
-                        atomicOr(args.error_flags, (unsigned long long)kErrSpillFull);
-                    }
+                    global_add<NW>(args, kb[u], run);
                 }
             }
         }
@@ -883,6 +953,68 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
         args.nrec[f] = (uint32_t)(s_nrec < rec_cap ? s_nrec : rec_cap);
         args.ndist[f] = s_dist;
         if (s_passes > 1) atomicAdd(args.extra_passes, (unsigned long long)(s_passes - 1));
+    }
+}
+
+// Chunks of the buckets handed over by the counting kernels, taken from a
+// work counter by a persistent grid (one workgroup per CU).  A chunk is
+// counted exactly in an LDS table (runs of equal keys in a wave counted once),
+// then the table is added into the global table; k_spill_emit writes those
+// keys' records with their whole count (no first occurrence, the outcome a
+// Bloom false positive gives in the reference).  Every workgroup leaves the
+// loop at the same chunk index, once the listed chunks are taken.
+template <int NW>
+__global__ void __launch_bounds__(CountCfg<NW>::kThreads)
+k_heavy(const uint64_t* __restrict__ B, CountArgs args)
+{
+    constexpr int T = CountCfg<NW>::kThreads;
+    constexpr uint32_t TS = CountCfg<NW>::kExactSlots;
+    constexpr int kLoads = CountCfg<NW>::kLoads;
+    __shared__ unsigned long long s_tkey[NW * TS];
+    __shared__ uint32_t s_tcnt[TS];
+    __shared__ unsigned long long s_chunk;
+    const int t = threadIdx.x;
+    const unsigned long long nchunks = min((unsigned long long)args.heavy_cap, *args.heavy_count);
+    for (;;) {
+        if (t == 0) s_chunk = atomicAdd(args.heavy_next, 1ULL);
+        __syncthreads();
+        const unsigned long long c = s_chunk;
+        if (c >= nchunks) break;
+        for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
+        for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
+        __syncthreads();
+        const unsigned long long lo = args.heavy_list[2 * c], hi = args.heavy_list[2 * c + 1];
+        const unsigned long long iters = (hi - lo + (unsigned long long)T * kLoads - 1) / ((unsigned long long)T * kLoads);
+        for (unsigned long long it = 0; it < iters; ++it) {
+            Key<NW> kb[kLoads];
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                if (i < hi) kb[u] = load_key<NW>(B, i);
+            }
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                const uint32_t run = run_head_length<NW>(kb[u], i < hi);
+                if (!run) continue;
+                const uint32_t x = count_hash<NW>(kb[u]);
+                bool claimed;
+                const int32_t s = lds_find_or_claim<NW, TS>(s_tkey, TS, ((x & 0xFFFFu) * TS) >> 16,
+                                                              CountCfg<NW>::kProbe, kb[u], claimed);
+                if (s >= 0) atomicAdd(&s_tcnt[s], run);
+                else global_add<NW>(args, kb[u], run);
+            }
+        }
+        __syncthreads();
+        for (uint32_t s = t; s < TS; s += T) {
+            const uint32_t cnt = s_tcnt[s];
+            if (!cnt) continue;
+            Key<NW> key;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
+            global_add<NW>(args, key, cnt);
+        }
+        __syncthreads();  // the table is cleared for the next chunk
     }
 }
 
@@ -1166,6 +1298,14 @@ hipError_t launch_count(uint32_t nf, bool bloom, const uint64_t* B, const unsign
 }
 
 template <int NW>
+hipError_t launch_heavy(uint32_t workgroups, const uint64_t* B, const CountArgs& args, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_heavy<NW>), dim3(workgroups), dim3(CountCfg<NW>::kThreads), 0, st, B, args);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+template <int NW>
 hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st)
 {
     hipLaunchKernelGGL((k_spill_emit<NW>), dim3(1024), dim3(256), 0, st, args);
@@ -1208,6 +1348,8 @@ template hipError_t launch_sub<1>(bool, uint64_t, const uint64_t*, const unsigne
 template hipError_t launch_sub<2>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_count<1>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_count<2>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_heavy<1>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
+template hipError_t launch_heavy<2>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
 template hipError_t launch_finalize<1>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);

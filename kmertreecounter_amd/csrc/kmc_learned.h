@@ -36,29 +36,36 @@ __device__ __forceinline__ void write_bf16_copy(uint32_t i, float v, __bf16* w1b
     }
 }
 
+// Defaults measured on the skewed workload's sample (1.2M keys, 0.78M
+// distinct): distinct-key training with the most likely bucket reaches
+// max/mean 1.13-1.15 over distinct keys (p99 1.02) from 6000 to 16000 steps
+// and for several seeds; occurrence-weighted labels (a microsatellite key
+// spans ~29 quantile buckets) reach only 3.5, and the expected bucket 2.2.
 struct MlpTrainOpts {
-    uint32_t steps = 10000;
+    uint32_t steps = 8000;
     uint32_t batch = 2048;  // multiple of 64
     float lr = 1e-2f;       // Adam, cosine decay to zero
     uint64_t seed = 1;
+    bool distinct = true;   // train on the sample's distinct keys (quantiles over distinct keys)
+    bool expect = false;    // boundaries from the expected bucket instead of the most likely one
 };
 
 void mlp_init_params(uint64_t seed, std::vector<float>& p);
 int mlp_train(const uint64_t* d_sorted, uint64_t n, const MlpTrainOpts& o, hipStream_t st, std::vector<float>& params,
               std::vector<float>& loss_curve, std::string& err);
-int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, hipStream_t st, std::vector<uint16_t>& pred,
-                std::string& err);
+int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, bool expect, hipStream_t st,
+                std::vector<float>& pred, std::string& err);
 int mlp_debug_loss_grad(const float* params, const uint64_t* keys, const uint32_t* labels, uint32_t nb, float* grad,
                         float* loss, std::string& err);
 int mlp_debug_fit(const uint64_t* sorted, uint64_t n, const MlpTrainOpts& o, uint64_t* bounds, float* loss_curve,
                   uint32_t loss_cap, std::string& err);
 int mlp_debug_gemm(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, uint32_t N, uint32_t K, std::string& err);
 
-// Monotone bucket boundaries from predictions on sorted keys (isotonic fit of
-// prediction against key order): b[0] = 0, b[j] = first key whose fitted
+// Monotone bucket boundaries from predictions (expected bucket) on sorted keys
+// (isotonic fit of prediction against key order): b[0] = 0, b[j] = first key whose fitted
 // prediction reaches j (j = 1..4095), b[4096] = UINT64_MAX.  Bucket j is
 // [b[j], b[j+1]).
-void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std::vector<uint16_t>& pred,
+void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std::vector<float>& pred,
                                  std::vector<uint64_t>& b);
 // Coarse (255) then per-coarse fine (256 x (2^sbits - 1)) routing boundaries
 // from the 4097 learned boundaries; fine ones interpolate inside a learned bucket.

@@ -62,13 +62,15 @@ k_gemm_nt(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt, float* __
     }
 }
 
-// out[i] = sum over slabs s (in order) of in[s * stride + i]
+// out[i] = sum over slabs s (in order) of in[s * stride + i]; out may be
+// slab 0 itself (each element is read before it is written, by one thread)
 __global__ void __launch_bounds__(256)
-k_sum_slabs(const float* __restrict__ in, uint32_t nslab, size_t stride, uint32_t n, float* __restrict__ out)
+k_sum_slabs(const float* in, uint32_t nslab, size_t stride, uint32_t n, float* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float acc = 0.f;
+#pragma unroll 4
     for (uint32_t sl = 0; sl < nslab; ++sl) acc += in[sl * stride + i];
     out[i] = acc;
 }
@@ -77,24 +79,29 @@ k_sum_slabs(const float* __restrict__ in, uint32_t nslab, size_t stride, uint32_
 
 __device__ __forceinline__ float xin(uint64_t key, int j) { return ((key >> (63 - j)) & 1ULL) ? 1.0f : -1.0f; }
 
-// Hidden layers for a batch of 64 samples per workgroup: thread (unit i =
-// tid % 64, sample group tid / 64) keeps row i of W1 and then of W2 in
-// registers; a1 of the 64 samples is exchanged through LDS.  h2 is written in
-// both layouts the products need (rows for the logits, columns for dW3).
-// nb is a multiple of 64.
+// Samples per workgroup in the hidden-layer kernels (four per wave group).
+constexpr uint32_t kSpb = 16;
+
+// Hidden layers for kSpb samples per workgroup: thread (unit i = tid % 64,
+// sample group tid / 64) keeps row i of W1 and then of W2 in registers; a1
+// of the samples is exchanged through LDS.  h2 is written in both layouts the
+// products need (rows for the logits, columns for dW3).  nb is a multiple of kSpb.
 __global__ void __launch_bounds__(256)
 k_mlp_hidden(const uint64_t* __restrict__ keys, uint32_t nb, const float* __restrict__ params,
              const __bf16* __restrict__ w1b, const __bf16* __restrict__ w2b, float* __restrict__ h1,
              __bf16* __restrict__ h2, __bf16* __restrict__ h2t, uint32_t ldt)
 {
-    __shared__ float s_a[64][kMlpH + 1];
+    constexpr uint32_t kPer = kSpb / 4;
+    __shared__ float s_a[kSpb][kMlpH + 1];
     const uint32_t unit = threadIdx.x % 64, grp = threadIdx.x / 64;
-    const uint32_t s0 = blockIdx.x * 64;
+    const uint32_t s0 = blockIdx.x * kSpb;
     float w[kMlpIn];
 #pragma unroll
     for (int j = 0; j < kMlpIn; ++j) w[j] = from_bf16(w1b[unit * kMlpIn + j]);
     float bias = params[kMlpOffB1 + unit];
-    for (uint32_t sl = grp; sl < 64; sl += 4) {
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t sl = grp + 4 * r;
         const uint64_t key = keys[s0 + sl];
         float acc = bias;
 #pragma unroll
@@ -107,9 +114,9 @@ k_mlp_hidden(const uint64_t* __restrict__ keys, uint32_t nb, const float* __rest
     for (int j = 0; j < kMlpH; ++j) w[j] = from_bf16(w2b[unit * kMlpH + j]);
     bias = params[kMlpOffB2 + unit];
     __syncthreads();
-    float out[16];
+    float out[kPer];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (uint32_t r = 0; r < kPer; ++r) {
         const uint32_t sl = grp + 4 * r;
         float acc = bias;
 #pragma unroll
@@ -120,10 +127,10 @@ k_mlp_hidden(const uint64_t* __restrict__ keys, uint32_t nb, const float* __rest
     if (!h2t) return;
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) s_a[grp + 4 * r][unit] = out[r];
+    for (uint32_t r = 0; r < kPer; ++r) s_a[grp + 4 * r][unit] = out[r];
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < 64 * kMlpH; idx += 256) {
-        const uint32_t u = idx / 64, sl = idx % 64;
+    for (uint32_t idx = threadIdx.x; idx < kSpb * kMlpH; idx += 256) {
+        const uint32_t u = idx / kSpb, sl = idx % kSpb;
         h2t[(size_t)u * ldt + s0 + sl] = to_bf16(s_a[sl][u]);
     }
 }
@@ -211,22 +218,21 @@ k_row_sum_bf16(const __bf16* __restrict__ m, uint32_t cols, float* __restrict__ 
     if (threadIdx.x == 0) out[blockIdx.x] = s_red[0];
 }
 
-// Backward through the hidden layers, 64 samples per workgroup (nb a
-// multiple of 64): da2 = dh2 * [a2 > 0], da1 = (W2^T da2) * [a1 > 0].  dh2
-// arrives as nslab partial slabs (stride apart) that are summed in order.
+// Backward through the hidden layers, kSpb samples per workgroup (nb a
+// multiple of kSpb): da2 = dh2 * [a2 > 0], da1 = (W2^T da2) * [a1 > 0].
 __global__ void __launch_bounds__(256)
-k_mlp_back_hidden(const float* __restrict__ dh2, uint32_t nslab, size_t stride, const __bf16* __restrict__ h2,
-                  const float* __restrict__ h1, const __bf16* __restrict__ w2b, uint32_t nb, float* __restrict__ da2,
-                  float* __restrict__ da1)
+k_mlp_back_hidden(const float* __restrict__ dh2, const __bf16* __restrict__ h2, const float* __restrict__ h1,
+                  const __bf16* __restrict__ w2b, uint32_t nb, float* __restrict__ da2, float* __restrict__ da1)
 {
-    __shared__ float s_d[64][kMlpH + 1];
+    constexpr uint32_t kPer = kSpb / 4;
+    __shared__ float s_d[kSpb][kMlpH + 1];
     const uint32_t unit = threadIdx.x % 64, grp = threadIdx.x / 64;
-    const uint32_t s0 = blockIdx.x * 64;
-    for (uint32_t sl = grp; sl < 64; sl += 4) {
+    const uint32_t s0 = blockIdx.x * kSpb;
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t sl = grp + 4 * r;
         const size_t e = (size_t)(s0 + sl) * kMlpH + unit;
-        float g = 0.f;
-        for (uint32_t z = 0; z < nslab; ++z) g += dh2[z * stride + e];
-        const float d = from_bf16(h2[e]) > 0.f ? g : 0.f;
+        const float d = from_bf16(h2[e]) > 0.f ? dh2[e] : 0.f;
         da2[e] = d;
         s_d[sl][unit] = d;
     }
@@ -234,7 +240,9 @@ k_mlp_back_hidden(const float* __restrict__ dh2, uint32_t nslab, size_t stride, 
 #pragma unroll
     for (int i = 0; i < kMlpH; ++i) w[i] = from_bf16(w2b[i * kMlpH + unit]);
     __syncthreads();
-    for (uint32_t sl = grp; sl < 64; sl += 4) {
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t sl = grp + 4 * r;
         float acc = 0.f;
 #pragma unroll
         for (int i = 0; i < kMlpH; ++i) acc += w[i] * s_d[sl][i];
@@ -243,41 +251,44 @@ k_mlp_back_hidden(const float* __restrict__ dh2, uint32_t nslab, size_t stride, 
     }
 }
 
-// Gradients of W1, b1, W2, b2, one workgroup per 64 samples writing its
-// partial sums to part[blockIdx.x][kSmallGrads] (summed in order afterwards).
+// Gradients of W1, b1, W2, b2, one workgroup per kSgChunk samples writing
+// its partial sums to part[blockIdx.x][kSmallGrads] (summed in order afterwards).
 // Output o: [0, 4096) dW2[i][j]; [4096, 8192) dW1[i][j]; then db2, db1.
 constexpr uint32_t kSmallGrads = 2 * kMlpH * kMlpH + 2 * kMlpH;
+constexpr uint32_t kSgChunk = 32;
 __global__ void __launch_bounds__(256)
 k_mlp_small_grads(const float* __restrict__ da2, const float* __restrict__ h1, const float* __restrict__ da1,
                   const uint64_t* __restrict__ keys, float* __restrict__ part)
 {
-    __shared__ float s_da2[64][kMlpH], s_h1[64][kMlpH], s_da1[64][kMlpH];
-    __shared__ uint64_t s_key[64];
-    const uint32_t s0 = blockIdx.x * 64;
-    for (uint32_t idx = threadIdx.x; idx < 64 * kMlpH; idx += 256) {
+    __shared__ float s_da2[kSgChunk][kMlpH], s_h1[kSgChunk][kMlpH], s_da1[kSgChunk][kMlpH];
+    __shared__ uint64_t s_key[kSgChunk];
+    const uint32_t s0 = blockIdx.x * kSgChunk;
+    for (uint32_t idx = threadIdx.x; idx < kSgChunk * kMlpH; idx += 256) {
         const uint32_t sl = idx / kMlpH, u = idx % kMlpH;
         const size_t e = (size_t)(s0 + sl) * kMlpH + u;
         s_da2[sl][u] = da2[e];
         s_h1[sl][u] = h1[e];
         s_da1[sl][u] = da1[e];
     }
-    if (threadIdx.x < 64) s_key[threadIdx.x] = keys[s0 + threadIdx.x];
+    if (threadIdx.x < kSgChunk) s_key[threadIdx.x] = keys[s0 + threadIdx.x];
     __syncthreads();
     float* out = part + (size_t)blockIdx.x * kSmallGrads;
     for (uint32_t o = threadIdx.x; o < kSmallGrads; o += 256) {
         float acc = 0.f;
         if (o < kMlpH * kMlpH) {
             const uint32_t i = o / kMlpH, j = o % kMlpH;
-            for (int b = 0; b < 64; ++b) acc += s_da2[b][i] * s_h1[b][j];
+#pragma unroll 8
+            for (uint32_t b = 0; b < kSgChunk; ++b) acc += s_da2[b][i] * s_h1[b][j];
         } else if (o < 2 * kMlpH * kMlpH) {
             const uint32_t u = o - kMlpH * kMlpH, i = u / kMlpIn, j = u % kMlpIn;
-            for (int b = 0; b < 64; ++b) acc += ((s_key[b] >> (63 - j)) & 1ULL) ? s_da1[b][i] : -s_da1[b][i];
+#pragma unroll 8
+            for (uint32_t b = 0; b < kSgChunk; ++b) acc += ((s_key[b] >> (63 - j)) & 1ULL) ? s_da1[b][i] : -s_da1[b][i];
         } else if (o < 2 * kMlpH * kMlpH + kMlpH) {
             const uint32_t i = o - 2 * kMlpH * kMlpH;
-            for (int b = 0; b < 64; ++b) acc += s_da2[b][i];
+            for (uint32_t b = 0; b < kSgChunk; ++b) acc += s_da2[b][i];
         } else {
             const uint32_t i = o - 2 * kMlpH * kMlpH - kMlpH;
-            for (int b = 0; b < 64; ++b) acc += s_da1[b][i];
+            for (uint32_t b = 0; b < kSgChunk; ++b) acc += s_da1[b][i];
         }
         out[o] = acc;
     }
@@ -342,9 +353,57 @@ k_gather_batch(const uint64_t* __restrict__ sorted, uint64_t n, uint64_t seed, u
     labels[i] = (uint32_t)((idx * (uint64_t)kMlpC) / n);
 }
 
-// argmax over each logits row (+ bias)
+// Expected bucket of each row: sum_c c * softmax(logits + b3)_c.  A model that
+// spreads its probability over neighbouring buckets still places the key
+// between them, which the argmax would not.
 __global__ void __launch_bounds__(256)
-k_argmax(const float* __restrict__ logits, const float* __restrict__ b3, uint32_t rows, uint16_t* __restrict__ out)
+k_expect(const float* __restrict__ logits, const float* __restrict__ b3, uint32_t rows, float* __restrict__ out)
+{
+    constexpr int kPer = kMlpC / 256;
+    __shared__ float s_red[256];
+    __shared__ float s_red2[256];
+    const uint32_t row = blockIdx.x;
+    if (row >= rows) return;
+    const float* z = logits + (size_t)row * kMlpC;
+    float v[kPer];
+    float mx = -3.0e38f;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const uint32_t c = threadIdx.x + 256 * r;
+        v[r] = z[c] + b3[c];
+        mx = fmaxf(mx, v[r]);
+    }
+    s_red[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_red[threadIdx.x] = fmaxf(s_red[threadIdx.x], s_red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    mx = s_red[0];
+    __syncthreads();
+    float se = 0.f, sc = 0.f;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const float e = __expf(v[r] - mx);
+        se += e;
+        sc += e * (float)(threadIdx.x + 256 * r);
+    }
+    s_red[threadIdx.x] = se;
+    s_red2[threadIdx.x] = sc;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s_red[threadIdx.x] += s_red[threadIdx.x + w];
+            s_red2[threadIdx.x] += s_red2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[row] = s_red2[0] / s_red[0];
+}
+
+// Most likely bucket of each row (lowest index on ties), as a float.
+__global__ void __launch_bounds__(256)
+k_argmax(const float* __restrict__ logits, const float* __restrict__ b3, uint32_t rows, float* __restrict__ out)
 {
     __shared__ float s_v[256];
     __shared__ uint32_t s_i[256];
@@ -370,7 +429,7 @@ k_argmax(const float* __restrict__ logits, const float* __restrict__ b3, uint32_
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[row] = (uint16_t)s_i[0];
+    if (threadIdx.x == 0) out[row] = (float)s_i[0];
 }
 
 // ------------------------------------------------------------------ host side
@@ -420,7 +479,7 @@ struct MlpWork {
         LCHK(dh2.alloc(4ULL * kSplitDh2 * batch * kMlpH));
         LCHK(da2.alloc(4ULL * batch * kMlpH));
         LCHK(da1.alloc(4ULL * batch * kMlpH));
-        const size_t np = std::max<size_t>((size_t)kSplitW3 * kMlpC * kMlpH, (size_t)(batch / 64) * kSmallGrads);
+        const size_t np = std::max<size_t>((size_t)kSplitW3 * kMlpC * kMlpH, (size_t)(batch / kSgChunk) * kSmallGrads);
         LCHK(part.alloc(4 * np));
         return 0;
     }
@@ -441,7 +500,7 @@ static hipError_t gemm_nt(const __bf16* A, const __bf16* Bt, float* C, uint32_t 
 static int mlp_loss_grad(MlpWork& w, uint32_t nb, hipStream_t st, std::string& err)
 {
     LCHK(hipMemsetAsync(w.loss.p, 0, 4, st));
-    hipLaunchKernelGGL(k_mlp_hidden, dim3(nb / 64), dim3(256), 0, st, w.keys.as<uint64_t>(), nb,
+    hipLaunchKernelGGL(k_mlp_hidden, dim3(nb / kSpb), dim3(256), 0, st, w.keys.as<uint64_t>(), nb,
                        w.params.as<float>(), w.w1b.as<__bf16>(), w.w2b.as<__bf16>(), w.h1.as<float>(),
                        w.h2.as<__bf16>(), w.h2t.as<__bf16>(), nb);
     LCHK(hipGetLastError());
@@ -466,16 +525,19 @@ static int mlp_loss_grad(MlpWork& w, uint32_t nb, hipStream_t st, std::string& e
     // dh2[nb][H] = dz[nb][C] . W3^T[H][C]^T, split over the 4096 buckets
     LCHK(gemm_nt(w.dz.as<__bf16>(), w.w3tb.as<__bf16>(), w.dh2.as<float>(), nb, kMlpH, kMlpC, kMlpC, kMlpC, kMlpH, st,
                  MlpWork::kSplitDh2));
-    hipLaunchKernelGGL(k_mlp_back_hidden, dim3(nb / 64), dim3(256), 0, st, w.dh2.as<float>(), MlpWork::kSplitDh2,
-                       (size_t)nb * kMlpH, w.h2.as<__bf16>(), w.h1.as<float>(), w.w2b.as<__bf16>(), nb,
-                       w.da2.as<float>(), w.da1.as<float>());
+    // the slabs are summed into slab 0 (element-parallel, fixed order)
+    hipLaunchKernelGGL(k_sum_slabs, dim3(nb * kMlpH / 256), dim3(256), 0, st, w.dh2.as<float>(), MlpWork::kSplitDh2,
+                       (size_t)nb * kMlpH, nb * kMlpH, w.dh2.as<float>());
+    LCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_back_hidden, dim3(nb / kSpb), dim3(256), 0, st, w.dh2.as<float>(), w.h2.as<__bf16>(),
+                       w.h1.as<float>(), w.w2b.as<__bf16>(), nb, w.da2.as<float>(), w.da1.as<float>());
     LCHK(hipGetLastError());
     // the per-workgroup partials reuse the dW3 slabs, which are consumed above
-    hipLaunchKernelGGL(k_mlp_small_grads, dim3(nb / 64), dim3(256), 0, st, w.da2.as<float>(), w.h1.as<float>(),
+    hipLaunchKernelGGL(k_mlp_small_grads, dim3(nb / kSgChunk), dim3(256), 0, st, w.da2.as<float>(), w.h1.as<float>(),
                        w.da1.as<float>(), w.keys.as<uint64_t>(), w.part.as<float>());
     LCHK(hipGetLastError());
     hipLaunchKernelGGL(k_small_grads_reduce, dim3((kSmallGrads + 255) / 256), dim3(256), 0, st, w.part.as<float>(),
-                       nb / 64, g);
+                       nb / kSgChunk, g);
     LCHK(hipGetLastError());
     return 0;
 }
@@ -578,32 +640,36 @@ int mlp_train(const uint64_t* d_sorted, uint64_t n, const MlpTrainOpts& o, hipSt
     return 0;
 }
 
-int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, hipStream_t st, std::vector<uint16_t>& pred,
-                std::string& err)
+int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, bool expect, hipStream_t st,
+                std::vector<float>& pred, std::string& err)
 {
     const uint32_t chunk = 8192;
     MlpWork w;
     if (w.init(chunk, err, false)) return -3;
     if (upload_params(w, params, st, err)) return -2;
     DevBuf out;
-    LCHK(out.alloc(2ULL * (n + chunk)));
+    LCHK(out.alloc(4ULL * (n + chunk)));
     for (uint64_t i = 0; i < n; i += chunk) {
         const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, n - i);
         const uint32_t mp = (m + 63) / 64 * 64;
         LCHK(hipMemsetAsync(w.keys.p, 0, 8ULL * mp, st));
         LCHK(hipMemcpyAsync(w.keys.p, d_keys + i, 8ULL * m, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_mlp_hidden, dim3(mp / 64), dim3(256), 0, st, w.keys.as<uint64_t>(), mp,
+        hipLaunchKernelGGL(k_mlp_hidden, dim3(mp / kSpb), dim3(256), 0, st, w.keys.as<uint64_t>(), mp,
                            w.params.as<float>(), w.w1b.as<__bf16>(), w.w2b.as<__bf16>(), w.h1.as<float>(),
                            w.h2.as<__bf16>(), (__bf16*)nullptr, mp);
         LCHK(hipGetLastError());
         LCHK(gemm_nt(w.h2.as<__bf16>(), w.w3b.as<__bf16>(), w.logits.as<float>(), mp, kMlpC, kMlpH, kMlpH, kMlpH,
                      kMlpC, st));
-        hipLaunchKernelGGL(k_argmax, dim3(m), dim3(256), 0, st, w.logits.as<float>(), w.params.as<float>() + kMlpOffB3,
-                           m, out.as<uint16_t>() + i);
+        if (expect)
+            hipLaunchKernelGGL(k_expect, dim3(m), dim3(256), 0, st, w.logits.as<float>(),
+                               w.params.as<float>() + kMlpOffB3, m, out.as<float>() + i);
+        else
+            hipLaunchKernelGGL(k_argmax, dim3(m), dim3(256), 0, st, w.logits.as<float>(),
+                               w.params.as<float>() + kMlpOffB3, m, out.as<float>() + i);
         LCHK(hipGetLastError());
     }
     pred.resize(n);
-    LCHK(hipMemcpyAsync(pred.data(), out.p, 2ULL * n, hipMemcpyDeviceToHost, st));
+    LCHK(hipMemcpyAsync(pred.data(), out.p, 4ULL * n, hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -611,23 +677,25 @@ int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, hipStre
 int mlp_debug_fit(const uint64_t* sorted, uint64_t n, const MlpTrainOpts& o, uint64_t* bounds, float* loss_curve,
                   uint32_t loss_cap, std::string& err)
 {
+    std::vector<uint64_t> keys(sorted, sorted + n);
+    if (o.distinct) keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     DevBuf d;
-    LCHK(d.alloc(8ULL * n));
-    LCHK(hipMemcpy(d.p, sorted, 8ULL * n, hipMemcpyHostToDevice));
+    LCHK(d.alloc(8ULL * keys.size()));
+    LCHK(hipMemcpy(d.p, keys.data(), 8ULL * keys.size(), hipMemcpyHostToDevice));
     std::vector<float> params, curve;
-    int rc = mlp_train(d.as<uint64_t>(), n, o, nullptr, params, curve, err);
+    int rc = mlp_train(d.as<uint64_t>(), keys.size(), o, nullptr, params, curve, err);
     if (rc) return rc;
-    std::vector<uint16_t> pred;
-    rc = mlp_predict(params.data(), d.as<uint64_t>(), n, nullptr, pred, err);
+    std::vector<float> pred;
+    rc = mlp_predict(params.data(), d.as<uint64_t>(), keys.size(), o.expect, nullptr, pred, err);
     if (rc) return rc;
-    std::vector<uint64_t> keys(sorted, sorted + n), b;
+    std::vector<uint64_t> b;
     boundaries_from_predictions(keys, pred, b);
     std::memcpy(bounds, b.data(), sizeof(uint64_t) * b.size());
     for (uint32_t i = 0; i < loss_cap && i < curve.size(); ++i) loss_curve[i] = curve[i];
     return 0;
 }
 
-void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std::vector<uint16_t>& pred,
+void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std::vector<float>& pred,
                                  std::vector<uint64_t>& b)
 {
     // Isotonic (non-decreasing) least-squares fit of the predicted bucket
@@ -655,8 +723,8 @@ void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std:
     uint32_t next = 1;
     size_t i = 0;
     for (size_t blk = 0; blk < val.size() && next < kMlpC; ++blk) {
-        // a pooled block predicts bucket round(val); its first key opens every
-        // bucket up to that one
+        // a pooled block's keys belong to bucket round(val); its first key opens
+        // every bucket up to that one
         const uint32_t q = (uint32_t)std::min<double>(kMlpC - 1, std::floor(val[blk] + 0.5));
         while (next <= q && next < kMlpC) b[next++] = sorted[i];
         i += len[blk];

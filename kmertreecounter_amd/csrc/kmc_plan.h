@@ -37,7 +37,9 @@ void bloom_capacity(const uint64_t* counts256, uint64_t est_file_size, uint64_t*
 
 int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sample, hipStream_t st,
                 PlanResult& out, std::string& err);
-// Train the CDF model on the sample's keys and fill out.learned_b (and its stats).
+// Train the CDF model on the sample's keys and fill out.learned_b (and its
+// stats).  A sample with fewer distinct keys than buckets leaves learned_b
+// empty: the run then uses the prefix partition.
 int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, PlanResult& out, std::string& err);
 
 }  // namespace kmc

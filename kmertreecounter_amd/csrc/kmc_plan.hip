@@ -168,33 +168,37 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     for (uint64_t i = 0; i < nkeys; i += stride) sorted.push_back(all[i * nw]);
     std::vector<uint64_t>().swap(all);
     std::sort(sorted.begin(), sorted.end());
-    if (sorted.size() < 64) {
+    MlpTrainOpts o;
+    // the network learns the quantiles of the keys it is shown
+    std::vector<uint64_t> uniq(sorted);
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    const std::vector<uint64_t>& train = o.distinct ? uniq : sorted;
+    if (uniq.size() < kMlpC) {
+        // fewer distinct sample keys than buckets: nothing to learn, the run
+        // keeps the prefix partition (learned_b stays empty)
         cleanup();
-        err = "learned plan: too few sample keys";
-        return KC_ERR_STATE;
+        out.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return KC_OK;
     }
-    e = hipMemcpy(d_keys, sorted.data(), sorted.size() * 8, hipMemcpyHostToDevice);
+    e = hipMemcpy(d_keys, train.data(), train.size() * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         cleanup();
         err = std::string("learned plan: ") + hipGetErrorString(e);
         return KC_ERR_HIP;
     }
-    MlpTrainOpts o;
     std::vector<float> params;
-    int rc = mlp_train(d_keys, sorted.size(), o, st, params, out.loss_curve, err);
-    std::vector<uint16_t> pred;
-    if (!rc) rc = mlp_predict(params.data(), d_keys, sorted.size(), st, pred, err);
+    int rc = mlp_train(d_keys, train.size(), o, st, params, out.loss_curve, err);
+    std::vector<float> pred;
+    if (!rc) rc = mlp_predict(params.data(), d_keys, train.size(), o.expect, st, pred, err);
     cleanup();
     if (rc) return KC_ERR_HIP;
-    boundaries_from_predictions(sorted, pred, out.learned_b);
-    out.train_keys = sorted.size();
+    boundaries_from_predictions(train, pred, out.learned_b);
+    out.train_keys = train.size();
     std::vector<uint64_t> lb(out.learned_b.begin(), out.learned_b.begin() + kMlpC);
     out.sample_max_over_mean_learned = imbalance(sorted, lb);
     std::vector<uint64_t> pb(kMlpC);
     for (uint32_t j = 0; j < kMlpC; ++j) pb[j] = (uint64_t)j << 52;
     out.sample_max_over_mean_prefix = imbalance(sorted, pb);
-    std::vector<uint64_t> uniq(sorted);
-    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
     out.sample_distinct_learned = imbalance(uniq, lb);
     out.sample_distinct_prefix = imbalance(uniq, pb);
     out.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -94,5 +94,22 @@ def test_learned_partition_balances_skewed_input_and_stays_exact():
     # single keys bound max/mean by occurrences for any key-range partition, so
     # balance is also compared over distinct keys
     assert sl.learned_sample_max_over_mean <= sl.prefix_sample_max_over_mean
-    assert sl.learned_sample_distinct_max_over_mean < 1.5
+    # measured 1.13-1.15 over distinct keys on this sample (several seeds and
+    # training lengths); the 12-bit prefixes give 8.45
+    assert sl.learned_sample_distinct_max_over_mean < 1.2
     assert sl.learned_sample_distinct_max_over_mean < 0.5 * sl.prefix_sample_distinct_max_over_mean
+    # fewer hash-range passes in the counting kernel
+    assert sl.extra_passes <= sp.extra_passes
+
+
+def test_learned_plan_falls_back_to_prefix_on_a_degenerate_sample():
+    # one distinct key in the sample: nothing to learn, the prefix partition runs
+    seq = (b"A" * 149 + b"\n") * 20000 + (b"ACGTTGCA" * 18 + b"\n") * 10
+    keys, counts = C.count_table(C.extract(seq, 31))
+    e = learned_engine_run(seq, 31, K.KC_PARTITION_LEARNED, sample_frac=0.5)
+    s = e.stats()
+    assert s.learned_train_keys == 0
+    mk, mc = C.merge_outputs(e.firsts(), e.records(), 1)
+    np.testing.assert_array_equal(mk, keys)
+    np.testing.assert_array_equal(mc, counts)
+    e.close()

@@ -110,7 +110,7 @@ def test_fanout_does_not_change_table(sub_bits):
     keys, counts = C.count_table(C.extract(seq, 31))
     f, r, s = run_engine([seq], 31, sub_bits=sub_bits)
     assert s.fine_buckets == 256 << (0 if sub_bits == K.KC_SUB_BITS_NONE else sub_bits)
-    check_outputs(f, r, s, 31, keys, counts, exact=True)
+    check_outputs(f, r, s, 31, keys, counts)
 
 
 def test_block_carry_path_matches():
@@ -153,9 +153,57 @@ def test_exact_counter_narrowed_passes(k, table_slots, pass_keys):
     seq = K.generate(1, scale=40, seed=22, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, k))
     # 256 buckets of ~4.7k keys (~1.4k distinct)
-    f, r, s = run_engine([seq], k, sub_bits=K.KC_SUB_BITS_NONE, table_slots=table_slots, pass_keys=pass_keys)
+    f, r, s = run_engine([seq], k, sub_bits=K.KC_SUB_BITS_NONE, table_slots=table_slots, pass_keys=pass_keys,
+                         counter=K.KC_COUNTER_EXACT)
     assert s.extra_passes > 0
     check_outputs(f, r, s, k, keys, counts, exact=True)
+
+
+@COUNTERS
+@pytest.mark.parametrize("heavy_keys", [1000, 9500])
+def test_heavy_buckets_counted_in_chunks(counter, heavy_keys):
+    # buckets above heavy_keys keys are listed in chunks and counted by the whole
+    # GPU into the global table; 256 buckets of ~9.4k keys: 1000 hands over every
+    # bucket (~9 chunks of 1024 keys merged across workgroups), 9500 the larger
+    # half (both paths in one run)
+    seq = K.generate(1, scale=20, seed=8, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    f, r, s = run_engine([seq], 31, sub_bits=K.KC_SUB_BITS_NONE, heavy_keys=heavy_keys, counter=counter)
+    assert 0 < s.heavy_buckets <= 256
+    if heavy_keys == 9500:
+        assert s.heavy_buckets < 256
+    check_outputs(f, r, s, 31, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+
+
+def microsatellite_reads(n_reads, seed, tract_frac=0.3):
+    """150 bp reads of a random genome whose tracts repeat units of period 1-6."""
+    rng = np.random.default_rng(seed)
+    g = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=2_000_000)
+    for j in range(0, g.size, 10_000):
+        if rng.random() < tract_frac:
+            p = int(rng.integers(1, 7))
+            unit = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=p)
+            g[j:j + 5000] = np.resize(unit, 5000)
+    starts = rng.integers(0, g.size - 150, size=n_reads)
+    out = np.empty((n_reads, 151), dtype=np.uint8)
+    out[:, :150] = g[starts[:, None] + np.arange(150)[None, :]]
+    out[:, 150] = ord("\n")
+    return out.tobytes()
+
+
+@COUNTERS
+@pytest.mark.parametrize("k", [31, 63])
+def test_microsatellite_heavy_hitters(counter, k):
+    # a few keys repeated hundreds of thousands of times, in runs: the wave-level
+    # run counting and the chunked heavy-bucket path must keep every count exact
+    seq = microsatellite_reads(200_000, 11 + k)
+    keys, counts = C.count_table(C.extract(seq, k))
+    assert counts.max() > 100_000
+    f, r, s = run_engine([seq], k, heavy_keys=50_000, counter=counter)
+    assert s.heavy_buckets > 0
+    check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+    f, r, s = run_engine([seq], k, counter=counter)   # default threshold: one workgroup per bucket
+    check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
 
 
 @COUNTERS
