@@ -454,24 +454,27 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
 
 // Runs of equal keys in consecutive lanes of a wave (the lanes read consecutive
 // keys of a bucket, and a microsatellite's repeated key stays clustered through
-// both scatters).  Returns this lane's run length if it heads a run (the run
-// stops at an invalid lane), 0 if it continues one or is invalid.
+// both scatters).  Each lane compares its key with the previous lane's through
+// a DPP row shift (row_shr:1, within rows of 16 lanes), so runs are cut at row
+// starts: lanes 0, 16, 32 and 48 always head a run.  Invalid lanes (past the
+// end of the range) are only ever the top lanes of a step, so a valid lane's
+// previous lane is valid.  Returns this lane's run length if it heads a run,
+// 0 if it continues one or is invalid.
 template <int NW>
 __device__ __forceinline__ uint32_t run_head_length(const Key<NW>& key, bool valid)
 {
+    constexpr int kRowShr1 = 0x111;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    bool same = valid && lane > 0;
+    bool same = valid && (lane & 15u) != 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        const unsigned long long prev = __shfl_up((unsigned long long)key.w[w], 1, kWave);
-        same = same && prev == key.w[w];
+        const int lo = (int)(uint32_t)key.w[w], hi = (int)(uint32_t)(key.w[w] >> 32);
+        const int plo = __builtin_amdgcn_update_dpp(0, lo, kRowShr1, 0xF, 0xF, true);
+        const int phi = __builtin_amdgcn_update_dpp(0, hi, kRowShr1, 0xF, 0xF, true);
+        same = same && plo == lo && phi == hi;
     }
-    const uint64_t pvalid = __ballot(valid);
-    const bool prev_valid = lane > 0 && ((pvalid >> (lane > 0 ? lane - 1 : 0)) & 1ULL);
-    same = same && prev_valid;
-    const bool head = !same;  // invalid lanes head empty runs, so no run crosses them
-    const uint64_t heads = __ballot(head);
-    if (!valid || !head) return 0;
+    const uint64_t heads = __ballot(!same);  // invalid lanes head empty runs, so no run crosses them
+    if (!valid || same) return 0;
     const uint64_t above = lane == 63 ? 0ULL : (heads >> (lane + 1));
     return above ? (uint32_t)__ffsll((long long)above) : 64u - lane;
 }
@@ -650,15 +653,25 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
         if (s_abort) break;
         const uint32_t hstart = s_hstart;
         const uint32_t hend = min(65536u, hstart + s_hwidth);
+        // kLoads keys per thread and step, and the next step's keys are loaded
+        // before this step's are processed: the loads stay in flight while
+        // the current keys are hashed and counted
+        Key<NW> nb[kLoads];
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+            const unsigned long long i = lo + (unsigned long long)u * T + t;
+            if (i < hi) nb[u] = load_key<NW>(B, i);
+        }
         for (unsigned long long it = 0; it < iters; ++it) {
-            // kLoads keys per thread in flight before any is processed: one
-            // outstanding load per thread cannot cover HBM latency
             Key<NW> kb[kLoads];
 #pragma unroll
+            for (int u = 0; u < kLoads; ++u) kb[u] = nb[u];
+#pragma unroll
             for (int u = 0; u < kLoads; ++u) {
-                const unsigned long long i = lo + (it * kLoads + u) * T + t;
-                if (i < hi) kb[u] = load_key<NW>(B, i);
+                const unsigned long long i = lo + ((it + 1) * kLoads + u) * T + t;
+                if (i < hi) nb[u] = load_key<NW>(B, i);
             }
+            uint64_t fbal[kLoads];  // first occurrences of each key slot, one reservation per step
 #pragma unroll
             for (int u = 0; u < kLoads; ++u) {
                 const unsigned long long i = lo + (it * kLoads + u) * T + t;
@@ -688,12 +701,19 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                         }
                     }
                 }
-                // first occurrences: one LDS atomic per wave, consecutive slots per wave
-                const uint64_t bal = __ballot(first);
-                uint32_t wbase = 0;
-                if (lane == 0 && bal) wbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bal));
-                wbase = __shfl(wbase, 0, kWave);
-                if (first) store_key<NW>(args.firsts, lo + wbase + lanes_below(bal), key);
+                fbal[u] = __ballot(first);
+            }
+            // first occurrences of the step: one LDS atomic per wave, consecutive slots
+            uint32_t nf = 0;
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) nf += (uint32_t)__popcll(fbal[u]);
+            uint32_t wbase = 0;
+            if (lane == 0 && nf) wbase = atomicAdd(&s_nfirst, nf);
+            wbase = __shfl(wbase, 0, kWave);
+#pragma unroll
+            for (int u = 0; u < kLoads; ++u) {
+                if ((fbal[u] >> lane) & 1ULL) store_key<NW>(args.firsts, lo + wbase + lanes_below(fbal[u]), kb[u]);
+                wbase += (uint32_t)__popcll(fbal[u]);
             }
         }
         __syncthreads();
@@ -1066,16 +1086,18 @@ __global__ void __launch_bounds__(256)
 k_ovf_hist(const uint8_t* __restrict__ ovf, const unsigned long long* __restrict__ n_ptr, uint64_t cap,
            unsigned long long* __restrict__ root_ovf)
 {
+    // root histogram of the overflow records: LDS per workgroup, then one
+    // global add per (workgroup, root); root_ovf starts zeroed
     __shared__ uint32_t s_h[256];
     s_h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t n = *n_ptr < cap ? *n_ptr : cap;
-    for (uint64_t i = threadIdx.x; i < n; i += 256) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
         const uint64_t w0 = *reinterpret_cast<const uint64_t*>(ovf + i * RecBytes<NW>::value);
         atomicAdd(&s_h[w0 >> 56], 1u);
     }
     __syncthreads();
-    root_ovf[threadIdx.x] = s_h[threadIdx.x];
+    if (s_h[threadIdx.x]) atomicAdd(&root_ovf[threadIdx.x], (unsigned long long)s_h[threadIdx.x]);
 }
 
 // Exclusive scans of the per-bucket outputs (one workgroup of 1024 threads):
@@ -1203,6 +1225,9 @@ k_root_scan(SegArgs a)
     if (threadIdx.x == 255) a.root_off[256] = all;
 }
 
+// Overflow records into their root's tail range: tiles of 2048 records are
+// ranked by root in LDS, and each root's part of a tile reserves its range
+// with one global add.
 template <int NW>
 __global__ void __launch_bounds__(256)
 k_ovf_scatter(const uint8_t* __restrict__ ovf, const unsigned long long* __restrict__ n_ptr, uint64_t cap,
@@ -1210,14 +1235,39 @@ k_ovf_scatter(const uint8_t* __restrict__ ovf, const unsigned long long* __restr
               const unsigned long long* __restrict__ ctr)
 {
     constexpr uint32_t RW = RecBytes<NW>::value / 8;
+    constexpr uint32_t kPer = 8;
+    constexpr uint32_t kTile = 256 * kPer;
+    __shared__ uint32_t s_h[256];
+    __shared__ unsigned long long s_base[256];
     if (ctr[kCtrErr] & kErrRecOut) return;
     const uint64_t n = *n_ptr < cap ? *n_ptr : cap;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(ovf + i * (RW * 8));
-        const unsigned long long pos = atomicAdd(&ovf_cursor[src[0] >> 56], 1ULL);
-        uint64_t* d = reinterpret_cast<uint64_t*>(rec_out + pos * (RW * 8));
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < n; t0 += (uint64_t)gridDim.x * kTile) {
+        s_h[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t root[kPer], rank[kPer];
 #pragma unroll
-        for (uint32_t w = 0; w < RW; ++w) d[w] = src[w];
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint64_t i = t0 + j * 256 + threadIdx.x;
+            if (i < n) {
+                root[j] = (uint32_t)(*reinterpret_cast<const uint64_t*>(ovf + i * (RW * 8)) >> 56);
+                rank[j] = atomicAdd(&s_h[root[j]], 1u);
+            }
+        }
+        __syncthreads();
+        const uint32_t h = s_h[threadIdx.x];
+        if (h) s_base[threadIdx.x] = atomicAdd(&ovf_cursor[threadIdx.x], (unsigned long long)h);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint64_t i = t0 + j * 256 + threadIdx.x;
+            if (i < n) {
+                const uint64_t* src = reinterpret_cast<const uint64_t*>(ovf + i * (RW * 8));
+                uint64_t* d = reinterpret_cast<uint64_t*>(rec_out + (s_base[root[j]] + rank[j]) * (RW * 8));
+#pragma unroll
+                for (uint32_t w = 0; w < RW; ++w) d[w] = src[w];
+            }
+        }
+        __syncthreads();  // s_h and s_base are reused by the next tile
     }
 }
 
@@ -1317,7 +1367,7 @@ template <int NW>
 hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_t* firsts_in, uint64_t* firsts_out,
                            const uint8_t* rec_in, uint8_t* rec_out, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_ovf_hist<NW>), dim3(1), dim3(256), 0, st, ca.ovf, ca.ovf_count, ca.ovf_cap, sa.root_ovf);
+    hipLaunchKernelGGL((k_ovf_hist<NW>), dim3(256), dim3(256), 0, st, ca.ovf, ca.ovf_count, ca.ovf_cap, sa.root_ovf);
     KMC_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, st, sa);
     KMC_LAUNCH_CHECK();
