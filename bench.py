@@ -206,6 +206,10 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
             per_phase[name] += v / steps
     wall_ms = (time.perf_counter() - t_wall) * 1e3 / steps
     log(f"[{partition}] {steps} steps: {wall_ms:.3f} ms/step wall, device " + ", ".join(f"{v:.3f}" for v in dev))
+    # histogram_tool's precise histogram straight from the device tables (outside the timed steps)
+    t_h = time.perf_counter()
+    eng.histogram(1, 10000)
+    hist_ms = (time.perf_counter() - t_h) * 1e3
     r = {
         "partition": partition,
         "ms_per_step": wall_ms,
@@ -215,6 +219,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         "per_phase": per_phase,
         "plan_ms": plan_ms,
         "max_over_mean_bucket": st.max_bucket_keys / max(st.mean_bucket_keys, 1e-9),
+        "histogram_ms": hist_ms,
     }
     if part == K.KC_PARTITION_LEARNED:
         r["learned"] = {
@@ -238,7 +243,8 @@ def summary(r: dict) -> dict:
         "kmers_per_s": r["kmers_per_s"], "ms_per_step": r["ms_per_step"],
         "device_ms_per_step": r["device_ms_per_step"], "fine_buckets": st.fine_buckets,
         "max_over_mean_bucket": r["max_over_mean_bucket"], "extra_passes": st.extra_passes,
-        "spilled_keys": st.spilled_keys, "phase_ms": r["per_phase"],
+        "spilled_keys": st.spilled_keys, "heavy_buckets": st.heavy_buckets, "phase_ms": r["per_phase"],
+        "histogram_ms": r["histogram_ms"],
     }
 
 

@@ -146,6 +146,19 @@ typedef struct kc_engine kc_engine;
 /* Records callback: `records` holds n ExportRecord<N> structs (N key words, u32
  * count, zero padding to 8 bytes: 16 bytes at N=1, 24 at N=2). */
 typedef int (*kc_records_cb)(void* user, const void* records, uint64_t n);
+/* Count histogram of the counted results, without writing low.bin and the
+ * root files (replaces `histogram_tool precise|approximate`,
+ * tool/histogram_tool.cpp:445-527 and :551-592).  hist[f - min_freq] for f in
+ * [min_freq, max_freq], as the tool writes it (tool/histogram_tool.cpp:425-442):
+ * PRECISE counts every k-mer at its merged count (with the tool's rule that a
+ * record of count r < min_freq - 1 is skipped); APPROXIMATE counts the root
+ * records only. */
+typedef enum kc_histogram_mode {
+    KC_HISTOGRAM_PRECISE = 0,
+    KC_HISTOGRAM_APPROXIMATE = 1
+} kc_histogram_mode;
+KC_API int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_freq, int64_t* hist);
+
 /* Keys callback: `keys` holds n keys of N u64 words each. */
 typedef int (*kc_keys_cb)(void* user, const uint64_t* keys, uint64_t n);
 

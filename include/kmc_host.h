@@ -53,10 +53,18 @@ typedef struct kmc_tree_opts {
     uint32_t verbose;
     uint64_t sample_bytes;  /* Stage-0 sample size (0 = 256 MiB, the reference's cap) */
     uint32_t counter;       /* kc_counter */
-    uint32_t reserved;
+    uint32_t histogram_only; /* 1: skip low.bin and the root files (needs histogram_path) */
+    const char* histogram_path; /* if set: write the precise count histogram here, as histogram_tool does */
+    uint32_t hist_min;      /* histogram range (0 = 1 and 10000, the tool's defaults) */
+    uint32_t hist_max;
 } kmc_tree_opts;
 
 KC_API int kmc_tree_run(const kmc_tree_opts* opts, kc_stats* stats, double* wall_ms);
+
+/* Histogram file as histogram_tool writes it (tool/histogram_tool.cpp:425-442):
+ * one line `f<TAB>n` for each f in [min_freq, max_freq]; hist has
+ * max_freq - min_freq + 1 entries (kc_histogram's output). */
+KC_API int kmc_write_histogram(const char* path, const int64_t* hist, uint32_t min_freq, uint32_t max_freq);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,8 @@ KC_PARTITION_LEARNED = 1
 KC_SUB_BITS_NONE = 0xFFFFFFFF
 KC_COUNTER_BLOOM = 0
 KC_COUNTER_EXACT = 1
+KC_HISTOGRAM_PRECISE = 0
+KC_HISTOGRAM_APPROXIMATE = 1
 
 
 class KmcError(RuntimeError):
@@ -90,7 +92,8 @@ class TreeOpts(ctypes.Structure):
         ("fastq_path", ctypes.c_char_p), ("tmp_dir", ctypes.c_char_p), ("k", ctypes.c_uint32),
         ("n_thread", ctypes.c_uint32), ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
         ("partition", ctypes.c_uint32), ("verbose", ctypes.c_uint32), ("sample_bytes", ctypes.c_uint64),
-        ("counter", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+        ("counter", ctypes.c_uint32), ("histogram_only", ctypes.c_uint32), ("histogram_path", ctypes.c_char_p),
+        ("hist_min", ctypes.c_uint32), ("hist_max", ctypes.c_uint32),
     ]
 
 
@@ -103,7 +106,8 @@ EXPORTED = [
     "kc_emit_root", "kc_stats_get", "kc_destroy", "kc_count_resident", "kc_words_for_k",
     "kc_last_error", "kc_debug_xxh3", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
-    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy",
+    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
+    "kmc_write_histogram",
 ]
 
 _lib = None
@@ -139,6 +143,8 @@ def lib() -> ctypes.CDLL:
     L.kc_last_error.restype = cp
     L.kc_debug_xxh3.argtypes = [vp, u64, u32, vp]
     L.kc_debug_copy.argtypes = [u64, u32, ctypes.POINTER(ctypes.c_float)]
+    L.kc_histogram.argtypes = [vp, u32, u32, u32, vp]
+    L.kmc_write_histogram.argtypes = [cp, vp, u32, u32]
     L.kc_debug_gemm_bf16.argtypes = [vp, vp, vp, u32, u32, u32]
     L.kc_debug_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
     L.kc_debug_learned_fit.argtypes = [vp, u64, u32, ctypes.c_float, u32, u64, u32, vp, vp, u32]
@@ -267,6 +273,14 @@ class Engine:
     def count_resident(self) -> None:
         self._check(self._L.kc_count_resident(self._h), "kc_count_resident")
 
+    def histogram(self, min_freq: int = 1, max_freq: int = 10000, precise: bool = True) -> np.ndarray:
+        """histogram_tool's count histogram (precise or approximate) of the counted
+        results: element f - min_freq holds the number of k-mers of count f."""
+        out = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
+        mode = KC_HISTOGRAM_PRECISE if precise else KC_HISTOGRAM_APPROXIMATE
+        self._check(self._L.kc_histogram(self._h, mode, min_freq, max_freq, out.ctypes.data), "kc_histogram")
+        return out
+
     def stats(self) -> Stats:
         s = Stats()
         self._check(self._L.kc_stats_get(self._h, ctypes.byref(s)), "kc_stats_get")
@@ -375,11 +389,21 @@ def debug_learned_fit(sorted_keys: np.ndarray, steps: int = 8000, lr: float = 1e
     return b, lc
 
 
+def write_histogram(path: str, hist: np.ndarray, min_freq: int = 1) -> None:
+    """Write a histogram as histogram_tool does: `f<TAB>n` lines for f = min_freq.."""
+    h = np.ascontiguousarray(hist, dtype=np.int64)
+    rc = lib().kmc_write_histogram(str(path).encode(), h.ctypes.data, min_freq, min_freq + h.size - 1)
+    if rc != 0:
+        raise KmcError(f"kmc_write_histogram failed: {rc}")
+
+
 def tree_run(fastq_path: str, tmp_dir: str, k: int, n_thread: int = 8, min_count: int = 1, max_count: int = 0,
              partition: int = KC_PARTITION_PREFIX, verbose: bool = False, sample_bytes: int = 0,
-             counter: int = KC_COUNTER_BLOOM) -> tuple[Stats, float]:
+             counter: int = KC_COUNTER_BLOOM, histogram_path: str | None = None, histogram_only: bool = False,
+             hist_min: int = 0, hist_max: int = 0) -> tuple[Stats, float]:
     o = TreeOpts(str(fastq_path).encode(), str(tmp_dir).encode(), k, n_thread, min_count, max_count, partition,
-                 1 if verbose else 0, sample_bytes, counter, 0)
+                 1 if verbose else 0, sample_bytes, counter, 1 if histogram_only else 0,
+                 str(histogram_path).encode() if histogram_path else None, hist_min, hist_max)
     s = Stats()
     wall = ctypes.c_double()
     rc = lib().kmc_tree_run(ctypes.byref(o), ctypes.byref(s), ctypes.byref(wall))

@@ -657,6 +657,68 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     return KC_OK;
 }
 
+int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_freq, int64_t* hist)
+{
+    if (!h || !hist || min_freq == 0 || max_freq < min_freq || mode > KC_HISTOGRAM_APPROXIMATE) return KC_ERR_ARG;
+    if (!h->have_results) return h->fail(KC_ERR_STATE, "no results: call kc_finish first");
+    HIPX(hipSetDevice(h->device));
+    const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
+    const bool precise = mode == KC_HISTOGRAM_PRECISE;
+    long long* d_hist = nullptr;
+    unsigned long long* d_err = nullptr;
+    unsigned long long* d_keys = nullptr;
+    uint32_t* d_vals = nullptr;
+    auto release = [&]() { hipFree(d_hist); hipFree(d_err); hipFree(d_keys); hipFree(d_vals); d_keys = nullptr; d_vals = nullptr; };
+    if (hipMalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || hipMalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) {
+        release();
+        return h->fail(KC_ERR_NOMEM, "histogram allocation failed");
+    }
+    // the record table is at most half full; a probe that runs out doubles it
+    uint64_t cap = precise ? pow2_at_least(std::max<uint64_t>(1024, 2 * h->n_rec)) : 0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (precise) {
+            hipFree(d_keys); hipFree(d_vals);
+            d_keys = nullptr; d_vals = nullptr;
+            if (hipMalloc(&d_keys, cap * 8ULL * h->nw) != hipSuccess || hipMalloc(&d_vals, cap * 4ULL) != hipSuccess) {
+                release();
+                return h->fail(KC_ERR_NOMEM, "histogram table allocation failed");
+            }
+        }
+        hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(long long) * nb, h->st);
+        if (e == hipSuccess) e = hipMemsetAsync(d_err, 0, sizeof(unsigned long long), h->st);
+        if (e == hipSuccess && precise) e = hipMemsetAsync(d_keys, 0xFF, cap * 8ULL * h->nw, h->st);
+        HistArgs a{};
+        a.hist = d_hist;
+        a.minf = min_freq;
+        a.maxf = max_freq;
+        a.precise = precise ? 1u : 0u;
+        a.keys = d_keys;
+        a.vals = d_vals;
+        a.cap = cap;
+        a.error_flags = d_err;
+        if (e == hipSuccess)
+            e = h->nw == 1 ? launch_histogram<1>(h->res_recs, h->n_rec, h->res_firsts, h->n_first, a, h->st)
+                           : launch_histogram<2>(h->res_recs, h->n_rec, h->res_firsts, h->n_first, a, h->st);
+        unsigned long long err = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, h->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+        if (e != hipSuccess) {
+            release();
+            return h->fail(KC_ERR_HIP, std::string("histogram: ") + hipGetErrorString(e));
+        }
+        if (err & kErrHistFull) {
+            cap *= 2;
+            continue;
+        }
+        e = hipMemcpy(hist, d_hist, sizeof(long long) * nb, hipMemcpyDeviceToHost);
+        release();
+        if (e != hipSuccess) return h->fail(KC_ERR_HIP, std::string("histogram: ") + hipGetErrorString(e));
+        return KC_OK;
+    }
+    release();
+    return h->fail(KC_ERR_CAPACITY, "histogram table did not converge");
+}
+
 int kc_stats_get(kc_engine* h, kc_stats* s)
 {
     if (!h || !s) return KC_ERR_ARG;

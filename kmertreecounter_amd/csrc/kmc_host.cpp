@@ -288,6 +288,20 @@ int kmc_write_outputs(kc_engine* h, const char* tmp_dir)
     return KC_OK;
 }
 
+int kmc_write_histogram(const char* path, const int64_t* hist, uint32_t min_freq, uint32_t max_freq)
+{
+    // `f\tn\n` for f = min_freq .. max_freq (tool/histogram_tool.cpp:425-442)
+    if (!path || !hist || max_freq < min_freq) return KC_ERR_ARG;
+    FILE* f = std::fopen(path, "w");
+    if (!f) {
+        std::fprintf(stderr, "failed to open output file: %s\n", path);
+        return KC_ERR_ARG;
+    }
+    for (uint64_t i = 0; i <= (uint64_t)max_freq - min_freq; ++i)
+        std::fprintf(f, "%llu\t%lld\n", (unsigned long long)(min_freq + i), (long long)hist[i]);
+    return std::fclose(f) == 0 ? KC_OK : KC_ERR_ARG;
+}
+
 int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
 {
     if (!o || !o->fastq_path || !o->tmp_dir || !kc_words_for_k(o->k)) return KC_ERR_ARG;
@@ -323,7 +337,18 @@ int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
     if (!rc) rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, 0, push_cb, &pc);
     if (!rc) rc = kc_finish(h);
     const auto t2 = clk::now();
-    if (!rc) rc = kmc_write_outputs(h, o->tmp_dir);
+    if (!rc && !o->histogram_only) rc = kmc_write_outputs(h, o->tmp_dir);
+    if (!rc && o->histogram_path) {
+        // the histogram_tool result straight from the device tables
+        const uint32_t lo = o->hist_min ? o->hist_min : 1, hi = o->hist_max ? o->hist_max : 10000;
+        if (hi < lo) {
+            rc = KC_ERR_ARG;
+        } else {
+            std::vector<int64_t> hist((size_t)hi - lo + 1);
+            rc = kc_histogram(h, KC_HISTOGRAM_PRECISE, lo, hi, hist.data());
+            if (!rc) rc = kmc_write_histogram(o->histogram_path, hist.data(), lo, hi);
+        }
+    }
     const auto t3 = clk::now();
     kc_stats st{};
     if (!rc) rc = kc_stats_get(h, &st);

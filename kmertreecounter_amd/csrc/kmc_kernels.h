@@ -13,6 +13,7 @@ constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, workgroup i -> 
 constexpr unsigned long long kErrSpillFull = 1ULL;
 constexpr unsigned long long kErrRecFull = 2ULL;  // overflow record list full
 constexpr unsigned long long kErrRecOut = 4ULL;   // records exceed the final record buffer
+constexpr unsigned long long kErrHistFull = 8ULL; // histogram record table: a probe ran out
 
 // P0/P1: one 16-byte word of sequence per thread, kThreads*16 positions per tile.
 template <int NW> struct ExtractCfg;
@@ -135,5 +136,20 @@ hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_
 template <int NW>
 hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t st);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
+
+// Count histogram of the resident results (precise or approximate mode).
+constexpr uint32_t kHistProbe = 4096;
+struct HistArgs {
+    long long* hist;                    // [maxf - minf + 1], zeroed by the caller
+    uint32_t minf, maxf;
+    uint32_t precise;                   // 1: records and low.bin keys; 0: records only
+    unsigned long long* keys;           // NW x cap words (structure of arrays), kEmpty = free
+    uint32_t* vals;                     // [cap] record counts
+    uint64_t cap;                       // power of two, at least twice the records
+    unsigned long long* error_flags;    // kErrHistFull (8) when a probe runs out
+};
+template <int NW>
+hipError_t launch_histogram(const uint8_t* recs, uint64_t nrec, const uint64_t* firsts, uint64_t nfirst, const HistArgs& a,
+                            hipStream_t st);
 
 }  // namespace kmc

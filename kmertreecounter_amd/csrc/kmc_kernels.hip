@@ -1404,6 +1404,108 @@ template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
 template hipError_t launch_finalize<1>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
 template hipError_t launch_finalize<2>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
+// ------------------------------------------------------------------ histogram
+
+// Count histogram of the resident results, histogram_tool's two modes
+// (tool/histogram_tool.cpp:445-527): hist[f - minf] for f in [minf, maxf].
+// Bins below minf + kHistLds are summed in LDS and added once per workgroup.
+constexpr uint32_t kHistLds = 4096;
+
+__device__ __forceinline__ void hist_add(int* s_h, long long* hist, uint32_t minf, uint32_t maxf, uint64_t f, int d)
+{
+    if (f < minf || f > maxf) return;
+    const uint64_t b = f - minf;
+    if (b < kHistLds) atomicAdd(&s_h[b], d);
+    else atomicAdd((unsigned long long*)&hist[b], (unsigned long long)(long long)d);
+}
+
+__device__ __forceinline__ void hist_flush(const int* s_h, long long* hist, uint32_t minf, uint32_t maxf)
+{
+    const uint64_t nb = (uint64_t)maxf - minf + 1;
+    for (uint32_t b = threadIdx.x; b < kHistLds && b < nb; b += blockDim.x)
+        if (s_h[b]) atomicAdd((unsigned long long*)&hist[b], (unsigned long long)(long long)s_h[b]);
+}
+
+// Records: each one is counted at its count r (precise mode skips a record
+// with r + 1 < minf, tool/HighFrequencyInsertThreadPool.h:150-177) and, in
+// precise mode, its key is entered into the table with r.
+template <int NW>
+__global__ void __launch_bounds__(256)
+k_hist_records(const uint8_t* __restrict__ recs, uint64_t nrec, HistArgs a)
+{
+    __shared__ int s_h[kHistLds];
+    for (uint32_t b = threadIdx.x; b < kHistLds; b += 256) s_h[b] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrec; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t* r = reinterpret_cast<const uint64_t*>(recs + i * RecBytes<NW>::value);
+        const uint32_t c = (uint32_t)r[NW];
+        if (a.precise && (uint64_t)c + 1 < a.minf) continue;
+        hist_add(s_h, a.hist, a.minf, a.maxf, c, 1);
+        if (!a.precise) continue;
+        Key<NW> key;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key.w[w] = r[w];
+        bool claimed;
+        const int64_t s = table_find_or_claim<NW>(a.keys, a.cap, a.cap - 1, spill_start<NW>(key), kHistProbe, key, claimed);
+        if (s >= 0) a.vals[s] = c;
+        else atomicOr(a.error_flags, kErrHistFull);
+    }
+    __syncthreads();
+    hist_flush(s_h, a.hist, a.minf, a.maxf);
+}
+
+// First-occurrence keys (precise mode, tool/LowFrequencyQueryThreadPool.h:184-232):
+// a key found with r moves one unit from bin r to bin r + 1, a key not found
+// adds one to bin 1.
+template <int NW>
+__global__ void __launch_bounds__(256)
+k_hist_firsts(const uint64_t* __restrict__ firsts, uint64_t nfirst, HistArgs a)
+{
+    __shared__ int s_h[kHistLds];
+    for (uint32_t b = threadIdx.x; b < kHistLds; b += 256) s_h[b] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nfirst; i += (uint64_t)gridDim.x * 256) {
+        const Key<NW> key = load_key<NW>(firsts, i);
+        // probe until the key or an empty slot (the table is at most half full)
+        uint64_t s = spill_start<NW>(key) & (a.cap - 1);
+        bool found = false;
+        for (uint32_t p = 0; p < kHistProbe; ++p) {
+            const unsigned long long w0 = a.keys[s];
+            if (w0 == kEmpty) break;
+            bool match = w0 == key.w[0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) match = match && a.keys[(uint64_t)w * a.cap + s] == key.w[w];
+            if (match) { found = true; break; }
+            s = (s + 1) & (a.cap - 1);
+        }
+        if (found) {
+            const uint32_t c = a.vals[s];
+            hist_add(s_h, a.hist, a.minf, a.maxf, c, -1);
+            hist_add(s_h, a.hist, a.minf, a.maxf, (uint64_t)c + 1, 1);
+        } else {
+            hist_add(s_h, a.hist, a.minf, a.maxf, 1, 1);
+        }
+    }
+    __syncthreads();
+    hist_flush(s_h, a.hist, a.minf, a.maxf);
+}
+
+template <int NW>
+hipError_t launch_histogram(const uint8_t* recs, uint64_t nrec, const uint64_t* firsts, uint64_t nfirst, const HistArgs& a,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL((k_hist_records<NW>), dim3(2048), dim3(256), 0, st, recs, nrec, a);
+    KMC_LAUNCH_CHECK();
+    if (a.precise) {
+        hipLaunchKernelGGL((k_hist_firsts<NW>), dim3(2048), dim3(256), 0, st, firsts, nfirst, a);
+        KMC_LAUNCH_CHECK();
+    }
+    return hipSuccess;
+}
+
+template hipError_t launch_histogram<1>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
+template hipError_t launch_histogram<2>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
+
 // Plain copy of n16 16-byte words: the traffic reference that HBM counters
 // of the pipeline kernels are scaled by (reads 16*n16 bytes, writes 16*n16).
 __global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16)

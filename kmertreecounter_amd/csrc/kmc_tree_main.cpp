@@ -7,7 +7,8 @@
 // histogram_tool.  The counting runs on the GPU.  KMC_PARTITION=learned picks
 // the learned partitioner (default: the 256 root-prefix partition);
 // KMC_COUNTER=exact the exact counter (default: the reference's Bloom
-// first-occurrence split).
+// first-occurrence split); KMC_HISTOGRAM=<file> also writes histogram_tool's
+// precise histogram computed on the device (KMC_HISTOGRAM_ONLY=1: no tmp/ files).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -56,6 +57,11 @@ int main(int argc, char** argv)
     o.min_count = (uint32_t)min_c;
     o.max_count = (uint32_t)max_c;
     o.partition = (part && std::strcmp(part, "learned") == 0) ? KC_PARTITION_LEARNED : KC_PARTITION_PREFIX;
+    // KMC_HISTOGRAM=<file>: also write histogram_tool's precise histogram (1..10000)
+    // from the device tables; KMC_HISTOGRAM_ONLY=1 skips the tmp/ files
+    o.histogram_path = std::getenv("KMC_HISTOGRAM");
+    const char* hist_only = std::getenv("KMC_HISTOGRAM_ONLY");
+    o.histogram_only = (o.histogram_path && hist_only && std::strcmp(hist_only, "1") == 0) ? 1u : 0u;
     const char* counter = std::getenv("KMC_COUNTER");
     o.counter = (counter && std::strcmp(counter, "exact") == 0) ? KC_COUNTER_EXACT : KC_COUNTER_BLOOM;
     o.verbose = 1;

@@ -626,7 +626,6 @@ ORC_API uint64_t orc_merge_outputs(const uint64_t* firsts, uint64_t nf, const ui
 /* tool/HighFrequencyInsertThreadPool.h:150-177,                              */
 /* tool/LowFrequencyQueryThreadPool.h:184-232).  hist has max-min+1 entries,  */
 /* index f-min.  The hash map of the tool is a sorted array here.             */
-/* ------------------------------------------------------------------------- */
 
 static uint32_t g_cmp_nw;
 static int cmp_rec(const void* a, const void* b)
@@ -681,6 +680,21 @@ ORC_API int orc_histogram_precise(const uint8_t* recs, uint64_t nr, const uint64
     }
     free(table);
     return 0;
+}
+
+/* histogram_tool approximate: the root records' counts only
+ * (tool/ApproximateHighFrequency.h:133-148, record counts outside
+ * [minf, maxf] dropped). */
+ORC_API void orc_histogram_approximate(const uint8_t* recs, uint64_t nr, uint32_t nw, uint32_t minf, uint32_t maxf,
+                                       int64_t* hist)
+{
+    const uint64_t rb = orc_record_bytes(nw);
+    memset(hist, 0, ((uint64_t)maxf - minf + 1) * sizeof(int64_t));
+    for (uint64_t i = 0; i < nr; ++i) {
+        uint32_t c;
+        memcpy(&c, recs + i * rb + nw * 8, 4);
+        if (c >= minf && c <= maxf) hist[c - minf] += 1;
+    }
 }
 
 /* Closed form: histogram of a merged (key, count) table. */

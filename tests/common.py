@@ -51,6 +51,7 @@ def oracle() -> ctypes.CDLL:
     L.orc_merge_outputs.argtypes = [vp, u64, vp, u64, u32, vp, vp]
     L.orc_histogram_precise.argtypes = [vp, u64, vp, u64, u32, u32, u32, vp, ctypes.c_int]
     L.orc_histogram_from_counts.argtypes = [vp, u64, u32, u32, vp]
+    L.orc_histogram_approximate.argtypes = [vp, u64, u32, u32, u32, vp]
     L.orc_prefix_counts.argtypes = [vp, u64, u32, vp]
     L.orc_clamp_prefix_counts.restype = u64
     L.orc_clamp_prefix_counts.argtypes = [vp]

@@ -5,6 +5,7 @@ from __future__ import annotations
 
 import ctypes
 import json
+import subprocess
 from collections import Counter
 
 import numpy as np
@@ -267,6 +268,48 @@ def test_histogram_equals_closed_form_of_merged_table():
     items = [(k[i].tolist(), int(c[i] - (0 if fp[i] else 1))) for i in range(k.shape[0]) if fp[i] or c[i] >= 2]
     for minf, maxf in ((1, 100), (2, 9), (5, 5)):
         assert precise(records(1, items), low, 1, minf, maxf) == C.histogram(c, minf, maxf).tolist()
+
+
+def approximate(recs, nw, minf, maxf):
+    rb = np.ascontiguousarray(recs)
+    h = np.zeros(maxf - minf + 1, dtype=np.int64)
+    O.orc_histogram_approximate(rb.ctypes.data, len(recs), nw, minf, maxf, h.ctypes.data)
+    return h.tolist()
+
+
+@pytest.mark.parametrize("minf,maxf", [(1, 60), (3, 20)])
+def test_histogram_modes_match_the_reference_tool(tmp_path, minf, maxf):
+    # synthetic low.bin + root files (k=31), counted by the reference's own
+    # histogram_tool (compiled into oracle/_ref) in both modes
+    tool = C.ROOT / "oracle" / "_ref" / "histogram_tool"
+    if not tool.exists():
+        pytest.skip("oracle/_ref not built here (reference sources absent)")
+    rng = np.random.default_rng(minf)
+    keys = np.unique(rng.integers(0, 2**62, size=30000, dtype=np.uint64) << np.uint64(2))
+    counts = rng.geometric(0.15, size=keys.size).astype(np.uint32)
+    fp = rng.random(keys.size) < 0.05          # Bloom false positives: no low.bin entry
+    in_rec = fp | (counts >= 2)
+    rec = np.zeros(int(in_rec.sum()), dtype=np.dtype([("key", "<u8", (1,)), ("count", "<u4"), ("pad", "<u4")]))
+    rec["key"][:, 0] = keys[in_rec]
+    rec["count"] = np.where(fp[in_rec], counts[in_rec], counts[in_rec] - 1)
+    low = np.ascontiguousarray(keys[~fp])
+    buf = (ctypes.c_uint8 * (O.orc_packed_bytes(31) * low.size))()
+    O.orc_pack_low(low.ctypes.data, low.size, 31, 1, buf)
+    (tmp_path / "low.bin").write_bytes(bytes(buf))
+    roots = rec["key"][:, 0] >> np.uint64(56)
+    for r in range(256):
+        (tmp_path / f"root_{r}.bin").write_bytes(rec[roots == r].tobytes())
+    for mode in ("precise", "approximate"):
+        out = tmp_path / f"{mode}.tsv"
+        subprocess.run([str(tool), mode, str(tmp_path), "31", "4", "1", str(out), str(minf), str(maxf)],
+                       check=True, capture_output=True, timeout=120)
+        got = [int(line.split("\t")[1]) for line in out.read_text().splitlines()]
+        want = (precise(rec, low.reshape(-1, 1), 1, minf, maxf) if mode == "precise"
+                else approximate(rec, 1, minf, maxf))
+        assert got == want, mode
+    # with minf = 1 the precise histogram is the merged table's
+    if minf == 1:
+        assert precise(rec, low.reshape(-1, 1), 1, minf, maxf) == C.histogram(counts, minf, maxf).tolist()
 
 
 # ------------------------------------------------------------------ Stage 0
