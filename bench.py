@@ -192,7 +192,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     st = eng.stats()
     log(f"[{partition}] resident in {time.time() - t0:.1f} s; K={st.total_kmers} firsts={st.firsts} "
         f"records={st.records} spilled={st.spilled_keys} fine_buckets={st.fine_buckets} "
-        f"max/mean={st.max_bucket_keys / max(st.mean_bucket_keys, 1):.2f} extra_passes={st.extra_passes}")
+        f"max/mean={st.max_bucket_keys / max(st.mean_bucket_keys, 1):.2f} extra_passes={st.extra_passes} "
+        f"rounds={st.rounds}")
     for _ in range(max(0, warmup - 1)):
         eng.count_resident()
     per_phase = {name: 0.0 for name in PHASES}
@@ -243,7 +244,8 @@ def summary(r: dict) -> dict:
         "kmers_per_s": r["kmers_per_s"], "ms_per_step": r["ms_per_step"],
         "device_ms_per_step": r["device_ms_per_step"], "fine_buckets": st.fine_buckets,
         "max_over_mean_bucket": r["max_over_mean_bucket"], "extra_passes": st.extra_passes,
-        "spilled_keys": st.spilled_keys, "heavy_buckets": st.heavy_buckets, "phase_ms": r["per_phase"],
+        "spilled_keys": st.spilled_keys, "heavy_buckets": st.heavy_buckets, "rounds": st.rounds,
+        "phase_ms": r["per_phase"],
         "histogram_ms": r["histogram_ms"],
     }
 

@@ -102,6 +102,11 @@ typedef struct kc_config {
     uint32_t heavy_keys;      /* a bucket of more keys than this (0 = 1Mi) is counted in chunks by the
                                  whole GPU instead of one workgroup; its keys are written as root
                                  records with their whole count (no low.bin entry) */
+    uint32_t reserved;
+    uint64_t max_round_keys;  /* keys one pass over the device holds (0 = as many as the free device
+                                 memory allows).  An input with more positions is counted in
+                                 bucket-range rounds: contiguous root-bucket ranges, one pass each,
+                                 whose outputs are kept in host memory */
 } kc_config;
 
 typedef struct kc_stats {
@@ -139,6 +144,7 @@ typedef struct kc_stats {
     double prefix_sample_distinct_max_over_mean;
     uint64_t extra_passes;              /* counting passes beyond one per bucket, summed over buckets */
     uint64_t heavy_buckets;             /* buckets counted in chunks by the whole GPU */
+    uint64_t rounds;                    /* bucket-range rounds of the last run (1 = one pass) */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;

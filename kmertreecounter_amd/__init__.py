@@ -44,7 +44,7 @@ class Config(ctypes.Structure):
         ("expected_bytes", ctypes.c_uint64), ("sub_bits", ctypes.c_uint32),
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
-        ("heavy_keys", ctypes.c_uint32),
+        ("heavy_keys", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
     ]
 
 
@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
         ("learned_sample_max_over_mean", ctypes.c_double), ("prefix_sample_max_over_mean", ctypes.c_double),
         ("learned_sample_distinct_max_over_mean", ctypes.c_double),
         ("prefix_sample_distinct_max_over_mean", ctypes.c_double), ("extra_passes", ctypes.c_uint64),
-        ("heavy_buckets", ctypes.c_uint64),
+        ("heavy_buckets", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -224,12 +224,13 @@ class Engine:
 
     def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
-                 pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0):
+                 pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0,
+                 max_round_keys: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
-                     counter, table_slots, heavy_keys)
+                     counter, table_slots, heavy_keys, 0, max_round_keys)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

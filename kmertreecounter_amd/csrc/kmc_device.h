@@ -179,6 +179,8 @@ struct Route {
     // (kernels stage one coarse bucket's fine bounds in LDS padded to 255 with UINT64_MAX)
     // learned: 4096-entry coarse table, 256 x 4096 fine tables, 256 shifts
     const uint8_t* lut;
+    // bucket-range round: the scatter keeps the keys of coarse buckets [rlo, rhi)
+    uint32_t rlo, rhi;
 };
 
 // Number of entries <= x in ascending b[0..255): branchless, eight fixed

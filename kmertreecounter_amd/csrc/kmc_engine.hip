@@ -79,6 +79,15 @@ struct kc_engine {
     bool have_results = false;
     uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0, extra_passes = 0;
     uint64_t heavy_buckets = 0;
+    // bucket-range rounds: each round's outputs in host memory (empty when the
+    // results of a single round stay on the device)
+    struct RoundOut {
+        std::vector<uint64_t> firsts;
+        std::vector<uint8_t> recs;
+        std::vector<unsigned long long> root_off;  // 2 x 257, as root_off
+    };
+    std::vector<RoundOut> rounds;
+    uint32_t n_rounds = 0;
     unsigned long long* d_heavy = nullptr;  // heavy-bucket chunk list, heavy_cap x (lo, hi)
     uint64_t heavy_cap = 0;
     uint32_t cu_count = 0;
@@ -236,7 +245,7 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
         h->ovf_cap = cap;
     }
     // every listed chunk but the last of a bucket is full: at most K/chunk + nf chunks
-    const uint64_t want_heavy = padded_len / heavy_chunk_keys(h) + nf + 1;
+    const uint64_t want_heavy = h->key_cap / heavy_chunk_keys(h) + nf + 1;
     if (h->heavy_cap < want_heavy) {
         hipFree(h->d_heavy);
         h->d_heavy = nullptr;
@@ -244,7 +253,7 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
         HIPX(hipMalloc(&h->d_heavy, sizeof(unsigned long long) * 2 * want_heavy));
         h->heavy_cap = want_heavy;
     }
-    const uint64_t want_spill = std::min<uint64_t>(1ULL << 26, std::max<uint64_t>(1ULL << 16, pow2_at_least(padded_len / 512)));
+    const uint64_t want_spill = std::min<uint64_t>(1ULL << 26, std::max<uint64_t>(1ULL << 16, pow2_at_least(h->key_cap / 512)));
     if (h->spill_cap < want_spill) {
         hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
         if (hipMalloc(&h->d_spill_keys, want_spill * 8ULL * h->nw) != hipSuccess ||
@@ -266,6 +275,27 @@ uint32_t choose_sbits(const kc_engine* h, uint64_t padded_len)
     return sb;
 }
 
+// Keys one round can hold: every position when the three key buffers fit the
+// device (or max_round_keys), else what the free device memory allows.
+template <int NW>
+uint64_t round_capacity(const kc_engine* h, uint64_t padded)
+{
+    uint64_t cap = padded;
+    if (h->cfg.max_round_keys) cap = std::min<uint64_t>(cap, h->cfg.max_round_keys);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const uint64_t per_key = 3ULL * 8 * NW;              // A, B and R
+        const uint64_t held = h->key_cap * per_key;          // buffers this engine already holds
+        const uint64_t reserve = 8ULL << 30;                 // tables, spill, histogram, outputs in flight
+        const uint64_t avail = free_b + held > reserve ? free_b + held - reserve : 0;
+        cap = std::min<uint64_t>(cap, avail / per_key);
+    }
+    return std::max<uint64_t>(cap, 1024);
+}
+
+template <int NW>
+int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded);
+
 template <int NW>
 int run_pipeline_t(kc_engine* h)
 {
@@ -274,8 +304,11 @@ int run_pipeline_t(kc_engine* h)
     const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
     const uint32_t sbits = choose_sbits(h, padded);
     const uint32_t nf = 256u << sbits;
-    int rc = ensure_work(h, padded > 0 ? padded : tile, nf);
+    const uint64_t cap = round_capacity<NW>(h, padded > 0 ? padded : tile);
+    int rc = ensure_work(h, std::min<uint64_t>(cap, padded > 0 ? padded : tile), nf);
     if (rc) return rc;
+    h->rounds.clear();
+    h->n_rounds = 1;
 
     const bool learned = h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
     if (learned && h->bounds_sbits != sbits) {
@@ -295,7 +328,81 @@ int run_pipeline_t(kc_engine* h)
     rt.coarse_bounds = h->d_bounds;
     rt.fine_bounds = h->d_bounds ? h->d_bounds + 255 : nullptr;
     rt.lut = h->d_lut;
+    rt.rlo = 0;
+    rt.rhi = 256;
+    if (padded <= h->key_cap) return run_round_t<NW>(h, rt, sbits, nf, padded);
 
+    // Bucket-range rounds: more positions than one round holds.  The coarse
+    // histogram decides contiguous bucket ranges of at most key_cap keys; each
+    // round extracts the whole sequence again but keeps its own buckets, and
+    // its outputs move to host memory (key ranges of rounds are disjoint).
+    unsigned long long* counts = h->d_small + kSmCounts;
+    if (padded > h->seq_len) HIPX(hipMemsetAsync(h->d_seq + h->seq_len, '\n', padded - h->seq_len, st));
+    HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * kSmTotal, st));
+    HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, nullptr, nullptr, st));
+    std::vector<unsigned long long> hc((size_t)kXcds * 256);
+    HIPX(hipMemcpyAsync(hc.data(), counts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    std::vector<std::pair<uint32_t, uint32_t>> ranges;
+    uint64_t acc = 0;
+    uint32_t start = 0;
+    for (uint32_t b = 0; b < 256; ++b) {
+        uint64_t c = 0;
+        for (int x = 0; x < kXcds; ++x) c += hc[(size_t)x * 256 + b];
+        if (c > h->key_cap) return h->fail(KC_ERR_CAPACITY, "one root bucket holds more keys than a round");
+        if (acc + c > h->key_cap) {
+            ranges.emplace_back(start, b);
+            start = b;
+            acc = 0;
+        }
+        acc += c;
+    }
+    ranges.emplace_back(start, 256u);
+    uint64_t K = 0, nfirst = 0, nrec = 0, spilled = 0, maxb = 0, dist = 0, xp = 0, heavy = 0;
+    float phase[7] = {};
+    for (const auto& r : ranges) {
+        rt.rlo = r.first;
+        rt.rhi = r.second;
+        rc = run_round_t<NW>(h, rt, sbits, nf, padded);
+        if (rc) return rc;
+        kc_engine::RoundOut out;
+        out.firsts.resize(h->n_first * NW);
+        out.recs.resize(h->n_rec * rec_bytes(NW));
+        out.root_off = h->root_off;
+        if (h->n_first)
+            HIPX(hipMemcpy(out.firsts.data(), h->res_firsts, h->n_first * 8ULL * NW, hipMemcpyDeviceToHost));
+        if (h->n_rec) HIPX(hipMemcpy(out.recs.data(), h->res_recs, out.recs.size(), hipMemcpyDeviceToHost));
+        h->rounds.push_back(std::move(out));
+        K += h->K;
+        nfirst += h->n_first;
+        nrec += h->n_rec;
+        spilled += h->n_spilled;
+        maxb = std::max<uint64_t>(maxb, h->max_bucket);
+        dist += h->n_distinct_rep;
+        xp += h->extra_passes;
+        heavy += h->heavy_buckets;
+        for (int p = 0; p < 7; ++p) phase[p] += h->phase_ms[p];
+    }
+    h->K = K;
+    h->n_first = nfirst;
+    h->n_rec = nrec;
+    h->n_spilled = spilled;
+    h->max_bucket = maxb;
+    h->n_distinct_rep = dist;
+    h->extra_passes = xp;
+    h->heavy_buckets = heavy;
+    for (int p = 0; p < 7; ++p) h->phase_ms[p] = phase[p];
+    h->res_firsts = nullptr;
+    h->res_recs = nullptr;
+    h->n_rounds = (uint32_t)ranges.size();
+    h->have_results = true;
+    return KC_OK;
+}
+
+template <int NW>
+int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded)
+{
+    hipStream_t st = h->st;
     unsigned long long* counts = h->d_small + kSmCounts;
     unsigned long long* cursor = h->d_small + kSmCursor;
     unsigned long long* coarse_off = h->d_small + kSmCoarseOff;
@@ -319,7 +426,7 @@ int run_pipeline_t(kc_engine* h)
         HIPX(hipEventRecord(h->ev[0], st));
         HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
         HIPX(hipEventRecord(h->ev[1], st));
-        HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st));
+        HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st, rt.rlo, rt.rhi));
         HIPX(hipEventRecord(h->ev[2], st));
         HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
         HIPX(hipEventRecord(h->ev[3], st));
@@ -628,6 +735,16 @@ int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t kb = 8ULL * h->nw;
     const uint64_t per = kEmitChunk / kb;
+    if (!h->rounds.empty()) {  // bucket-range rounds: outputs in host memory
+        for (const auto& r : h->rounds) {
+            const uint64_t n = r.firsts.size() / h->nw;
+            for (uint64_t i = 0; i < n; i += per) {
+                int rc = cb(user, r.firsts.data() + i * h->nw, std::min(per, n - i));
+                if (rc) return h->fail(rc, "keys callback returned an error");
+            }
+        }
+        return KC_OK;
+    }
     for (uint64_t i = 0; i < h->n_first; i += per) {
         const uint64_t n = std::min(per, h->n_first - i);
         HIPX(hipMemcpy(h->h_bounce, h->res_firsts + i * h->nw, n * kb, hipMemcpyDeviceToHost));
@@ -644,6 +761,18 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t rb = rec_bytes(h->nw);
     const uint64_t per = kEmitChunk / rb;
+    if (!h->rounds.empty()) {  // bucket-range rounds: outputs in host memory
+        for (const auto& r : h->rounds) {
+            for (int part = 0; part < 2; ++part) {
+                const uint64_t b = r.root_off[part * 257 + root_id], e = r.root_off[part * 257 + root_id + 1];
+                for (uint64_t i = b; i < e; i += per) {
+                    int rc = cb(user, r.recs.data() + i * rb, std::min(per, e - i));
+                    if (rc) return h->fail(rc, "records callback returned an error");
+                }
+            }
+        }
+        return KC_OK;
+    }
     // in-bucket records of the root, then its overflow records
     for (int part = 0; part < 2; ++part) {
         const uint64_t b = h->root_off[part * 257 + root_id], e = h->root_off[part * 257 + root_id + 1];
@@ -657,24 +786,29 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     return KC_OK;
 }
 
-int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_freq, int64_t* hist)
+namespace {
+
+// Histogram of one set of device-resident results into hist (added, not
+// stored): the record table doubles until no probe runs out.
+int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_freq, const uint8_t* recs, uint64_t nrec,
+                   const uint64_t* firsts, uint64_t nfirst, int64_t* hist)
 {
-    if (!h || !hist || min_freq == 0 || max_freq < min_freq || mode > KC_HISTOGRAM_APPROXIMATE) return KC_ERR_ARG;
-    if (!h->have_results) return h->fail(KC_ERR_STATE, "no results: call kc_finish first");
-    HIPX(hipSetDevice(h->device));
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
-    const bool precise = mode == KC_HISTOGRAM_PRECISE;
     long long* d_hist = nullptr;
     unsigned long long* d_err = nullptr;
     unsigned long long* d_keys = nullptr;
     uint32_t* d_vals = nullptr;
-    auto release = [&]() { hipFree(d_hist); hipFree(d_err); hipFree(d_keys); hipFree(d_vals); d_keys = nullptr; d_vals = nullptr; };
+    auto release = [&]() {
+        hipFree(d_hist); hipFree(d_err); hipFree(d_keys); hipFree(d_vals);
+        d_hist = nullptr; d_err = nullptr; d_keys = nullptr; d_vals = nullptr;
+    };
     if (hipMalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || hipMalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) {
         release();
         return h->fail(KC_ERR_NOMEM, "histogram allocation failed");
     }
     // the record table is at most half full; a probe that runs out doubles it
-    uint64_t cap = precise ? pow2_at_least(std::max<uint64_t>(1024, 2 * h->n_rec)) : 0;
+    uint64_t cap = precise ? pow2_at_least(std::max<uint64_t>(1024, 2 * nrec)) : 0;
+    std::vector<long long> part(nb);
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (precise) {
             hipFree(d_keys); hipFree(d_vals);
@@ -697,8 +831,8 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
         a.cap = cap;
         a.error_flags = d_err;
         if (e == hipSuccess)
-            e = h->nw == 1 ? launch_histogram<1>(h->res_recs, h->n_rec, h->res_firsts, h->n_first, a, h->st)
-                           : launch_histogram<2>(h->res_recs, h->n_rec, h->res_firsts, h->n_first, a, h->st);
+            e = h->nw == 1 ? launch_histogram<1>(recs, nrec, firsts, nfirst, a, h->st)
+                           : launch_histogram<2>(recs, nrec, firsts, nfirst, a, h->st);
         unsigned long long err = 0;
         if (e == hipSuccess) e = hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, h->st);
         if (e == hipSuccess) e = hipStreamSynchronize(h->st);
@@ -710,13 +844,38 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
             cap *= 2;
             continue;
         }
-        e = hipMemcpy(hist, d_hist, sizeof(long long) * nb, hipMemcpyDeviceToHost);
+        e = hipMemcpy(part.data(), d_hist, sizeof(long long) * nb, hipMemcpyDeviceToHost);
         release();
         if (e != hipSuccess) return h->fail(KC_ERR_HIP, std::string("histogram: ") + hipGetErrorString(e));
+        for (uint64_t b = 0; b < nb; ++b) hist[b] += part[b];
         return KC_OK;
     }
     release();
     return h->fail(KC_ERR_CAPACITY, "histogram table did not converge");
+}
+
+}  // namespace
+
+int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_freq, int64_t* hist)
+{
+    if (!h || !hist || min_freq == 0 || max_freq < min_freq || mode > KC_HISTOGRAM_APPROXIMATE) return KC_ERR_ARG;
+    if (!h->have_results) return h->fail(KC_ERR_STATE, "no results: call kc_finish first");
+    HIPX(hipSetDevice(h->device));
+    const bool precise = mode == KC_HISTOGRAM_PRECISE;
+    std::fill(hist, hist + ((uint64_t)max_freq - min_freq + 1), (int64_t)0);
+    if (h->rounds.empty())
+        return histogram_pass(h, precise, min_freq, max_freq, h->res_recs, h->n_rec, h->res_firsts, h->n_first, hist);
+    // bucket-range rounds: the key ranges are disjoint, so the rounds' histograms
+    // add up; each round's outputs go back into the (large enough) key buffers
+    for (const auto& r : h->rounds) {
+        const uint64_t nrec = r.recs.size() / rec_bytes(h->nw), nfirst = r.firsts.size() / h->nw;
+        if (nrec) HIPX(hipMemcpy(h->d_R, r.recs.data(), r.recs.size(), hipMemcpyHostToDevice));
+        if (nfirst) HIPX(hipMemcpy(h->d_A, r.firsts.data(), r.firsts.size() * 8ULL, hipMemcpyHostToDevice));
+        const int rc = histogram_pass(h, precise, min_freq, max_freq, reinterpret_cast<const uint8_t*>(h->d_R), nrec,
+                                      h->d_A, nfirst, hist);
+        if (rc) return rc;
+    }
+    return KC_OK;
 }
 
 int kc_stats_get(kc_engine* h, kc_stats* s)
@@ -735,6 +894,7 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->max_bucket_keys = h->max_bucket;
     s->extra_passes = h->extra_passes;
     s->heavy_buckets = h->heavy_buckets;
+    s->rounds = h->n_rounds;
     s->mean_bucket_keys = h->nf_last ? (double)h->K / h->nf_last : 0.0;
     s->ms_extract_count = h->phase_ms[0];
     s->ms_extract_scatter = h->phase_ms[1];

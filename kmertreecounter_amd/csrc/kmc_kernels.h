@@ -116,7 +116,8 @@ template <int NW>
 hipError_t launch_extract(bool scatter, const uint8_t* seq, uint64_t padded_len, uint32_t k, const Route& rt,
                           unsigned long long* counts, unsigned long long* cursor, uint64_t* out, hipStream_t st);
 hipError_t launch_scan_coarse(const unsigned long long* counts, unsigned long long* coarse_off, unsigned long long* cursor,
-                              unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st);
+                              unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st,
+                              uint32_t rlo = 0, uint32_t rhi = 256);
 hipError_t launch_scan_fine(const unsigned long long* counts, uint32_t nf, unsigned long long* fine_off,
                             unsigned long long* cursor, hipStream_t st);
 template <int NW>

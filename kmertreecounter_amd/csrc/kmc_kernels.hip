@@ -234,10 +234,13 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         const bool ok = window_key<NW>(c, v, i, k, key);
         keys[i] = key;
         if (ok) {
-            okm |= 1u << i;
             const uint32_t b = coarse_of(rt, s_cb, s_lut, key.w[0]);
-            bk[i] = b;
-            rank[i] = atomicAdd(&s_hist[b], 1u);
+            // the scatter of a bucket-range round keeps its own buckets only
+            if (!SCATTER || (b >= rt.rlo && b < rt.rhi)) {
+                okm |= 1u << i;
+                bk[i] = b;
+                rank[i] = atomicAdd(&s_hist[b], 1u);
+            }
         }
     }
     __syncthreads();
@@ -280,15 +283,17 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
 __global__ void __launch_bounds__(256)
 k_scan_coarse(const unsigned long long* __restrict__ counts, unsigned long long* __restrict__ coarse_off,
               unsigned long long* __restrict__ cursor, unsigned long long* __restrict__ tile_start,
-              uint32_t sub_tile)
+              uint32_t sub_tile, uint32_t rlo, uint32_t rhi)
 {
+    // buckets outside the round [rlo, rhi) get no keys
     __shared__ unsigned long long s_scan[8];
     const int b = threadIdx.x;
+    const bool in = (uint32_t)b >= rlo && (uint32_t)b < rhi;
     unsigned long long tot = 0;
     unsigned long long per[kXcds];
 #pragma unroll
     for (int x = 0; x < kXcds; ++x) {
-        per[x] = counts[(size_t)x * 256 + b];
+        per[x] = in ? counts[(size_t)x * 256 + b] : 0ULL;
         tot += per[x];
     }
     unsigned long long all;
@@ -1305,9 +1310,9 @@ hipError_t launch_extract(bool scatter, const uint8_t* seq, uint64_t padded_len,
 }
 
 hipError_t launch_scan_coarse(const unsigned long long* counts, unsigned long long* coarse_off, unsigned long long* cursor,
-                              unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st)
+                              unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st, uint32_t rlo, uint32_t rhi)
 {
-    hipLaunchKernelGGL(k_scan_coarse, dim3(1), dim3(256), 0, st, counts, coarse_off, cursor, tile_start, sub_tile);
+    hipLaunchKernelGGL(k_scan_coarse, dim3(1), dim3(256), 0, st, counts, coarse_off, cursor, tile_start, sub_tile, rlo, rhi);
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }

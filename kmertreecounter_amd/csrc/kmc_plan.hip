@@ -72,6 +72,7 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
         if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
         if (e == hipSuccess) e = hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * kXcds * 256, st);
         Route rt{};
+    rt.rhi = 256;  // every coarse bucket
         if (e == hipSuccess)
             e = nw == 1 ? launch_extract<1>(false, d_seq, padded, k, rt, d_counts, nullptr, nullptr, st)
                         : launch_extract<2>(false, d_seq, padded, k, rt, d_counts, nullptr, nullptr, st);
@@ -137,6 +138,7 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     unsigned long long* coff = d_small + 16 * 256;
     unsigned long long* tstart = coff + 257;
     Route rt{};
+    rt.rhi = 256;  // every coarse bucket
     hipError_t e = hipMemcpyAsync(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
     if (e == hipSuccess) e = hipMemsetAsync(d_small, 0, sizeof(unsigned long long) * (16 * 256 + 600), st);

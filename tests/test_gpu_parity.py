@@ -249,3 +249,43 @@ def test_root_files_hold_their_prefix():
             recs = e.root(rid)
             if len(recs):
                 assert (recs["key"][:, 0] >> np.uint64(56) == rid).all()
+
+
+@pytest.mark.parametrize("k", [31, 63])
+@pytest.mark.parametrize("partition", [K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED], ids=["prefix", "learned"])
+def test_bucket_range_rounds(k, partition):
+    # a key capacity below the input's positions: the engine counts contiguous
+    # root-bucket ranges in separate rounds and keeps each round's outputs in
+    # host memory; tables, root files and histograms must not change
+    seq = K.generate(1, scale=10, seed=14, fmt="seq")
+    nw = C.nwords(k)
+    keys, counts = C.count_table(C.extract(seq, k))
+    with K.Engine(k, partition=partition, max_round_keys=1_500_000) as e:
+        if partition == K.KC_PARTITION_LEARNED:
+            e.sample_block(seq[: len(seq) // 20])
+            e.plan()
+        e.push_block(seq)
+        e.finish()
+        f, r, s = e.firsts(), e.records(), e.stats()
+        assert s.rounds >= max(2, -(-s.total_kmers // 1_500_000))
+        check_outputs(f, r, s, k, keys, counts)
+        for rid in (0, 17, 128, 255):
+            rr = e.root(rid)
+            if len(rr):
+                assert (rr["key"][:, 0] >> np.uint64(56) == rid).all()
+        np.testing.assert_array_equal(e.histogram(1, 1000), C.histogram(counts, 1, 1000))
+        # a second pass over the resident input gives the same tables
+        e.count_resident()
+        f2, r2 = e.firsts(), e.records()
+        k2, c2 = C.merge_outputs(f2, r2, nw)
+        np.testing.assert_array_equal(k2, keys)
+        np.testing.assert_array_equal(c2, counts)
+
+
+def test_rounds_reject_a_root_bucket_larger_than_a_round():
+    seq = K.generate(1, scale=10, seed=14, fmt="seq")
+    # ~19k keys per root bucket on average, far above 4096
+    with K.Engine(31, max_round_keys=4096) as e:
+        e.push_block(seq)
+        with pytest.raises(K.KmcError):
+            e.finish()
