@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/kmc.h"
@@ -82,8 +83,9 @@ struct kc_engine {
     // bucket-range rounds: each round's outputs in host memory (empty when the
     // results of a single round stay on the device)
     struct RoundOut {
-        std::vector<uint64_t> firsts;
-        std::vector<uint8_t> recs;
+        std::unique_ptr<uint64_t[]> firsts;  // n_first x NW words (not zero-filled before the copy)
+        std::unique_ptr<uint8_t[]> recs;     // n_rec records
+        uint64_t n_first = 0, n_rec = 0;
         std::vector<unsigned long long> root_off;  // 2 x 257, as root_off
     };
     std::vector<RoundOut> rounds;
@@ -366,12 +368,15 @@ int run_pipeline_t(kc_engine* h)
         rc = run_round_t<NW>(h, rt, sbits, nf, padded);
         if (rc) return rc;
         kc_engine::RoundOut out;
-        out.firsts.resize(h->n_first * NW);
-        out.recs.resize(h->n_rec * rec_bytes(NW));
+        out.n_first = h->n_first;
+        out.n_rec = h->n_rec;
+        out.firsts.reset(new (std::nothrow) uint64_t[std::max<uint64_t>(1, h->n_first * NW)]);
+        out.recs.reset(new (std::nothrow) uint8_t[std::max<uint64_t>(1, h->n_rec * rec_bytes(NW))]);
+        if (!out.firsts || !out.recs) return h->fail(KC_ERR_NOMEM, "host memory for a round's outputs");
         out.root_off = h->root_off;
         if (h->n_first)
-            HIPX(hipMemcpy(out.firsts.data(), h->res_firsts, h->n_first * 8ULL * NW, hipMemcpyDeviceToHost));
-        if (h->n_rec) HIPX(hipMemcpy(out.recs.data(), h->res_recs, out.recs.size(), hipMemcpyDeviceToHost));
+            HIPX(hipMemcpy(out.firsts.get(), h->res_firsts, h->n_first * 8ULL * NW, hipMemcpyDeviceToHost));
+        if (h->n_rec) HIPX(hipMemcpy(out.recs.get(), h->res_recs, h->n_rec * rec_bytes(NW), hipMemcpyDeviceToHost));
         h->rounds.push_back(std::move(out));
         K += h->K;
         nfirst += h->n_first;
@@ -737,9 +742,9 @@ int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
     const uint64_t per = kEmitChunk / kb;
     if (!h->rounds.empty()) {  // bucket-range rounds: outputs in host memory
         for (const auto& r : h->rounds) {
-            const uint64_t n = r.firsts.size() / h->nw;
+            const uint64_t n = r.n_first;
             for (uint64_t i = 0; i < n; i += per) {
-                int rc = cb(user, r.firsts.data() + i * h->nw, std::min(per, n - i));
+                int rc = cb(user, r.firsts.get() + i * h->nw, std::min(per, n - i));
                 if (rc) return h->fail(rc, "keys callback returned an error");
             }
         }
@@ -766,7 +771,7 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
             for (int part = 0; part < 2; ++part) {
                 const uint64_t b = r.root_off[part * 257 + root_id], e = r.root_off[part * 257 + root_id + 1];
                 for (uint64_t i = b; i < e; i += per) {
-                    int rc = cb(user, r.recs.data() + i * rb, std::min(per, e - i));
+                    int rc = cb(user, r.recs.get() + i * rb, std::min(per, e - i));
                     if (rc) return h->fail(rc, "records callback returned an error");
                 }
             }
@@ -791,16 +796,21 @@ namespace {
 // Histogram of one set of device-resident results into hist (added, not
 // stored): the record table doubles until no probe runs out.
 int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_freq, const uint8_t* recs, uint64_t nrec,
-                   const uint64_t* firsts, uint64_t nfirst, int64_t* hist)
+                   const uint64_t* firsts, uint64_t nfirst, void* scratch, uint64_t scratch_bytes, int64_t* hist)
 {
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
     long long* d_hist = nullptr;
     unsigned long long* d_err = nullptr;
     unsigned long long* d_keys = nullptr;
     uint32_t* d_vals = nullptr;
+    bool own_table = false;  // the table lives in the free key buffer when it fits there
+    auto free_table = [&]() {
+        if (own_table) { hipFree(d_keys); hipFree(d_vals); }
+        d_keys = nullptr; d_vals = nullptr; own_table = false;
+    };
     auto release = [&]() {
-        hipFree(d_hist); hipFree(d_err); hipFree(d_keys); hipFree(d_vals);
-        d_hist = nullptr; d_err = nullptr; d_keys = nullptr; d_vals = nullptr;
+        hipFree(d_hist); hipFree(d_err); free_table();
+        d_hist = nullptr; d_err = nullptr;
     };
     if (hipMalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || hipMalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) {
         release();
@@ -811,11 +821,17 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
     std::vector<long long> part(nb);
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (precise) {
-            hipFree(d_keys); hipFree(d_vals);
-            d_keys = nullptr; d_vals = nullptr;
-            if (hipMalloc(&d_keys, cap * 8ULL * h->nw) != hipSuccess || hipMalloc(&d_vals, cap * 4ULL) != hipSuccess) {
-                release();
-                return h->fail(KC_ERR_NOMEM, "histogram table allocation failed");
+            free_table();
+            const uint64_t kbytes = cap * 8ULL * h->nw;
+            if (scratch && kbytes + cap * 4ULL <= scratch_bytes) {
+                d_keys = static_cast<unsigned long long*>(scratch);
+                d_vals = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(scratch) + kbytes);
+            } else {
+                own_table = true;
+                if (hipMalloc(&d_keys, kbytes) != hipSuccess || hipMalloc(&d_vals, cap * 4ULL) != hipSuccess) {
+                    release();
+                    return h->fail(KC_ERR_NOMEM, "histogram table allocation failed");
+                }
             }
         }
         hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(long long) * nb, h->st);
@@ -863,16 +879,17 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
     HIPX(hipSetDevice(h->device));
     const bool precise = mode == KC_HISTOGRAM_PRECISE;
     std::fill(hist, hist + ((uint64_t)max_freq - min_freq + 1), (int64_t)0);
-    if (h->rounds.empty())
-        return histogram_pass(h, precise, min_freq, max_freq, h->res_recs, h->n_rec, h->res_firsts, h->n_first, hist);
+    const uint64_t buf_bytes = h->key_cap * 8ULL * h->nw;
+    if (h->rounds.empty())  // results in A and B; the record slices (R) are free
+        return histogram_pass(h, precise, min_freq, max_freq, h->res_recs, h->n_rec, h->res_firsts, h->n_first, h->d_R,
+                              buf_bytes, hist);
     // bucket-range rounds: the key ranges are disjoint, so the rounds' histograms
-    // add up; each round's outputs go back into the (large enough) key buffers
+    // add up; each round's outputs go back into R and A, and B holds the table
     for (const auto& r : h->rounds) {
-        const uint64_t nrec = r.recs.size() / rec_bytes(h->nw), nfirst = r.firsts.size() / h->nw;
-        if (nrec) HIPX(hipMemcpy(h->d_R, r.recs.data(), r.recs.size(), hipMemcpyHostToDevice));
-        if (nfirst) HIPX(hipMemcpy(h->d_A, r.firsts.data(), r.firsts.size() * 8ULL, hipMemcpyHostToDevice));
-        const int rc = histogram_pass(h, precise, min_freq, max_freq, reinterpret_cast<const uint8_t*>(h->d_R), nrec,
-                                      h->d_A, nfirst, hist);
+        if (r.n_rec) HIPX(hipMemcpy(h->d_R, r.recs.get(), r.n_rec * rec_bytes(h->nw), hipMemcpyHostToDevice));
+        if (r.n_first) HIPX(hipMemcpy(h->d_A, r.firsts.get(), r.n_first * 8ULL * h->nw, hipMemcpyHostToDevice));
+        const int rc = histogram_pass(h, precise, min_freq, max_freq, reinterpret_cast<const uint8_t*>(h->d_R), r.n_rec,
+                                      h->d_A, r.n_first, h->d_B, buf_bytes, hist);
         if (rc) return rc;
     }
     return KC_OK;
