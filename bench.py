@@ -174,11 +174,11 @@ def cpu_baseline_port(workload: int, k: int) -> dict:
             "sample": f"workload {workload} at 1/20 scale ({n} k-mers), oracle extract + sort + run-length, {dt:.2f} s"}
 
 
-def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, counter: str) -> dict:
+def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, counter: str, graph: bool) -> dict:
     """Resident run of one partition method: plan (learned only), copy, warm up, time `steps` passes."""
     part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
     cnt = K.KC_COUNTER_BLOOM if counter == "bloom" else K.KC_COUNTER_EXACT
-    eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt)
+    eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt, graph=graph)
     plan_ms = 0.0
     if part == K.KC_PARTITION_LEARNED:
         # Stage 0 sample: the first 1% of the reads; the network trains on the GPU
@@ -259,6 +259,8 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--partition", choices=["prefix", "learned", "both"], default="both")
     ap.add_argument("--counter", choices=["bloom", "exact"], default="bloom")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the pipeline as a captured HIP graph (per-kernel times are then not recorded)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
@@ -278,7 +280,7 @@ def main() -> int:
     seq_bytes = len(seq)
     log(f"generated {seq_bytes / 1e9:.2f} GB of sequence in {time.time() - t0:.1f} s")
     parts = ["prefix", "learned"] if args.partition == "both" else [args.partition]
-    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter) for p in parts}
+    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter, args.graph) for p in parts}
     del seq
     main_run = runs["learned"] if "learned" in runs else runs["prefix"]
     st = main_run["stats"]
@@ -287,7 +289,8 @@ def main() -> int:
 
     alg = algorithmic_bytes(st, seq_bytes + (-(seq_bytes) % (512 * 16)))
     dominant = max(per_phase, key=per_phase.get)
-    achieved = alg[dominant] / (per_phase[dominant] * 1e-3) / 1e9
+    # a graph replay has no per-kernel times: no kernel roofline then
+    achieved = alg[dominant] / (per_phase[dominant] * 1e-3) / 1e9 if per_phase[dominant] > 0 else None
     total_alg = sum(alg.values())
     traffic = traffic_from_profiles(dominant, args.workload, main_run["partition"])
 
@@ -317,6 +320,7 @@ def main() -> int:
             "workload": desc + (f" at 1/{args.scale:g} scale" if args.scale != 1 else ""),
             "partition": main_run["partition"],
             "counter": args.counter,
+            "graph": args.graph,
             "k": k,
             "kmers": st.total_kmers,
             "distinct_first": st.firsts,
@@ -332,7 +336,7 @@ def main() -> int:
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": achieved / HBM_PEAK_GBS if achieved is not None else None,
             "traffic": traffic,
         },
         "pipeline": {

@@ -107,6 +107,10 @@ typedef struct kc_config {
                                  memory allows).  An input with more positions is counted in
                                  bucket-range rounds: contiguous root-bucket ranges, one pass each,
                                  whose outputs are kept in host memory */
+    uint32_t graph;           /* 1: capture one pass of the pipeline in a HIP graph and replay it
+                                 (one-round runs; the graph is captured again when buffers or
+                                 options change); the stats then carry the total time only */
+    uint32_t reserved2;
 } kc_config;
 
 typedef struct kc_stats {

@@ -26,6 +26,13 @@ constexpr uint64_t kFineTarget = 32768;         // keys per fine bucket the fan-
 
 }  // namespace
 
+// What a captured pipeline graph depends on: buffers, sizes and options.  A
+// run whose key differs captures the graph again.
+struct GraphKey {
+    const void* ptrs[18];
+    uint64_t vals[18];
+};
+
 struct kc_engine {
     kc_config cfg{};
     uint32_t k = 0, nw = 0;
@@ -90,6 +97,8 @@ struct kc_engine {
     };
     std::vector<RoundOut> rounds;
     uint32_t n_rounds = 0;
+    hipGraphExec_t graph_exec = nullptr;  // captured single-round pipeline (kc_config.graph)
+    GraphKey graph_key{};
     unsigned long long* d_heavy = nullptr;  // heavy-bucket chunk list, heavy_cap x (lo, hi)
     uint64_t heavy_cap = 0;
     uint32_t cu_count = 0;
@@ -296,6 +305,23 @@ uint64_t round_capacity(const kc_engine* h, uint64_t padded)
 }
 
 template <int NW>
+GraphKey graph_key(const kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded)
+{
+    GraphKey g{};
+    const void* p[18] = {h->d_seq, h->d_A, h->d_B, h->d_R, h->d_small, h->d_ctr, h->d_spill_keys, h->d_spill_cnt,
+                         h->d_ovf, h->d_heavy, h->d_fcounts, h->d_fcursor, h->d_fine_off, h->d_foff, h->d_roff,
+                         h->d_bucket_u32, h->d_bounds, h->d_lut};
+    const uint64_t v[18] = {padded, h->seq_len, sbits, nf, h->key_cap, h->spill_cap, h->ovf_cap, h->heavy_cap,
+                            h->cu_count, rt.mode, h->k, h->cfg.min_count, h->cfg.max_count, h->cfg.pass_keys,
+                            h->cfg.counter, h->cfg.table_slots, h->cfg.heavy_keys, h->bounds_sbits};
+    for (int i = 0; i < 18; ++i) {
+        g.ptrs[i] = p[i];
+        g.vals[i] = v[i];
+    }
+    return g;
+}
+
+template <int NW>
 int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded);
 
 template <int NW>
@@ -416,96 +442,133 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
     uint32_t* d_nrec = d_nfirst + nf;
     uint32_t* d_ndist = d_nrec + nf;
 
+    // keys grouped by fine bucket (P3 input) and the per-bucket first-occurrence slices
+    uint64_t* src = sbits ? h->d_B : h->d_A;
+    uint64_t* slices = sbits ? h->d_A : h->d_B;
     for (int attempt = 0; attempt < 6; ++attempt) {
-        if (padded > h->seq_len) HIPX(hipMemsetAsync(h->d_seq + h->seq_len, '\n', padded - h->seq_len, st));
-        HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * kSmTotal, st));
-        HIPX(hipMemsetAsync(h->d_ctr, 0, sizeof(unsigned long long) * kCtrCount, st));
-        if (h->spill_dirty) {
-            HIPX(hipMemsetAsync(h->d_spill_keys, 0xFF, h->spill_cap * 8ULL * NW, st));
-            HIPX(hipMemsetAsync(h->d_spill_cnt, 0, h->spill_cap * 4ULL, st));
+        // One pass of the pipeline on the stream.  With `graph` it is being
+        // captured: clears are fill kernels (no memset nodes) and no events.
+        auto enqueue = [&](bool graph) -> int {
+            auto fill = [&](void* p, uint64_t bytes, uint8_t v) -> hipError_t {
+                return graph ? launch_fill(p, bytes, v, st) : hipMemsetAsync(p, v, bytes, st);
+            };
+            auto mark = [&](int e) -> hipError_t { return graph ? hipSuccess : hipEventRecord(h->ev[e], st); };
+            if (padded > h->seq_len) HIPX(fill(h->d_seq + h->seq_len, padded - h->seq_len, '\n'));
+            HIPX(fill(h->d_small, sizeof(unsigned long long) * kSmTotal, 0));
+            HIPX(fill(h->d_ctr, sizeof(unsigned long long) * kCtrCount, 0));
+            // a graph cannot skip the clear, so a captured run always clears the spill table
+            if (h->spill_dirty || graph) {
+                HIPX(fill(h->d_spill_keys, h->spill_cap * 8ULL * NW, 0xFF));
+                HIPX(fill(h->d_spill_cnt, h->spill_cap * 4ULL, 0));
+            }
+            if (sbits) HIPX(fill(h->d_fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
+
+            // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]
+            HIPX(mark(0));
+            HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
+            HIPX(mark(1));
+            HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st, rt.rlo, rt.rhi));
+            HIPX(mark(2));
+            HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
+            HIPX(mark(3));
+            const unsigned long long* fine_off = coarse_off;
+            if (sbits) {
+                const uint64_t max_tiles = padded / sub_tile(NW) + 256;
+                HIPX(mark(4));
+                HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
+                HIPX(mark(5));
+                HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, st));
+                HIPX(mark(6));
+                HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
+                HIPX(mark(7));
+                fine_off = h->d_fine_off;
+            } else {
+                for (int e = 4; e < 8; ++e) HIPX(mark(e));
+            }
+            CountArgs a{};
+            a.firsts = slices;
+            a.rec_tmp = h->d_R;
+            a.nfirst = d_nfirst;
+            a.nrec = d_nrec;
+            a.ndist = d_ndist;
+            a.ovf = h->d_ovf;
+            a.ovf_count = h->d_ctr + kCtrOvf;
+            a.ovf_cap = h->ovf_cap;
+            a.spill_keys = h->d_spill_keys;
+            a.spill_cnt = h->d_spill_cnt;
+            a.spill_cap = h->spill_cap;
+            a.spilled = h->d_ctr + kCtrSpilled;
+            a.spill_distinct = h->d_ctr + kCtrSpillDistinct;
+            a.error_flags = h->d_ctr + kCtrErr;
+            a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
+            a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
+            // the exact counter reads 0 as its own default (its table size)
+            a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys
+                                           : (h->cfg.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
+            a.extra_passes = h->d_ctr + kCtrExtraPasses;
+            a.table_slots = h->cfg.table_slots;
+            a.heavy_keys = heavy_keys(h);
+            a.heavy_chunk = heavy_chunk_keys(h);
+            a.heavy_list = h->d_heavy;
+            a.heavy_cap = h->heavy_cap;
+            a.heavy_count = h->d_ctr + kCtrHeavyChunks;
+            a.heavy_next = h->d_ctr + kCtrHeavyNext;
+            a.heavy_buckets = h->d_ctr + kCtrHeavyBuckets;
+            HIPX(mark(8));
+            HIPX(launch_count<NW>(nf, h->cfg.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
+            HIPX(launch_heavy<NW>(h->cu_count, src, a, st));
+            HIPX(mark(9));
+            HIPX(launch_spill_emit<NW>(a, st));
+            SegArgs sa{};
+            sa.nf = nf;
+            sa.sbits = sbits;
+            sa.fine_off = fine_off;
+            sa.nfirst = d_nfirst;
+            sa.nrec = d_nrec;
+            sa.ndist = d_ndist;
+            sa.foff = h->d_foff;
+            sa.roff = h->d_roff;
+            sa.root_ovf = h->d_small + kSmRootOvf;
+            sa.root_main = h->d_small + kSmRootMain;
+            sa.root_off = h->d_small + kSmRootOff;
+            sa.ovf_off = h->d_small + kSmOvfOff;
+            sa.ovf_cursor = h->d_small + kSmOvfCursor;
+            sa.ctr = h->d_ctr;
+            sa.rec_out_cap = (h->key_cap * 8ULL * NW) / rec_bytes(NW);
+            // final firsts into the P3 key input (free now), final records into the slices buffer
+            HIPX(launch_finalize<NW>(sa, a, slices, src, h->d_R, reinterpret_cast<uint8_t*>(slices), st));
+            return KC_OK;
+        };
+        const bool use_graph = h->cfg.graph && rt.rlo == 0 && rt.rhi == 256;
+        if (use_graph) {
+            const GraphKey key = graph_key<NW>(h, rt, sbits, nf, padded);
+            if (!h->graph_exec || std::memcmp(&key, &h->graph_key, sizeof(key)) != 0) {
+                if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+                h->graph_exec = nullptr;
+                HIPX(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                const int erc = enqueue(true);
+                hipGraph_t g = nullptr;
+                const hipError_t ee = hipStreamEndCapture(st, &g);
+                if (erc) {
+                    if (g) hipGraphDestroy(g);
+                    return erc;
+                }
+                HIPX(ee);
+                const hipError_t ie = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                HIPX(ie);
+                h->graph_key = key;
+            }
+            HIPX(hipEventRecord(h->ev[0], st));
+            HIPX(hipGraphLaunch(h->graph_exec, st));
+            HIPX(hipEventRecord(h->ev[10], st));
+            h->spill_dirty = false;
+        } else {
+            const int erc = enqueue(false);
+            if (erc) return erc;
+            HIPX(hipEventRecord(h->ev[10], st));
             h->spill_dirty = false;
         }
-        if (sbits) HIPX(hipMemsetAsync(h->d_fcounts, 0, sizeof(unsigned long long) * kXcds * nf, st));
-
-        // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]
-        HIPX(hipEventRecord(h->ev[0], st));
-        HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
-        HIPX(hipEventRecord(h->ev[1], st));
-        HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st, rt.rlo, rt.rhi));
-        HIPX(hipEventRecord(h->ev[2], st));
-        HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
-        HIPX(hipEventRecord(h->ev[3], st));
-        uint64_t* src = h->d_A;        // keys grouped by fine bucket (P3 input)
-        uint64_t* slices = h->d_B;     // per-bucket first-occurrence slices
-        const unsigned long long* fine_off = coarse_off;
-        if (sbits) {
-            const uint64_t max_tiles = padded / sub_tile(NW) + 256;
-            HIPX(hipEventRecord(h->ev[4], st));
-            HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
-            HIPX(hipEventRecord(h->ev[5], st));
-            HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, st));
-            HIPX(hipEventRecord(h->ev[6], st));
-            HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
-            HIPX(hipEventRecord(h->ev[7], st));
-            src = h->d_B;
-            slices = h->d_A;
-            fine_off = h->d_fine_off;
-        } else {
-            for (int i = 4; i < 8; ++i) HIPX(hipEventRecord(h->ev[i], st));
-        }
-        CountArgs a{};
-        a.firsts = slices;
-        a.rec_tmp = h->d_R;
-        a.nfirst = d_nfirst;
-        a.nrec = d_nrec;
-        a.ndist = d_ndist;
-        a.ovf = h->d_ovf;
-        a.ovf_count = h->d_ctr + kCtrOvf;
-        a.ovf_cap = h->ovf_cap;
-        a.spill_keys = h->d_spill_keys;
-        a.spill_cnt = h->d_spill_cnt;
-        a.spill_cap = h->spill_cap;
-        a.spilled = h->d_ctr + kCtrSpilled;
-        a.spill_distinct = h->d_ctr + kCtrSpillDistinct;
-        a.error_flags = h->d_ctr + kCtrErr;
-        a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
-        a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
-        // the exact counter reads 0 as its own default (its table size)
-        a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys
-                                       : (h->cfg.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
-        a.extra_passes = h->d_ctr + kCtrExtraPasses;
-        a.table_slots = h->cfg.table_slots;
-        a.heavy_keys = heavy_keys(h);
-        a.heavy_chunk = heavy_chunk_keys(h);
-        a.heavy_list = h->d_heavy;
-        a.heavy_cap = h->heavy_cap;
-        a.heavy_count = h->d_ctr + kCtrHeavyChunks;
-        a.heavy_next = h->d_ctr + kCtrHeavyNext;
-        a.heavy_buckets = h->d_ctr + kCtrHeavyBuckets;
-        HIPX(hipEventRecord(h->ev[8], st));
-        HIPX(launch_count<NW>(nf, h->cfg.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
-        HIPX(launch_heavy<NW>(h->cu_count, src, a, st));
-        HIPX(hipEventRecord(h->ev[9], st));
-        HIPX(launch_spill_emit<NW>(a, st));
-        SegArgs sa{};
-        sa.nf = nf;
-        sa.sbits = sbits;
-        sa.fine_off = fine_off;
-        sa.nfirst = d_nfirst;
-        sa.nrec = d_nrec;
-        sa.ndist = d_ndist;
-        sa.foff = h->d_foff;
-        sa.roff = h->d_roff;
-        sa.root_ovf = h->d_small + kSmRootOvf;
-        sa.root_main = h->d_small + kSmRootMain;
-        sa.root_off = h->d_small + kSmRootOff;
-        sa.ovf_off = h->d_small + kSmOvfOff;
-        sa.ovf_cursor = h->d_small + kSmOvfCursor;
-        sa.ctr = h->d_ctr;
-        sa.rec_out_cap = (h->key_cap * 8ULL * NW) / rec_bytes(NW);
-        // final firsts into the P3 key input (free now), final records into the slices buffer
-        HIPX(launch_finalize<NW>(sa, a, slices, src, h->d_R, reinterpret_cast<uint8_t*>(slices), st));
-        HIPX(hipEventRecord(h->ev[10], st));
         HIPX(hipStreamSynchronize(st));
 
         unsigned long long ctr[kCtrCount];
@@ -549,9 +612,14 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         h->root_off.assign(2 * 257, 0);
         HIPX(hipMemcpy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
         // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total
+        // (a graph replay records the total only)
         const int pairs[6][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {9, 10}};
         float ms = 0;
         for (int p = 0; p < 6; ++p) {
+            if (use_graph) {
+                h->phase_ms[p] = 0.f;
+                continue;
+            }
             HIPX(hipEventElapsedTime(&ms, h->ev[pairs[p][0]], h->ev[pairs[p][1]]));
             h->phase_ms[p] = ms;
         }
@@ -642,6 +710,7 @@ int kc_destroy(kc_engine* h)
     hipFree(h->d_spill_keys);
     hipFree(h->d_spill_cnt);
     hipFree(h->d_bounds);
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     hipFree(h->d_lut);
     hipFree(h->d_heavy);
     for (int i = 0; i < 2; ++i) {

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "kmc_device.h"
 #include "kmc_kernels.h"
 
@@ -1510,6 +1512,38 @@ hipError_t launch_histogram(const uint8_t* recs, uint64_t nrec, const uint64_t* 
 
 template hipError_t launch_histogram<1>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
 template hipError_t launch_histogram<2>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
+
+// Fills used instead of memset nodes in the captured pipeline graph.
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u8(uint8_t* __restrict__ p, uint64_t n, uint8_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill(void* p, uint64_t bytes, uint8_t v, hipStream_t st)
+{
+    if (!bytes) return hipSuccess;
+    const uint64_t aligned = (reinterpret_cast<uintptr_t>(p) % 4 == 0) ? bytes / 4 : 0;
+    if (aligned) {
+        const uint32_t v4 = 0x01010101u * v;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(8192, (aligned + 255) / 256);
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, st, reinterpret_cast<uint32_t*>(p), aligned, v4);
+        KMC_LAUNCH_CHECK();
+    }
+    const uint64_t rest = bytes - aligned * 4;
+    if (rest) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(8192, (rest + 255) / 256);
+        hipLaunchKernelGGL(k_fill_u8, dim3(grid), dim3(256), 0, st, reinterpret_cast<uint8_t*>(p) + aligned * 4, rest, v);
+        KMC_LAUNCH_CHECK();
+    }
+    return hipSuccess;
+}
 
 // Plain copy of n16 16-byte words: the traffic reference that HBM counters
 // of the pipeline kernels are scaled by (reads 16*n16 bytes, writes 16*n16).
