@@ -174,7 +174,8 @@ def cpu_baseline_port(workload: int, k: int) -> dict:
             "sample": f"workload {workload} at 1/20 scale ({n} k-mers), oracle extract + sort + run-length, {dt:.2f} s"}
 
 
-def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, counter: str, graph: bool) -> dict:
+def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, counter: str, graph: bool,
+                  autotune: bool = False) -> dict:
     """Resident run of one partition method: plan (learned only), copy, warm up, time `steps` passes."""
     part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
     cnt = K.KC_COUNTER_BLOOM if counter == "bloom" else K.KC_COUNTER_EXACT
@@ -189,6 +190,14 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     t0 = time.time()
     eng.push_block(seq)
     eng.finish()  # H2D copy + first full pass (also the first warmup)
+    tuned = None
+    if autotune:  # untimed: time the candidates once, keep the fastest
+        t_t = time.perf_counter()
+        r = eng.autotune()
+        tuned = {"sub_bits": r.sub_bits, "counter": "exact" if r.counter == K.KC_COUNTER_EXACT else "bloom",
+                 "heavy_keys": r.heavy_keys, "candidates": r.candidates, "best_ms": r.best_ms,
+                 "tune_ms": (time.perf_counter() - t_t) * 1e3}
+        log(f"[{partition}] autotune: {tuned}")
     st = eng.stats()
     log(f"[{partition}] resident in {time.time() - t0:.1f} s; K={st.total_kmers} firsts={st.firsts} "
         f"records={st.records} spilled={st.spilled_keys} fine_buckets={st.fine_buckets} "
@@ -221,6 +230,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         "plan_ms": plan_ms,
         "max_over_mean_bucket": st.max_bucket_keys / max(st.mean_bucket_keys, 1e-9),
         "histogram_ms": hist_ms,
+        "autotune": tuned,
     }
     if part == K.KC_PARTITION_LEARNED:
         r["learned"] = {
@@ -246,7 +256,7 @@ def summary(r: dict) -> dict:
         "max_over_mean_bucket": r["max_over_mean_bucket"], "extra_passes": st.extra_passes,
         "spilled_keys": st.spilled_keys, "heavy_buckets": st.heavy_buckets, "rounds": st.rounds,
         "phase_ms": r["per_phase"],
-        "histogram_ms": r["histogram_ms"],
+        "histogram_ms": r["histogram_ms"], "autotune": r["autotune"],
     }
 
 
@@ -259,6 +269,8 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--partition", choices=["prefix", "learned", "both"], default="both")
     ap.add_argument("--counter", choices=["bloom", "exact"], default="bloom")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the candidate fan-outs, counters and heavy-bucket thresholds first (untimed)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the pipeline as a captured HIP graph (per-kernel times are then not recorded)")
     ap.add_argument("--seed", type=int, default=1)
@@ -280,7 +292,8 @@ def main() -> int:
     seq_bytes = len(seq)
     log(f"generated {seq_bytes / 1e9:.2f} GB of sequence in {time.time() - t0:.1f} s")
     parts = ["prefix", "learned"] if args.partition == "both" else [args.partition]
-    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter, args.graph) for p in parts}
+    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter, args.graph, args.autotune)
+            for p in parts}
     del seq
     main_run = runs["learned"] if "learned" in runs else runs["prefix"]
     st = main_run["stats"]

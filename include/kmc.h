@@ -188,6 +188,20 @@ KC_API int kc_destroy(kc_engine* h);
  * the reference cannot separate ingest from counting. */
 KC_API int kc_count_resident(kc_engine* h);
 
+/* Autotuner (no counterpart in the reference; its CPU pipeline has no such
+ * parameters): times one pass over the resident input for each candidate --
+ * the automatic fine fan-out and one bit less (tile and bucket size), the
+ * Bloom and the exact counter, two heavy-bucket thresholds -- keeps the
+ * fastest in the handle's configuration and leaves its results in place. */
+typedef struct kc_tune_result {
+    uint32_t sub_bits;    /* chosen fine fan-out bits */
+    uint32_t counter;     /* chosen kc_counter */
+    uint32_t heavy_keys;  /* chosen heavy-bucket threshold */
+    uint32_t candidates;  /* configurations timed */
+    double best_ms;       /* device time of the chosen one */
+} kc_tune_result;
+KC_API int kc_autotune(kc_engine* h, kc_tune_result* out);
+
 /* Number of u64 words per key for k (1 or 2), 0 if unsupported. */
 KC_API uint32_t kc_words_for_k(uint32_t k);
 

@@ -49,6 +49,11 @@ class Config(ctypes.Structure):
     ]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("sub_bits", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("heavy_keys", ctypes.c_uint32),
+                ("candidates", ctypes.c_uint32), ("best_ms", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("k", ctypes.c_uint32), ("key_words", ctypes.c_uint32),
@@ -108,7 +113,7 @@ EXPORTED = [
     "kc_last_error", "kc_debug_xxh3", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
-    "kmc_write_histogram",
+    "kmc_write_histogram", "kc_autotune",
 ]
 
 _lib = None
@@ -145,6 +150,7 @@ def lib() -> ctypes.CDLL:
     L.kc_debug_xxh3.argtypes = [vp, u64, u32, vp]
     L.kc_debug_copy.argtypes = [u64, u32, ctypes.POINTER(ctypes.c_float)]
     L.kc_histogram.argtypes = [vp, u32, u32, u32, vp]
+    L.kc_autotune.argtypes = [vp, vp]
     L.kmc_write_histogram.argtypes = [cp, vp, u32, u32]
     L.kc_debug_gemm_bf16.argtypes = [vp, vp, vp, u32, u32, u32]
     L.kc_debug_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
@@ -274,6 +280,12 @@ class Engine:
 
     def count_resident(self) -> None:
         self._check(self._L.kc_count_resident(self._h), "kc_count_resident")
+
+    def autotune(self) -> TuneResult:
+        """Time the candidate configurations on the resident input and keep the fastest."""
+        r = TuneResult()
+        self._check(self._L.kc_autotune(self._h, ctypes.byref(r)), "kc_autotune")
+        return r
 
     def histogram(self, min_freq: int = 1, max_freq: int = 10000, precise: bool = True) -> np.ndarray:
         """histogram_tool's count histogram (precise or approximate) of the counted

@@ -802,6 +802,61 @@ int kc_count_resident(kc_engine* h)
     return run_pipeline(h);
 }
 
+int kc_autotune(kc_engine* h, kc_tune_result* out)
+{
+    if (!h) return KC_ERR_ARG;
+    HIPX(hipSetDevice(h->device));
+    if (h->stage_len) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
+    if (!h->seq_len) return h->fail(KC_ERR_STATE, "no resident input: push blocks and call kc_finish first");
+    // candidates: fine fan-out around the automatic one, both counters, two
+    // heavy-bucket thresholds; each is timed on one pass over the resident input
+    const kc_config saved = h->cfg;
+    const uint64_t tile = ext_tile(h->nw);
+    const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
+    kc_config probe = saved;
+    probe.sub_bits = 0;
+    h->cfg = probe;
+    const uint32_t sb_auto = choose_sbits(h, padded);
+    std::vector<uint32_t> sbs{sb_auto};
+    if (sb_auto > 1) sbs.push_back(sb_auto - 1);
+    if (saved.sub_bits && saved.sub_bits != KC_SUB_BITS_NONE && saved.sub_bits != sb_auto) sbs.push_back(saved.sub_bits);
+    const uint32_t counters[2] = {KC_COUNTER_BLOOM, KC_COUNTER_EXACT};
+    const uint32_t heavies[2] = {0u, 1u << 18};
+    double best = 1e300;
+    kc_config best_cfg = saved;
+    uint32_t tried = 0;
+    for (uint32_t sb : sbs)
+        for (uint32_t c : counters)
+            for (uint32_t hv : heavies) {
+                kc_config t = saved;
+                t.sub_bits = sb ? sb : KC_SUB_BITS_NONE;
+                t.counter = c;
+                t.heavy_keys = hv;
+                h->cfg = t;
+                const int rc = run_pipeline(h);
+                if (rc) {
+                    h->cfg = saved;
+                    return rc;
+                }
+                ++tried;
+                if (h->phase_ms[6] < best) {
+                    best = h->phase_ms[6];
+                    best_cfg = t;
+                }
+            }
+    h->cfg = best_cfg;
+    const int rc = run_pipeline(h);  // results of the chosen configuration
+    if (rc) return rc;
+    if (out) {
+        out->sub_bits = best_cfg.sub_bits == KC_SUB_BITS_NONE ? 0u : best_cfg.sub_bits;
+        out->counter = best_cfg.counter;
+        out->heavy_keys = best_cfg.heavy_keys ? best_cfg.heavy_keys : (uint32_t)(1u << 20);
+        out->candidates = tried;
+        out->best_ms = best;
+    }
+    return KC_OK;
+}
+
 int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
 {
     if (!h || !cb) return KC_ERR_ARG;

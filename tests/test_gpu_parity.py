@@ -289,3 +289,21 @@ def test_rounds_reject_a_root_bucket_larger_than_a_round():
         e.push_block(seq)
         with pytest.raises(K.KmcError):
             e.finish()
+
+
+def test_autotune_keeps_the_tables_exact():
+    seq = K.generate(2, scale=400, seed=15, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    with K.Engine(31) as e:
+        e.push_block(seq)
+        e.finish()
+        r = e.autotune()
+        assert r.candidates >= 4
+        assert r.counter in (K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT)
+        assert r.best_ms > 0
+        f, rr, s = e.firsts(), e.records(), e.stats()
+        check_outputs(f, rr, s, 31, keys, counts)
+        e.count_resident()  # the tuned configuration stays in the handle
+        k2, c2 = C.merge_outputs(e.firsts(), e.records(), 1)
+        np.testing.assert_array_equal(k2, keys)
+        np.testing.assert_array_equal(c2, counts)
