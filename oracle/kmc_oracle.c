@@ -379,12 +379,17 @@ ORC_API uint64_t orc_count_sorted(uint64_t* keys, uint64_t n, uint32_t nw, uint3
 }
 
 /* ------------------------------------------------------------------------- */
-/* A7: XXH3-128 for 8- and 16-byte inputs, seed 0, default secret.           */
+/* A7: XXH3-128 for 8-, 16- and 32-byte inputs, seed 0, default secret.     */
 /* Restated from the published XXH3 algorithm as vendored by the reference   */
 /* (include/xxhash.h v0.8.3: XXH3_len_4to8_128b :6900, XXH3_len_9to16_128b   */
-/* :6927, XXH3_avalanche :4616).  Only the 48 secret bytes these two paths   */
-/* read (offsets 16..63 of XXH3_kSecret, :4398) are needed.                  */
+/* :6927, XXH3_len_17to128_128b with XXH128_mix32B, XXH3_avalanche :4616).   */
+/* Only the secret bytes these paths read (offsets 0..63 of XXH3_kSecret,    */
+/* :4398) are needed.                                                        */
 /* ------------------------------------------------------------------------- */
+
+static const uint8_t kSecret0to16[16] = {
+    0xb8, 0xfe, 0x6c, 0x39, 0x23, 0xa4, 0x4b, 0xbe, 0x7c, 0x01, 0x81, 0x2c, 0xf7, 0x21, 0xad, 0x1c,
+};
 
 static const uint8_t kSecret16to64[48] = {
     0xde, 0xd4, 0x6d, 0xe9, 0x83, 0x90, 0x97, 0xdb, 0x72, 0x40, 0xa4, 0xa4, 0xb7, 0xb3, 0x67, 0x1f,
@@ -404,6 +409,7 @@ static inline uint64_t le64(const uint8_t* p)
 #define P32_2 0x85EBCA77ULL
 #define PMX1 0x165667919E3779F9ULL
 #define PMX2 0x9FB21C651E98DF25ULL
+#define P64_4 0x85EBCA77C2B2AE63ULL
 
 static inline void mul128(uint64_t a, uint64_t b, uint64_t* lo, uint64_t* hi)
 {
@@ -453,6 +459,20 @@ ORC_API int orc_xxh3_128(const uint64_t* words, uint32_t nw, uint64_t* out_lo, u
         hhi += mhi * P64_2;
         *out_lo = aval(hlo);
         *out_hi = aval(hhi);
+        return 0;
+    }
+    if (nw == 4) {
+        /* 32 bytes: one XXH128_mix32B over (bytes 0..15, bytes 16..31) */
+        const uint64_t len = 32;
+        uint64_t alo = len * P64_1, ahi = 0, l, h;
+        mul128(words[0] ^ le64(kSecret0to16), words[1] ^ le64(kSecret0to16 + 8), &l, &h);   /* mix16B(in1, s+0) */
+        alo += l ^ h;
+        alo ^= words[2] + words[3];
+        mul128(words[2] ^ le64(kSecret16to64), words[3] ^ le64(kSecret16to64 + 8), &l, &h); /* mix16B(in2, s+16) */
+        ahi += l ^ h;
+        ahi ^= words[0] + words[1];
+        *out_lo = aval(alo + ahi);
+        *out_hi = (uint64_t)0 - aval(alo * P64_1 + ahi * P64_4 + len * P64_2);
         return 0;
     }
     return -1;

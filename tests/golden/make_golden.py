@@ -149,7 +149,7 @@ def harness_vectors() -> dict:
     H.ref_bloom_classify.argtypes = [vp, u64, u32, u64, vp, vp]
     rng = np.random.default_rng(2024)
     out = {"xxh3": [], "extract": [], "bloom": []}
-    for nw in (1, 2):
+    for nw in (1, 2, 4):
         for i in range(64):
             key = rng.integers(0, 2**63, size=nw, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, size=nw, dtype=np.uint64)
             if i < 2:
@@ -158,7 +158,7 @@ def harness_vectors() -> dict:
             H.ref_xxh3_128(key.ctypes.data, 8 * nw, ctypes.byref(lo), ctypes.byref(hi))
             out["xxh3"].append({"key": [int(x) for x in key], "lo": lo.value, "hi": hi.value})
     alphabet = np.frombuffer(b"ACGTacgtuUNn\n\rX", dtype=np.uint8)
-    for k in (1, 3, 15, 16, 17, 31, 32, 33, 47, 63, 64, 65, 128):
+    for k in (1, 3, 15, 16, 17, 31, 32, 33, 47, 63, 64, 65, 95, 96, 97, 100, 127, 128):
         p = np.array([.22, .22, .22, .22, .02, .02, .02, .02, .005, .005, .01, .005, .005, .003, .002])
         seq = bytes(rng.choice(alphabet, size=1500, p=p / p.sum()))
         nw = 1 if k <= 32 else (2 if k <= 64 else 4)
