@@ -170,7 +170,10 @@ def lib() -> ctypes.CDLL:
 
 
 def words_for_k(k: int) -> int:
-    return 1 if k <= 32 else 2
+    """Key words N of k (src/main.cpp:579-595): 1 up to 32, 2 up to 64, 4 up to 128."""
+    if not 1 <= k <= 128:
+        raise ValueError("k_len must be in [1, 128]")
+    return 1 if k <= 32 else (2 if k <= 64 else 4)
 
 
 def record_dtype(nw: int) -> np.dtype:

@@ -1,5 +1,5 @@
 // kmc_device.h -- device-side building blocks shared by the engine kernels:
-// base encoding, canonical k-mer construction, XXH3-128 for 8/16-byte keys,
+// base encoding, canonical k-mer construction, XXH3-128 for 8/16/32-byte keys,
 // Bloom probe and key routing.  gfx950 (CDNA4), wave64.
 #pragma once
 
@@ -9,7 +9,8 @@
 namespace kmc {
 
 constexpr int kWave = 64;
-constexpr uint64_t kEmpty = ~0ULL;  // never a canonical key word for k <= 63
+// empty table word: tables store keys encoded so that no stored word is all ones (TableKey)
+constexpr uint64_t kEmpty = ~0ULL;
 
 template <int NW>
 struct Key {
@@ -71,21 +72,66 @@ __device__ __forceinline__ Key<1> canon1(uint64_t F, uint32_t k)
     return r;
 }
 
-// 33 <= k <= 63: F = (fhi, flo) is a 2k-bit value
+// 33 <= k <= 64: F = (fhi, flo) is a 2k-bit value
 __device__ __forceinline__ Key<2> canon2(uint64_t fhi, uint64_t flo, uint32_t k)
 {
-    const uint32_t hb = 2 * k - 64;  // bits of fhi in use, 2..62
-    const uint64_t hmask = (1ULL << hb) - 1ULL;
+    const uint32_t hb = 2 * k - 64;  // bits of fhi in use, 2..64
+    const uint64_t hmask = hb == 64 ? ~0ULL : (1ULL << hb) - 1ULL;
     fhi &= hmask;
     const uint64_t chi = ~fhi & hmask, clo = ~flo;
-    const uint32_t sh = 128 - 2 * k;  // 2..62
-    const uint64_t rhi = (chi << sh) | (clo >> (64 - sh));
+    const uint32_t sh = 128 - 2 * k;  // 0..62
+    const uint64_t rhi = sh ? (chi << sh) | (clo >> (64 - sh)) : chi;
     const uint64_t rlo = clo << sh;
     const uint64_t shi = rev2(flo), slo = rev2(fhi);
     const bool seq_less = (shi < rhi) || (shi == rhi && slo < rlo);
     Key<2> r;
     r.w[0] = seq_less ? shi : rhi;
     r.w[1] = seq_less ? slo : rlo;
+    return r;
+}
+
+// 65 <= k <= 128: S = (s0, s1, s2, s3) holds the codes of the last 128 bases
+// ending at the window's last base (oldest on top, 2 bits each).  As in
+// canon1/canon2, seq = pack(reverse(window)) and rev = pack(complement(window)),
+// left aligned in four words.  For k = 96 the reference never masks word 2 of
+// seq (kmer_last_index 2, remainder 0, src/GetKmer.h:38-47,:71-73), so the
+// bases shifted out of the window stay in word 3 of seq: the 32 bases before
+// the window, newest first, zero where they precede the current run.  `pre`
+// holds those 32 codes (oldest on top), already zeroed before the run start.
+__device__ __forceinline__ Key<4> canon4(uint64_t s0, uint64_t s1, uint64_t s2, uint64_t s3, uint32_t k, uint64_t pre)
+{
+    // forward window F: the low 2k bits of S
+    const uint32_t bits = 2 * k;                     // 130..256
+    uint64_t f0 = bits >= 256 ? s0 : (bits > 192 ? s0 & ((1ULL << (bits - 192)) - 1ULL) : 0ULL);
+    uint64_t f1 = bits >= 192 ? s1 : s1 & ((1ULL << (bits - 128)) - 1ULL);
+    const uint64_t f2 = s2, f3 = s3;
+    // rev = ~F (2k bits) shifted to the top of 256 bits
+    const uint64_t m0 = bits >= 256 ? ~0ULL : (bits > 192 ? (1ULL << (bits - 192)) - 1ULL : 0ULL);
+    const uint64_t m1 = bits >= 192 ? ~0ULL : (1ULL << (bits - 128)) - 1ULL;
+    uint64_t c0 = ~f0 & m0, c1 = ~f1 & m1, c2 = ~f2, c3 = ~f3;
+    const uint32_t sh = 256 - bits;                  // 0..126
+    uint64_t r0, r1, r2, r3;
+    if (sh >= 64) {  // 130 <= 2k <= 192: F fits words 1..3
+        const uint32_t t = sh - 64;
+        r0 = t ? (c1 << t) | (c2 >> (64 - t)) : c1;
+        r1 = t ? (c2 << t) | (c3 >> (64 - t)) : c2;
+        r2 = c3 << t;
+        r3 = 0;
+    } else {
+        r0 = sh ? (c0 << sh) | (c1 >> (64 - sh)) : c0;
+        r1 = sh ? (c1 << sh) | (c2 >> (64 - sh)) : c1;
+        r2 = sh ? (c2 << sh) | (c3 >> (64 - sh)) : c2;
+        r3 = c3 << sh;
+    }
+    // seq = the 128 two-bit groups of F in reverse order (newest base on top)
+    const uint64_t q0 = rev2(f3), q1 = rev2(f2), q2 = rev2(f1);
+    const uint64_t q3 = k == 96 ? rev2(pre) : rev2(f0);
+    const bool seq_less = q0 != r0 ? q0 < r0 : (q1 != r1 ? q1 < r1 : (q2 != r2 ? q2 < r2 : q3 < r3));
+    Key<4> r;
+    r.w[0] = seq_less ? q0 : r0;
+    r.w[1] = seq_less ? q1 : r1;
+    r.w[2] = seq_less ? q2 : r2;
+    r.w[3] = seq_less ? q3 : r3;
     return r;
 }
 
@@ -151,6 +197,76 @@ __device__ __forceinline__ void xxh3_128(const Key<2>& key, uint64_t& lo, uint64
     hh += mh * kXxP64_2;
     lo = xx_avalanche(hl);
     hi = xx_avalanche(hh);
+}
+
+// 32 bytes (XXH3_len_17to128_128b: one XXH128_mix32B, :6999); kXxSecret0..3
+// are secret bytes 0..31 read little-endian.
+constexpr uint64_t kXxP64_4 = 0x85EBCA77C2B2AE63ULL;
+constexpr uint64_t kXxSecret0 = 0xbe4ba423396cfeb8ULL;
+constexpr uint64_t kXxSecret1 = 0x1cad21f72c81017cULL;
+constexpr uint64_t kXxSecret2 = 0xdb979083e96dd4deULL;
+constexpr uint64_t kXxSecret3 = 0x1f67b3b7a4a44072ULL;
+
+__device__ __forceinline__ uint64_t xx_fold(uint64_t a, uint64_t b) { return (a * b) ^ __umul64hi(a, b); }
+
+__device__ __forceinline__ void xxh3_128(const Key<4>& key, uint64_t& lo, uint64_t& hi)
+{
+    uint64_t alo = 32ULL * kXxP64_1, ahi = 0;
+    alo += xx_fold(key.w[0] ^ kXxSecret0, key.w[1] ^ kXxSecret1);
+    alo ^= key.w[2] + key.w[3];
+    ahi += xx_fold(key.w[2] ^ kXxSecret2, key.w[3] ^ kXxSecret3);
+    ahi ^= key.w[0] + key.w[1];
+    lo = xx_avalanche(alo + ahi);
+    hi = 0ULL - xx_avalanche(alo * kXxP64_1 + ahi * kXxP64_4 + 32ULL * kXxP64_2);
+}
+
+// ---------------------------------------------------------------- table keys
+// Open-addressing tables mark a free word with kEmpty (all ones), so a stored
+// word must never be all ones.  k <= 32: a canonical key word never is.
+// k <= 64: only k = 64 reaches it, and only with the key (~0, 0) (first 32
+// bases A, last 32 T); it is stored as (~0 - 1, 1), which no canonical key of
+// k <= 64 can be.  k <= 128: any word can be all ones, so the 256 key bits
+// are stored as five chunks of at most 63 bits.
+template <int NW> struct TableWords { static constexpr int value = NW; };
+template <> struct TableWords<4> { static constexpr int value = 5; };
+
+template <int NW>
+__device__ __forceinline__ Key<TableWords<NW>::value> to_table(const Key<NW>& k)
+{
+    if constexpr (NW == 1) {
+        return k;
+    } else if constexpr (NW == 2) {
+        Key<2> t = k;
+        if (k.w[0] == ~0ULL && k.w[1] == 0ULL) { t.w[0] = ~0ULL - 1; t.w[1] = 1ULL; }
+        return t;
+    } else {
+        Key<5> t;
+        t.w[0] = k.w[0] >> 1;
+        t.w[1] = ((k.w[0] & 1ULL) << 62) | (k.w[1] >> 2);
+        t.w[2] = ((k.w[1] & 3ULL) << 61) | (k.w[2] >> 3);
+        t.w[3] = ((k.w[2] & 7ULL) << 60) | (k.w[3] >> 4);
+        t.w[4] = k.w[3] & 15ULL;
+        return t;
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ Key<NW> from_table(const Key<TableWords<NW>::value>& t)
+{
+    if constexpr (NW == 1) {
+        return t;
+    } else if constexpr (NW == 2) {
+        Key<2> k = t;
+        if (t.w[0] == ~0ULL - 1 && t.w[1] == 1ULL) { k.w[0] = ~0ULL; k.w[1] = 0ULL; }
+        return k;
+    } else {
+        Key<4> k;
+        k.w[0] = (t.w[0] << 1) | (t.w[1] >> 62);
+        k.w[1] = (t.w[1] << 2) | (t.w[2] >> 61);
+        k.w[2] = (t.w[2] << 3) | (t.w[3] >> 60);
+        k.w[3] = (t.w[3] << 4) | t.w[4];
+        return k;
+    }
 }
 
 // Bloom probe of ConcurrentBloomFilter (src/BloomFilter.h:207-214, :248-258):

@@ -129,9 +129,14 @@ struct kc_engine {
 
 namespace {
 
-uint64_t ext_tile(uint32_t nw) { return nw == 1 ? ExtractCfg<1>::kThreads * 16ULL : ExtractCfg<2>::kThreads * 16ULL; }
-uint64_t sub_tile(uint32_t nw) { return nw == 1 ? (uint64_t)SubCfg<1>::kTile : (uint64_t)SubCfg<2>::kTile; }
+uint64_t ext_tile(uint32_t nw)
+{
+    return (nw == 1 ? ExtractCfg<1>::kThreads : nw == 2 ? ExtractCfg<2>::kThreads : ExtractCfg<4>::kThreads) * 16ULL;
+}
+uint64_t sub_tile(uint32_t nw) { return nw == 1 ? SubCfg<1>::kTile : nw == 2 ? SubCfg<2>::kTile : SubCfg<4>::kTile; }
 uint64_t rec_bytes(uint32_t nw) { return 8ULL * nw + 8; }
+// words of a key in the open-addressing tables (k <= 128 keys take five 63-bit words)
+uint64_t table_words(uint32_t nw) { return nw == 4 ? (uint64_t)TableWords<4>::value : nw; }
 
 uint64_t pow2_at_least(uint64_t v)
 {
@@ -267,7 +272,7 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
     const uint64_t want_spill = std::min<uint64_t>(1ULL << 26, std::max<uint64_t>(1ULL << 16, pow2_at_least(h->key_cap / 512)));
     if (h->spill_cap < want_spill) {
         hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
-        if (hipMalloc(&h->d_spill_keys, want_spill * 8ULL * h->nw) != hipSuccess ||
+        if (hipMalloc(&h->d_spill_keys, want_spill * 8ULL * table_words(h->nw)) != hipSuccess ||
             hipMalloc(&h->d_spill_cnt, want_spill * 4ULL) != hipSuccess)
             return h->fail(KC_ERR_NOMEM, "device allocation for the spill table failed");
         h->spill_cap = want_spill;
@@ -458,7 +463,7 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
             HIPX(fill(h->d_ctr, sizeof(unsigned long long) * kCtrCount, 0));
             // a graph cannot skip the clear, so a captured run always clears the spill table
             if (h->spill_dirty || graph) {
-                HIPX(fill(h->d_spill_keys, h->spill_cap * 8ULL * NW, 0xFF));
+                HIPX(fill(h->d_spill_keys, h->spill_cap * 8ULL * TableWords<NW>::value, 0xFF));
                 HIPX(fill(h->d_spill_cnt, h->spill_cap * 4ULL, 0));
             }
             if (sbits) HIPX(fill(h->d_fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
@@ -581,7 +586,7 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
             const uint64_t ncap = std::min<uint64_t>(1ULL << 31, std::max<uint64_t>(h->spill_cap * 4, pow2_at_least(ctr[kCtrSpilled] * 4)));
             hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
             h->d_spill_keys = nullptr; h->d_spill_cnt = nullptr; h->spill_cap = 0;
-            if (hipMalloc(&h->d_spill_keys, ncap * 8ULL * NW) != hipSuccess || hipMalloc(&h->d_spill_cnt, ncap * 4ULL) != hipSuccess)
+            if (hipMalloc(&h->d_spill_keys, ncap * 8ULL * TableWords<NW>::value) != hipSuccess || hipMalloc(&h->d_spill_cnt, ncap * 4ULL) != hipSuccess)
                 return h->fail(KC_ERR_NOMEM, "spill table growth failed");
             h->spill_cap = ncap;
             h->spill_dirty = true;
@@ -634,7 +639,11 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
 int run_pipeline(kc_engine* h)
 {
     h->have_results = false;
-    return h->nw == 1 ? run_pipeline_t<1>(h) : run_pipeline_t<2>(h);
+    switch (h->nw) {
+    case 1: return run_pipeline_t<1>(h);
+    case 2: return run_pipeline_t<2>(h);
+    default: return run_pipeline_t<4>(h);
+    }
 }
 
 }  // namespace
@@ -643,8 +652,8 @@ extern "C" {
 
 uint32_t kc_words_for_k(uint32_t k)
 {
-    if (k == 0 || k > 63) return 0;
-    return k <= 32 ? 1u : 2u;
+    if (k == 0 || k > 128) return 0;
+    return k <= 32 ? 1u : k <= 64 ? 2u : 4u;
 }
 
 const char* kc_last_error(kc_engine* h)
@@ -946,7 +955,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (precise) {
             free_table();
-            const uint64_t kbytes = cap * 8ULL * h->nw;
+            const uint64_t kbytes = cap * 8ULL * table_words(h->nw);
             if (scratch && kbytes + cap * 4ULL <= scratch_bytes) {
                 d_keys = static_cast<unsigned long long*>(scratch);
                 d_vals = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(scratch) + kbytes);
@@ -960,7 +969,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
         }
         hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(long long) * nb, h->st);
         if (e == hipSuccess) e = hipMemsetAsync(d_err, 0, sizeof(unsigned long long), h->st);
-        if (e == hipSuccess && precise) e = hipMemsetAsync(d_keys, 0xFF, cap * 8ULL * h->nw, h->st);
+        if (e == hipSuccess && precise) e = hipMemsetAsync(d_keys, 0xFF, cap * 8ULL * table_words(h->nw), h->st);
         HistArgs a{};
         a.hist = d_hist;
         a.minf = min_freq;
@@ -971,8 +980,9 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
         a.cap = cap;
         a.error_flags = d_err;
         if (e == hipSuccess)
-            e = h->nw == 1 ? launch_histogram<1>(recs, nrec, firsts, nfirst, a, h->st)
-                           : launch_histogram<2>(recs, nrec, firsts, nfirst, a, h->st);
+            e = h->nw == 1   ? launch_histogram<1>(recs, nrec, firsts, nfirst, a, h->st)
+                : h->nw == 2 ? launch_histogram<2>(recs, nrec, firsts, nfirst, a, h->st)
+                             : launch_histogram<4>(recs, nrec, firsts, nfirst, a, h->st);
         unsigned long long err = 0;
         if (e == hipSuccess) e = hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, h->st);
         if (e == hipSuccess) e = hipStreamSynchronize(h->st);
@@ -1069,14 +1079,19 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
 int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out)
 {
     if ((!keys || !out) && n) return KC_ERR_ARG;
-    if (nw != 1 && nw != 2) return KC_ERR_ARG;
+    if (nw != 1 && nw != 2 && nw != 4) return KC_ERR_ARG;
     if (!n) return KC_OK;
     uint64_t *dk = nullptr, *dout = nullptr;
     if (hipMalloc(&dk, n * 8 * nw) != hipSuccess) return KC_ERR_NOMEM;
     if (hipMalloc(&dout, n * 16) != hipSuccess) { hipFree(dk); return KC_ERR_NOMEM; }
     int rc = KC_OK;
     if (hipMemcpy(dk, keys, n * 8 * nw, hipMemcpyHostToDevice) != hipSuccess) rc = KC_ERR_HIP;
-    if (!rc && (nw == 1 ? launch_debug_hash<1>(dk, n, dout, nullptr) : launch_debug_hash<2>(dk, n, dout, nullptr)) != hipSuccess) rc = KC_ERR_HIP;
+    if (!rc) {
+        const hipError_t e = nw == 1   ? launch_debug_hash<1>(dk, n, dout, nullptr)
+                             : nw == 2 ? launch_debug_hash<2>(dk, n, dout, nullptr)
+                                       : launch_debug_hash<4>(dk, n, dout, nullptr);
+        if (e != hipSuccess) rc = KC_ERR_HIP;
+    }
     if (!rc && hipMemcpy(out, dout, n * 16, hipMemcpyDeviceToHost) != hipSuccess) rc = KC_ERR_HIP;
     hipFree(dk);
     hipFree(dout);

@@ -269,7 +269,7 @@ int kmc_write_outputs(kc_engine* h, const char* tmp_dir)
     if (!dir.empty() && dir.back() != '/') dir.push_back('/');
     LowWriter lw;
     lw.k = k;
-    lw.nw = k <= 32 ? 1 : 2;
+    lw.nw = k <= 32 ? 1 : (k <= 64 ? 2 : 4);
     lw.pb = kmc_packed_bytes(k);
     lw.fd = ::open((dir + "low.bin").c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
     if (lw.fd < 0) return KC_ERR_ARG;

@@ -19,11 +19,14 @@ constexpr unsigned long long kErrHistFull = 8ULL; // histogram record table: a p
 template <int NW> struct ExtractCfg;
 template <> struct ExtractCfg<1> { static constexpr int kThreads = 512; };
 template <> struct ExtractCfg<2> { static constexpr int kThreads = 256; };
+// 256 threads at least: the bucket histogram and its scan take one thread per bucket
+template <> struct ExtractCfg<4> { static constexpr int kThreads = 256; };
 
 // P2: kTile keys of one coarse bucket per workgroup.
 template <int NW> struct SubCfg;
 template <> struct SubCfg<1> { static constexpr int kThreads = 512; static constexpr int kTile = 8192; };
 template <> struct SubCfg<2> { static constexpr int kThreads = 256; static constexpr int kTile = 4096; };
+template <> struct SubCfg<4> { static constexpr int kThreads = 256; static constexpr int kTile = 2048; };
 
 // P3: one workgroup per fine bucket (1 per CU).  LDS: Bloom words + count
 // table; buckets above kPassKeys keys are processed in hash-range passes.
@@ -50,6 +53,17 @@ template <> struct CountCfg<2> {
     static constexpr uint32_t kProbe = 64;
     static constexpr int kLoads = 4;
     static constexpr uint32_t kExactSlots = 6144;
+};
+// k 65..128: a table slot is five 63-bit words plus its count (44 bytes)
+template <> struct CountCfg<4> {
+    static constexpr int kThreads = 1024;
+    static constexpr uint32_t kBloomWords = 4096;
+    static constexpr uint32_t kTableSlots = 2048;
+    static constexpr uint32_t kPassKeys = 24576;
+    static constexpr uint32_t kPassDistinct = 1250;
+    static constexpr uint32_t kProbe = 64;
+    static constexpr int kLoads = 4;
+    static constexpr uint32_t kExactSlots = 2560;
 };
 
 // probes into the global spill table before a key is declared unplaceable

@@ -76,9 +76,12 @@ __device__ __forceinline__ Key<NW> load_key(const uint64_t* p, uint64_t i)
     if constexpr (NW == 1) {
         k.w[0] = p[i];
     } else {
-        const ulonglong2 v = reinterpret_cast<const ulonglong2*>(p)[i];
-        k.w[0] = v.x;
-        k.w[1] = v.y;
+#pragma unroll
+        for (int h = 0; h < NW / 2; ++h) {
+            const ulonglong2 v = reinterpret_cast<const ulonglong2*>(p)[i * (NW / 2) + h];
+            k.w[2 * h] = v.x;
+            k.w[2 * h + 1] = v.y;
+        }
     }
     return k;
 }
@@ -89,10 +92,13 @@ __device__ __forceinline__ void store_key(uint64_t* p, uint64_t i, const Key<NW>
     if constexpr (NW == 1) {
         p[i] = k.w[0];
     } else {
-        ulonglong2 v;
-        v.x = k.w[0];
-        v.y = k.w[1];
-        reinterpret_cast<ulonglong2*>(p)[i] = v;
+#pragma unroll
+        for (int h = 0; h < NW / 2; ++h) {
+            ulonglong2 v;
+            v.x = k.w[2 * h];
+            v.y = k.w[2 * h + 1];
+            reinterpret_cast<ulonglong2*>(p)[i * (NW / 2) + h] = v;
+        }
     }
 }
 
@@ -164,6 +170,42 @@ __device__ __forceinline__ bool window_key<2>(const uint32_t* c, const uint32_t*
     return ok;
 }
 
+template <>
+__device__ __forceinline__ bool window_key<4>(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<4>& key)
+{
+    // 288-bit code string c[0]:..:c[8]; S = the 128 bases ending at base i of c[8]
+    const int s = 2 * (15 - i);
+    const uint64_t x0 = ((uint64_t)c[7] << 32) | c[8];
+    const uint64_t x1 = ((uint64_t)c[5] << 32) | c[6];
+    const uint64_t x2 = ((uint64_t)c[3] << 32) | c[4];
+    const uint64_t x3 = ((uint64_t)c[1] << 32) | c[2];
+    const uint64_t x4 = c[0];
+    const uint64_t s3 = s ? ((x0 >> s) | (x1 << (64 - s))) : x0;
+    const uint64_t s2 = s ? ((x1 >> s) | (x2 << (64 - s))) : x1;
+    const uint64_t s1 = s ? ((x2 >> s) | (x3 << (64 - s))) : x2;
+    const uint64_t s0 = s ? ((x3 >> s) | (x4 << (64 - s))) : x3;
+    // 144-bit validity string v[0]:..:v[8], same alignment (one bit per base)
+    const int t = 15 - i;
+    const uint64_t vl = ((uint64_t)v[5] << 48) | ((uint64_t)v[6] << 32) | ((uint64_t)v[7] << 16) | (uint64_t)v[8];
+    const uint64_t vm = ((uint64_t)v[1] << 48) | ((uint64_t)v[2] << 32) | ((uint64_t)v[3] << 16) | (uint64_t)v[4];
+    const uint64_t vh = v[0];
+    const uint64_t wlo = t ? ((vl >> t) | (vm << (64 - t))) : vl;
+    const uint64_t whi = t ? ((vm >> t) | (vh << (64 - t))) : vm;
+    const uint32_t kh = k - 64;  // 1..64
+    const uint64_t hm = kh == 64 ? ~0ULL : ((1ULL << kh) - 1ULL);
+    const bool ok = wlo == ~0ULL && (whi & hm) == hm;
+    // k = 96: the 32 bases before the window, zeroed before the current run
+    uint64_t pre = 0;
+    if (k == 96) {
+        const uint32_t vb = (uint32_t)(whi >> 32);  // bit 0: the base just before the window
+        const uint32_t run = (uint32_t)__ffs(~vb);  // 1 + valid bases before the window, 0 if all 32 are
+        const uint32_t keep = run ? run - 1 : 32;
+        pre = keep >= 32 ? s0 : (s0 & ((1ULL << (2 * keep)) - 1ULL));
+    }
+    key = canon4(s0, s1, s2, s3, k, pre);
+    return ok;
+}
+
 template <int NW, bool SCATTER>
 __global__ void __launch_bounds__(ExtractCfg<NW>::kThreads)
 k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Route rt,
@@ -171,7 +213,7 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
           uint64_t* __restrict__ out)
 {
     constexpr int T = ExtractCfg<NW>::kThreads;
-    constexpr int H = (NW == 1) ? 2 : 4;
+    constexpr int H = (NW == 1) ? 2 : (NW == 2 ? 4 : 8);  // words before a thread's own for a k-base window
     constexpr int TILE = T * 16;
     // The encoded tile and the coarse bounds are dead once every thread has
     // routed its keys (a barrier later), so they share LDS with the staged
@@ -547,7 +589,8 @@ template <int NW>
 __device__ __forceinline__ uint64_t spill_start(const Key<NW>& k)
 {
     uint64_t z = k.w[0];
-    if constexpr (NW == 2) z ^= (k.w[1] >> 7) * 0x9E3779B97F4A7C15ULL;
+#pragma unroll
+    for (int w = 1; w < NW; ++w) z ^= ((k.w[w] >> 7) ^ (uint64_t)w) * 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;  // SplitMix64 finalizer
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
@@ -559,8 +602,9 @@ __device__ __forceinline__ void global_add(const CountArgs& args, const Key<NW>&
 {
     bool claimed;
     const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
-    const int64_t g = table_find_or_claim<NW>(args.spill_keys, args.spill_cap, args.spill_cap - 1, spill_start<NW>(key),
-                                             glimit, key, claimed);
+    constexpr int TW = TableWords<NW>::value;
+    const int64_t g = table_find_or_claim<TW>(args.spill_keys, args.spill_cap, args.spill_cap - 1, spill_start<NW>(key),
+                                             glimit, to_table<NW>(key), claimed);
     if (g >= 0) {
         atomicAdd(&args.spill_cnt[g], add);
         if (claimed) atomicAdd(args.spilled, 1ULL);
@@ -614,7 +658,8 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
     constexpr uint32_t BW = CountCfg<NW>::kBloomWords;
     constexpr uint32_t TS = CountCfg<NW>::kTableSlots;
     __shared__ unsigned long long s_bloom[BW];
-    __shared__ unsigned long long s_tkey[NW * TS];
+    constexpr int TW = TableWords<NW>::value;
+    __shared__ unsigned long long s_tkey[TW * TS];
     __shared__ uint32_t s_tcnt[TS];
     __shared__ uint32_t s_nfirst;
     __shared__ uint32_t s_nrec;
@@ -648,7 +693,7 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
 
     for (;;) {
         for (uint32_t i = t; i < BW; i += T) s_bloom[i] = 0ULL;
-        for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
+        for (uint32_t i = t; i < TW * TS; i += T) s_tkey[i] = kEmpty;
         for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
         // a full spill table voids the run (the host regrows it and runs again):
         // skip the remaining work
@@ -701,8 +746,9 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                         const uint32_t add = run - (first ? 1u : 0u);
                         if (add) {
                             bool claimed;
-                            const int64_t s = table_find_or_claim<NW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
-                                                                     (hh >> 32), CountCfg<NW>::kProbe, key, claimed);
+                            const int64_t s = table_find_or_claim<TW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
+                                                                     (hh >> 32), CountCfg<NW>::kProbe, to_table<NW>(key),
+                                                                     claimed);
                             if (s >= 0) atomicAdd(&s_tcnt[s], add);
                             else global_add<NW>(args, key, add);
                         }
@@ -738,8 +784,12 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
             wbase = __shfl(wbase, 0, kWave);
             if (keep) {
                 Key<NW> key;
+                {
+                    Key<TW> tk;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
+                    for (int w = 0; w < TW; ++w) tk.w[w] = s_tkey[(uint32_t)w * TS + s];
+                    key = from_table<NW>(tk);
+                }
                 const uint64_t pos = wbase + lanes_below(bal);
                 if (pos < rec_cap) write_record<NW>(rec_region, pos, key, cnt);
                 else append_overflow<NW>(args, key, cnt);
@@ -777,7 +827,8 @@ template <int NW>
 __device__ __forceinline__ uint32_t count_hash(const Key<NW>& k)
 {
     uint32_t x = (uint32_t)(k.w[0] >> 32) * 0x9E3779B1u ^ (uint32_t)k.w[0];
-    if constexpr (NW == 2) x = (x ^ (uint32_t)(k.w[1] >> 32)) * 0x85EBCA77u ^ (uint32_t)k.w[1];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) x = (x ^ (uint32_t)(k.w[w] >> 32)) * 0x85EBCA77u ^ (uint32_t)k.w[w];
     x ^= x >> 16;
     x *= 0x7FEB352Du;
     x ^= x >> 15;
@@ -838,7 +889,8 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
     constexpr int T = CountCfg<NW>::kThreads;
     constexpr uint32_t TS = CountCfg<NW>::kExactSlots;
     constexpr int kLoads = CountCfg<NW>::kLoads;
-    __shared__ unsigned long long s_tkey[NW * TS];
+    constexpr int TW = TableWords<NW>::value;
+    __shared__ unsigned long long s_tkey[TW * TS];
     __shared__ uint32_t s_tcnt[TS];
     __shared__ uint32_t s_nfirst, s_nrec, s_dist, s_dpass, s_kept, s_claims, s_abort, s_hstart, s_hend, s_width, s_passes;
 
@@ -867,7 +919,7 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
 
     for (;;) {
 #pragma unroll
-        for (int w = 0; w < NW; ++w)
+        for (int w = 0; w < TW; ++w)
             for (uint32_t i = t; i < ts; i += T) s_tkey[(uint32_t)w * TS + i] = kEmpty;
         for (uint32_t i = t; i < ts; i += T) s_tcnt[i] = 0u;
         if (t == 0) {
@@ -899,8 +951,8 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
                 const uint32_t hb = x >> 16;
                 if (hb < hstart || hb >= hend) continue;
                 bool claimed;
-                const int32_t s = lds_find_or_claim<NW, TS>(s_tkey, ts, ((x & 0xFFFFu) * ts) >> 16,
-                                                              CountCfg<NW>::kProbe, kb[u], claimed);
+                const int32_t s = lds_find_or_claim<TW, TS>(s_tkey, ts, ((x & 0xFFFFu) * ts) >> 16,
+                                                              CountCfg<NW>::kProbe, to_table<NW>(kb[u]), claimed);
                 if (s >= 0) {
                     atomicAdd(&s_tcnt[s], run);
                     if (claimed) {
@@ -937,8 +989,10 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
             if (s < ts) {
                 cnt = s_tcnt[s];
                 if (cnt) {
+                    Key<TW> tk;
 #pragma unroll
-                    for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
+                    for (int w = 0; w < TW; ++w) tk.w[w] = s_tkey[(uint32_t)w * TS + s];
+                    key = from_table<NW>(tk);
                     if ((count_hash<NW>(key) >> 16) >= hend) cnt = 0;  // counted again next pass
                 }
             }
@@ -997,7 +1051,8 @@ k_heavy(const uint64_t* __restrict__ B, CountArgs args)
     constexpr int T = CountCfg<NW>::kThreads;
     constexpr uint32_t TS = CountCfg<NW>::kExactSlots;
     constexpr int kLoads = CountCfg<NW>::kLoads;
-    __shared__ unsigned long long s_tkey[NW * TS];
+    constexpr int TW = TableWords<NW>::value;
+    __shared__ unsigned long long s_tkey[TW * TS];
     __shared__ uint32_t s_tcnt[TS];
     __shared__ unsigned long long s_chunk;
     const int t = threadIdx.x;
@@ -1007,7 +1062,7 @@ k_heavy(const uint64_t* __restrict__ B, CountArgs args)
         __syncthreads();
         const unsigned long long c = s_chunk;
         if (c >= nchunks) break;
-        for (uint32_t i = t; i < NW * TS; i += T) s_tkey[i] = kEmpty;
+        for (uint32_t i = t; i < TW * TS; i += T) s_tkey[i] = kEmpty;
         for (uint32_t i = t; i < TS; i += T) s_tcnt[i] = 0u;
         __syncthreads();
         const unsigned long long lo = args.heavy_list[2 * c], hi = args.heavy_list[2 * c + 1];
@@ -1026,8 +1081,8 @@ k_heavy(const uint64_t* __restrict__ B, CountArgs args)
                 if (!run) continue;
                 const uint32_t x = count_hash<NW>(kb[u]);
                 bool claimed;
-                const int32_t s = lds_find_or_claim<NW, TS>(s_tkey, TS, ((x & 0xFFFFu) * TS) >> 16,
-                                                              CountCfg<NW>::kProbe, kb[u], claimed);
+                const int32_t s = lds_find_or_claim<TW, TS>(s_tkey, TS, ((x & 0xFFFFu) * TS) >> 16,
+                                                              CountCfg<NW>::kProbe, to_table<NW>(kb[u]), claimed);
                 if (s >= 0) atomicAdd(&s_tcnt[s], run);
                 else global_add<NW>(args, kb[u], run);
             }
@@ -1037,8 +1092,12 @@ k_heavy(const uint64_t* __restrict__ B, CountArgs args)
             const uint32_t cnt = s_tcnt[s];
             if (!cnt) continue;
             Key<NW> key;
+            {
+                Key<TW> tk;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) key.w[w] = s_tkey[(uint32_t)w * TS + s];
+                for (int w = 0; w < TW; ++w) tk.w[w] = s_tkey[(uint32_t)w * TS + s];
+                key = from_table<NW>(tk);
+            }
             global_add<NW>(args, key, cnt);
         }
         __syncthreads();  // the table is cleared for the next chunk
@@ -1063,8 +1122,11 @@ k_spill_emit(CountArgs args)
         if (s < args.spill_cap) {
             cnt = args.spill_cnt[s];
             if (cnt) {
+                constexpr int TW = TableWords<NW>::value;
+                Key<TW> tk;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) key.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
+                for (int w = 0; w < TW; ++w) tk.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
+                key = from_table<NW>(tk);
                 keep = cnt >= args.min_count && cnt <= args.max_count;
                 atomicAdd(&s_dist, 1u);
             }
@@ -1401,16 +1463,22 @@ hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hi
 
 template hipError_t launch_extract<1>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_extract<2>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
+template hipError_t launch_extract<4>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_sub<1>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_sub<2>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
+template hipError_t launch_sub<4>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_count<1>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_count<2>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<4>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_heavy<1>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
 template hipError_t launch_heavy<2>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
+template hipError_t launch_heavy<4>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<1>(const CountArgs&, hipStream_t);
 template hipError_t launch_spill_emit<2>(const CountArgs&, hipStream_t);
+template hipError_t launch_spill_emit<4>(const CountArgs&, hipStream_t);
 template hipError_t launch_finalize<1>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
 template hipError_t launch_finalize<2>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
+template hipError_t launch_finalize<4>(const SegArgs&, const CountArgs&, const uint64_t*, uint64_t*, const uint8_t*, uint8_t*, hipStream_t);
 // ------------------------------------------------------------------ histogram
 
 // Count histogram of the resident results, histogram_tool's two modes
@@ -1453,7 +1521,8 @@ k_hist_records(const uint8_t* __restrict__ recs, uint64_t nrec, HistArgs a)
 #pragma unroll
         for (int w = 0; w < NW; ++w) key.w[w] = r[w];
         bool claimed;
-        const int64_t s = table_find_or_claim<NW>(a.keys, a.cap, a.cap - 1, spill_start<NW>(key), kHistProbe, key, claimed);
+        const int64_t s = table_find_or_claim<TableWords<NW>::value>(a.keys, a.cap, a.cap - 1, spill_start<NW>(key),
+                                                                    kHistProbe, to_table<NW>(key), claimed);
         if (s >= 0) a.vals[s] = c;
         else atomicOr(a.error_flags, kErrHistFull);
     }
@@ -1472,16 +1541,18 @@ k_hist_firsts(const uint64_t* __restrict__ firsts, uint64_t nfirst, HistArgs a)
     for (uint32_t b = threadIdx.x; b < kHistLds; b += 256) s_h[b] = 0;
     __syncthreads();
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nfirst; i += (uint64_t)gridDim.x * 256) {
+        constexpr int TW = TableWords<NW>::value;
         const Key<NW> key = load_key<NW>(firsts, i);
+        const Key<TW> tkey = to_table<NW>(key);
         // probe until the key or an empty slot (the table is at most half full)
         uint64_t s = spill_start<NW>(key) & (a.cap - 1);
         bool found = false;
         for (uint32_t p = 0; p < kHistProbe; ++p) {
             const unsigned long long w0 = a.keys[s];
             if (w0 == kEmpty) break;
-            bool match = w0 == key.w[0];
+            bool match = w0 == tkey.w[0];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) match = match && a.keys[(uint64_t)w * a.cap + s] == key.w[w];
+            for (int w = 1; w < TW; ++w) match = match && a.keys[(uint64_t)w * a.cap + s] == tkey.w[w];
             if (match) { found = true; break; }
             s = (s + 1) & (a.cap - 1);
         }
@@ -1512,6 +1583,7 @@ hipError_t launch_histogram(const uint8_t* recs, uint64_t nrec, const uint64_t* 
 
 template hipError_t launch_histogram<1>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
 template hipError_t launch_histogram<2>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
+template hipError_t launch_histogram<4>(const uint8_t*, uint64_t, const uint64_t*, uint64_t, const HistArgs&, hipStream_t);
 
 // Fills used instead of memset nodes in the captured pipeline graph.
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
@@ -1563,5 +1635,6 @@ hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream
 
 template hipError_t launch_debug_hash<1>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template hipError_t launch_debug_hash<2>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
+template hipError_t launch_debug_hash<4>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 
 }  // namespace kmc

@@ -52,12 +52,33 @@ void bloom_capacity(const uint64_t* counts, uint64_t est_file_size, uint64_t* ca
     }
 }
 
+namespace {
+
+uint32_t key_words(uint32_t k) { return k <= 32 ? 1u : k <= 64 ? 2u : 4u; }
+
+uint64_t extract_tile(uint32_t nw)
+{
+    return (nw == 1 ? ExtractCfg<1>::kThreads : nw == 2 ? ExtractCfg<2>::kThreads : ExtractCfg<4>::kThreads) * 16ULL;
+}
+
+hipError_t extract(uint32_t nw, bool scatter, const uint8_t* seq, uint64_t padded, uint32_t k, const Route& rt,
+                   unsigned long long* counts, unsigned long long* cursor, uint64_t* out, hipStream_t st)
+{
+    switch (nw) {
+    case 1: return launch_extract<1>(scatter, seq, padded, k, rt, counts, cursor, out, st);
+    case 2: return launch_extract<2>(scatter, seq, padded, k, rt, counts, cursor, out, st);
+    default: return launch_extract<4>(scatter, seq, padded, k, rt, counts, cursor, out, st);
+    }
+}
+
+}  // namespace
+
 int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sample, hipStream_t st,
                 PlanResult& out, std::string& err)
 {
     out = PlanResult();
-    const uint32_t nw = k <= 32 ? 1u : 2u;
-    const uint64_t tile = (nw == 1 ? ExtractCfg<1>::kThreads : ExtractCfg<2>::kThreads) * 16ULL;
+    const uint32_t nw = key_words(k);
+    const uint64_t tile = extract_tile(nw);
     const uint64_t padded = ((sample.size() + tile - 1) / tile) * tile;
     if (padded) {
         uint8_t* d_seq = nullptr;
@@ -72,10 +93,8 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
         if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
         if (e == hipSuccess) e = hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * kXcds * 256, st);
         Route rt{};
-    rt.rhi = 256;  // every coarse bucket
-        if (e == hipSuccess)
-            e = nw == 1 ? launch_extract<1>(false, d_seq, padded, k, rt, d_counts, nullptr, nullptr, st)
-                        : launch_extract<2>(false, d_seq, padded, k, rt, d_counts, nullptr, nullptr, st);
+        rt.rhi = 256;  // every coarse bucket
+        if (e == hipSuccess) e = extract(nw, false, d_seq, padded, k, rt, d_counts, nullptr, nullptr, st);
         std::vector<unsigned long long> h(kXcds * 256);
         if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_counts, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -118,8 +137,8 @@ double imbalance(const std::vector<uint64_t>& sorted, const std::vector<uint64_t
 int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, PlanResult& out, std::string& err)
 {
     const auto t0 = std::chrono::steady_clock::now();
-    const uint32_t nw = k <= 32 ? 1u : 2u;
-    const uint64_t tile = (nw == 1 ? ExtractCfg<1>::kThreads : ExtractCfg<2>::kThreads) * 16ULL;
+    const uint32_t nw = key_words(k);
+    const uint64_t tile = extract_tile(nw);
     const uint64_t padded = ((sample.size() + tile - 1) / tile) * tile;
     if (padded == 0) { err = "learned plan: empty Stage-0 sample"; return KC_ERR_STATE; }
     // sample keys on the device (root-grouped, the order does not matter here)
@@ -142,13 +161,9 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     hipError_t e = hipMemcpyAsync(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
     if (e == hipSuccess) e = hipMemsetAsync(d_small, 0, sizeof(unsigned long long) * (16 * 256 + 600), st);
-    if (e == hipSuccess)
-        e = nw == 1 ? launch_extract<1>(false, d_seq, padded, k, rt, counts, cursor, nullptr, st)
-                    : launch_extract<2>(false, d_seq, padded, k, rt, counts, cursor, nullptr, st);
+    if (e == hipSuccess) e = extract(nw, false, d_seq, padded, k, rt, counts, cursor, nullptr, st);
     if (e == hipSuccess) e = launch_scan_coarse(counts, coff, cursor, tstart, 4096, st);
-    if (e == hipSuccess)
-        e = nw == 1 ? launch_extract<1>(true, d_seq, padded, k, rt, counts, cursor, d_keys, st)
-                    : launch_extract<2>(true, d_seq, padded, k, rt, counts, cursor, d_keys, st);
+    if (e == hipSuccess) e = extract(nw, true, d_seq, padded, k, rt, counts, cursor, d_keys, st);
     unsigned long long nkeys = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&nkeys, coff + 256, sizeof(nkeys), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -43,8 +43,8 @@ int main(int argc, char** argv)
         return -1;
     }
     if (map_cap <= 1 || map_cap >= 16ULL * 1024 * 1024 || max_c < min_c || max_c > 0xFFFFFFFFULL) return usage(argv[0]);
-    if (k == 0 || k > 63) {
-        std::fprintf(stderr, "k_len must be in [1, 63] on the GPU engine\n");
+    if (k == 0 || k > 128) {  // src/main.cpp:579-595
+        std::fprintf(stderr, "k_len must be <= 128\n");
         return 1;
     }
     (void)mem;  // device memory is sized from the input, not from this budget

@@ -89,6 +89,19 @@ def extract(seq: bytes, k: int) -> np.ndarray:
     return out[: n * nw].reshape(n, nw)
 
 
+def extract_blocks(blocks: list[bytes], k: int) -> np.ndarray:
+    """Oracle keys of reader blocks, each parsed from a cleared state, shape (n, N)."""
+    O = oracle()
+    nw = nwords(k)
+    data = b"".join(blocks)
+    lens = np.array([len(b) for b in blocks] or [0], dtype=np.uint64)
+    cap = max(1, len(data))
+    out = np.zeros(cap * nw, dtype=np.uint64)
+    n = O.orc_extract_blocks(data, lens.ctypes.data, len(blocks), k, out.ctypes.data, cap)
+    assert n != UMAX
+    return out[: n * nw].reshape(n, nw)
+
+
 def count_table(keys: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """Oracle exact table: sorted distinct keys and their counts."""
     O = oracle()

@@ -39,7 +39,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_words_for_k_boundary():
     L = K.lib()
-    assert [L.kc_words_for_k(k) for k in (0, 1, 32, 33, 63, 64, 65)] == [0, 1, 1, 2, 2, 0, 0]
+    # src/main.cpp:579-595: N = 1 up to k 32, 2 up to 64, 4 up to 128
+    ks = (0, 1, 32, 33, 63, 64, 65, 96, 128, 129)
+    assert [L.kc_words_for_k(k) for k in ks] == [0, 1, 1, 2, 2, 2, 4, 4, 4, 0]
+    assert [K.words_for_k(k) for k in ks[1:-1]] == [1, 1, 2, 2, 2, 4, 4, 4]
+    for bad in (0, 129):
+        with pytest.raises(ValueError):
+            K.words_for_k(bad)
 
 
 def oracle_blocks(fq: bytes, k: int, bs: int) -> list[bytes]:
@@ -107,7 +113,7 @@ def test_file_reader_plain_and_gzip(tmp_path):
         assert blocks == K.fastq_blocks(fq, 31)
 
 
-@pytest.mark.parametrize("k", [17, 31, 32, 33, 63])
+@pytest.mark.parametrize("k", [17, 31, 32, 33, 63, 64, 65, 96, 100, 128])
 def test_pack_low_matches_oracle(k):
     rng = np.random.default_rng(k)
     nw = C.nwords(k)
@@ -145,7 +151,7 @@ def test_tree_cli_rejects_bad_arguments(tmp_path):
     if not exe.exists():
         pytest.skip("kmc_tree not built")
     bad = [[], ["x.fq", "31", "4", "8"], ["x.fq", "31", "8", "8", "1"], ["x.fq", "31", "8", "8", "1024", "5", "2"],
-           ["x.fq", "0", "8", "8"], ["x.fq", "64", "8", "8"]]
+           ["x.fq", "0", "8", "8"], ["x.fq", "129", "8", "8"]]
     for args in bad:
         r = subprocess.run([str(exe), *args], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode != 0, args
