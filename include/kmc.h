@@ -33,10 +33,12 @@
  * them at once.  Emit callbacks lend buffers that are valid only during the
  * call.  One host thread drives a handle; handles are independent.
  *
- * Key layout: N = 1 (k <= 32) or 2 (33 <= k <= 63) little-endian u64 words per
- * key, word 0 most significant, 2 bits per base MSB-first, left aligned
- * (src/kmer.h:12-21).  Canonical key = min(pack(reverse(w)), pack(complement(w)))
- * for a window w, as GetKmer computes it (src/GetKmer.h:58-129).
+ * Key layout: N = 1 (k <= 32), 2 (33 <= k <= 64) or 4 (65 <= k <= 128)
+ * little-endian u64 words per key (src/main.cpp:579-595), word 0 most
+ * significant, 2 bits per base MSB-first, left aligned (src/kmer.h:12-21).
+ * Canonical key = min(pack(reverse(w)), pack(complement(w))) for a window w,
+ * as GetKmer computes it (src/GetKmer.h:58-129); at k = 96 word 3 holds the 32
+ * bases before the window, as GetKmer leaves it.
  */
 #ifndef KMC_H
 #define KMC_H
@@ -83,7 +85,7 @@ typedef enum kc_counter {
 } kc_counter;
 
 typedef struct kc_config {
-    uint32_t k;               /* k-mer length, 1..63 */
+    uint32_t k;               /* k-mer length, 1..128 */
     uint32_t partition;       /* kc_partition */
     uint32_t min_count;       /* root record filter on the repeat count, inclusive */
     uint32_t max_count;       /*   (append_export_record, src/NewKmerTree.h:1159-1168); 0 = UINT32_MAX */
@@ -202,7 +204,7 @@ typedef struct kc_tune_result {
 } kc_tune_result;
 KC_API int kc_autotune(kc_engine* h, kc_tune_result* out);
 
-/* Number of u64 words per key for k (1 or 2), 0 if unsupported. */
+/* Number of u64 words per key for k (1, 2 or 4), 0 if k is 0 or above 128. */
 KC_API uint32_t kc_words_for_k(uint32_t k);
 
 /* Human-readable text of the last error on this handle (never NULL). */
