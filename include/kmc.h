@@ -210,11 +210,6 @@ KC_API uint32_t kc_words_for_k(uint32_t k);
 /* Human-readable text of the last error on this handle (never NULL). */
 KC_API const char* kc_last_error(kc_engine* h);
 
-/* Test hook, no reference counterpart: the device XXH3-128 of n keys of nw
- * words (the Bloom filter's hash, src/BloomFilter.h:209); out gets n (low64,
- * high64) pairs.  Lets tests pin the device hash against the host one. */
-KC_API int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out);
-
 /* kc_debug_copy: one plain device copy of `bytes` (a multiple of 16) with a
  * 16-byte-per-lane kernel; *ms = its device time.  Reads `bytes` and writes
  * `bytes` of HBM: the known-size reference for hardware traffic counters. */

@@ -110,7 +110,7 @@ BLOCK_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes
 EXPORTED = [
     "kc_create", "kc_sample_block", "kc_plan", "kc_push_block", "kc_finish", "kc_emit_firsts",
     "kc_emit_root", "kc_stats_get", "kc_destroy", "kc_count_resident", "kc_words_for_k",
-    "kc_last_error", "kc_debug_xxh3", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
+    "kc_last_error", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
     "kmc_write_histogram", "kc_autotune",
@@ -147,7 +147,6 @@ def lib() -> ctypes.CDLL:
     L.kc_words_for_k.restype = u32
     L.kc_last_error.argtypes = [vp]
     L.kc_last_error.restype = cp
-    L.kc_debug_xxh3.argtypes = [vp, u64, u32, vp]
     L.kc_debug_copy.argtypes = [u64, u32, ctypes.POINTER(ctypes.c_float)]
     L.kc_histogram.argtypes = [vp, u32, u32, u32, vp]
     L.kc_autotune.argtypes = [vp, vp]
@@ -352,16 +351,6 @@ def debug_copy(nbytes: int, device: int = 0) -> float:
     if rc != 0:
         raise KmcError(f"kc_debug_copy failed: {rc}")
     return ms.value
-
-
-def debug_xxh3(keys: np.ndarray, nw: int) -> np.ndarray:
-    keys = np.ascontiguousarray(keys, dtype=np.uint64)
-    n = keys.size // nw
-    out = np.zeros(2 * n, dtype=np.uint64)
-    rc = lib().kc_debug_xxh3(keys.ctypes.data, n, nw, out.ctypes.data)
-    if rc != 0:
-        raise KmcError(f"kc_debug_xxh3 failed: {rc}")
-    return out.reshape(n, 2)
 
 
 MLP_NPARAM = 64 * 64 + 64 + 64 * 64 + 64 + 4096 * 64 + 4096

@@ -144,82 +144,6 @@ __device__ __forceinline__ bool key_eq(const Key<NW>& a, const Key<NW>& b)
     return e;
 }
 
-// ---------------------------------------------------------------- XXH3-128
-// Restatement of XXH3_128bits for 8- and 16-byte inputs with seed 0 and the
-// default secret (include/xxhash.h v0.8.3: XXH3_len_4to8_128b :6900,
-// XXH3_len_9to16_128b :6927, XXH3_avalanche :4616).  The constants below are
-// the secret words those paths read: XOR of secret bytes 16..23/24..31,
-// 32..39/40..47 and 48..55/56..63, read little-endian.
-constexpr uint64_t kXxP64_1 = 0x9E3779B185EBCA87ULL;
-constexpr uint64_t kXxP64_2 = 0xC2B2AE3D27D4EB4FULL;
-constexpr uint64_t kXxP32_2 = 0x85EBCA77ULL;
-constexpr uint64_t kXxMx1 = 0x165667919E3779F9ULL;
-constexpr uint64_t kXxMx2 = 0x9FB21C651E98DF25ULL;
-constexpr uint64_t kXxFlip8 = 0xdb979083e96dd4deULL ^ 0x1f67b3b7a4a44072ULL;
-constexpr uint64_t kXxFlip16L = 0x78e5c0cc4ee679cbULL ^ 0x2172ffcc7dd05a82ULL;
-constexpr uint64_t kXxFlip16H = 0x8e2443f7744608b8ULL ^ 0x4c263a81e69035e0ULL;
-
-__device__ __forceinline__ uint64_t xx_avalanche(uint64_t h)
-{
-    h ^= h >> 37;
-    h *= kXxMx1;
-    return h ^ (h >> 32);
-}
-
-__device__ __forceinline__ void xxh3_128(const Key<1>& key, uint64_t& lo, uint64_t& hi)
-{
-    const uint64_t keyed = key.w[0] ^ kXxFlip8;
-    const uint64_t m = kXxP64_1 + (8ULL << 2);
-    uint64_t l = keyed * m;
-    uint64_t h = __umul64hi(keyed, m);
-    h += l << 1;
-    l ^= h >> 3;
-    l ^= l >> 35;
-    l *= kXxMx2;
-    l ^= l >> 28;
-    lo = l;
-    hi = xx_avalanche(h);
-}
-
-__device__ __forceinline__ void xxh3_128(const Key<2>& key, uint64_t& lo, uint64_t& hi)
-{
-    const uint64_t in_lo = key.w[0];
-    uint64_t in_hi = key.w[1];
-    const uint64_t a = in_lo ^ in_hi ^ kXxFlip16L;
-    uint64_t ml = a * kXxP64_1;
-    uint64_t mh = __umul64hi(a, kXxP64_1);
-    ml += 15ULL << 54;
-    in_hi ^= kXxFlip16H;
-    mh += in_hi + (uint64_t)(uint32_t)in_hi * (kXxP32_2 - 1);
-    ml ^= __builtin_bswap64(mh);
-    uint64_t hl = ml * kXxP64_2;
-    uint64_t hh = __umul64hi(ml, kXxP64_2);
-    hh += mh * kXxP64_2;
-    lo = xx_avalanche(hl);
-    hi = xx_avalanche(hh);
-}
-
-// 32 bytes (XXH3_len_17to128_128b: one XXH128_mix32B, :6999); kXxSecret0..3
-// are secret bytes 0..31 read little-endian.
-constexpr uint64_t kXxP64_4 = 0x85EBCA77C2B2AE63ULL;
-constexpr uint64_t kXxSecret0 = 0xbe4ba423396cfeb8ULL;
-constexpr uint64_t kXxSecret1 = 0x1cad21f72c81017cULL;
-constexpr uint64_t kXxSecret2 = 0xdb979083e96dd4deULL;
-constexpr uint64_t kXxSecret3 = 0x1f67b3b7a4a44072ULL;
-
-__device__ __forceinline__ uint64_t xx_fold(uint64_t a, uint64_t b) { return (a * b) ^ __umul64hi(a, b); }
-
-__device__ __forceinline__ void xxh3_128(const Key<4>& key, uint64_t& lo, uint64_t& hi)
-{
-    uint64_t alo = 32ULL * kXxP64_1, ahi = 0;
-    alo += xx_fold(key.w[0] ^ kXxSecret0, key.w[1] ^ kXxSecret1);
-    alo ^= key.w[2] + key.w[3];
-    ahi += xx_fold(key.w[2] ^ kXxSecret2, key.w[3] ^ kXxSecret3);
-    ahi ^= key.w[0] + key.w[1];
-    lo = xx_avalanche(alo + ahi);
-    hi = 0ULL - xx_avalanche(alo * kXxP64_1 + ahi * kXxP64_4 + 32ULL * kXxP64_2);
-}
-
 // ---------------------------------------------------------------- table keys
 // Open-addressing tables mark a free word with kEmpty (all ones), so a stored
 // word must never be all ones.  k <= 32: a canonical key word never is.
@@ -269,8 +193,9 @@ __device__ __forceinline__ Key<NW> from_table(const Key<TableWords<NW>::value>& 
     }
 }
 
-// Bloom probe of ConcurrentBloomFilter (src/BloomFilter.h:207-214, :248-258):
-// word = low64 & (cap-1); three bits (low64 + i*(high64|1)) & 63.
+// Bloom probe rule of ConcurrentBloomFilter (src/BloomFilter.h:207-214,
+// :248-258): word = low64 & (cap-1); three bits (low64 + i*(high64|1)) & 63.
+// The counting kernels feed it two words of their own 64-bit key hash.
 __device__ __forceinline__ uint64_t bloom_mask(uint64_t lo, uint64_t hi)
 {
     const uint64_t h2 = hi | 1ULL;

@@ -1076,28 +1076,6 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     return KC_OK;
 }
 
-int kc_debug_xxh3(const uint64_t* keys, uint64_t n, uint32_t nw, uint64_t* out)
-{
-    if ((!keys || !out) && n) return KC_ERR_ARG;
-    if (nw != 1 && nw != 2 && nw != 4) return KC_ERR_ARG;
-    if (!n) return KC_OK;
-    uint64_t *dk = nullptr, *dout = nullptr;
-    if (hipMalloc(&dk, n * 8 * nw) != hipSuccess) return KC_ERR_NOMEM;
-    if (hipMalloc(&dout, n * 16) != hipSuccess) { hipFree(dk); return KC_ERR_NOMEM; }
-    int rc = KC_OK;
-    if (hipMemcpy(dk, keys, n * 8 * nw, hipMemcpyHostToDevice) != hipSuccess) rc = KC_ERR_HIP;
-    if (!rc) {
-        const hipError_t e = nw == 1   ? launch_debug_hash<1>(dk, n, dout, nullptr)
-                             : nw == 2 ? launch_debug_hash<2>(dk, n, dout, nullptr)
-                                       : launch_debug_hash<4>(dk, n, dout, nullptr);
-        if (e != hipSuccess) rc = KC_ERR_HIP;
-    }
-    if (!rc && hipMemcpy(out, dout, n * 16, hipMemcpyDeviceToHost) != hipSuccess) rc = KC_ERR_HIP;
-    hipFree(dk);
-    hipFree(dout);
-    return rc;
-}
-
 int kc_debug_copy(uint64_t bytes, uint32_t device, float* ms)
 {
     if (bytes == 0 || bytes % 16) return KC_ERR_ARG;

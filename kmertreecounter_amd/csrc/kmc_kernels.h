@@ -148,8 +148,6 @@ hipError_t launch_spill_emit(const CountArgs& args, hipStream_t st);
 template <int NW>
 hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_t* firsts_in, uint64_t* firsts_out,
                            const uint8_t* rec_in, uint8_t* rec_out, hipStream_t st);
-template <int NW>
-hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t st);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream_t st);
 // bytes of p set to v by kernels (the captured pipeline graph has no memset nodes)
 hipError_t launch_fill(void* p, uint64_t bytes, uint8_t v, hipStream_t st);

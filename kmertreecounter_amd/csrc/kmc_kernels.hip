@@ -582,16 +582,18 @@ __device__ __forceinline__ void append_overflow(const CountArgs& a, const Key<NW
     else atomicOr(a.error_flags, (unsigned long long)kErrRecFull);
 }
 
-// First probe slot of a key in the global (spill) table.  Every kernel that
-// inserts into that table uses this one function, so a key always walks the
-// same probe sequence.
+// 64-bit hash of a key: the key words folded into one, then the SplitMix64
+// finalizer.  The global tables start their probes at its low bits (every
+// kernel that inserts into a table uses this one function, so a key always
+// walks the same probe sequence); the Bloom counter takes its filter word and
+// bits, its table slot and its pass range from it.
 template <int NW>
-__device__ __forceinline__ uint64_t spill_start(const Key<NW>& k)
+__device__ __forceinline__ uint64_t key_mix(const Key<NW>& k)
 {
     uint64_t z = k.w[0];
 #pragma unroll
     for (int w = 1; w < NW; ++w) z ^= ((k.w[w] >> 7) ^ (uint64_t)w) * 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;  // SplitMix64 finalizer
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
@@ -603,7 +605,7 @@ __device__ __forceinline__ void global_add(const CountArgs& args, const Key<NW>&
     bool claimed;
     const uint32_t glimit = args.spill_cap < kSpillProbe ? (uint32_t)args.spill_cap : kSpillProbe;
     constexpr int TW = TableWords<NW>::value;
-    const int64_t g = table_find_or_claim<TW>(args.spill_keys, args.spill_cap, args.spill_cap - 1, spill_start<NW>(key),
+    const int64_t g = table_find_or_claim<TW>(args.spill_keys, args.spill_cap, args.spill_cap - 1, key_mix<NW>(key),
                                              glimit, to_table<NW>(key), claimed);
     if (g >= 0) {
         atomicAdd(&args.spill_cnt[g], add);
@@ -733,21 +735,22 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                 // repeats: the head lane tests the filter once and counts the rest
                 const uint32_t run = run_head_length<NW>(key, i < hi);
                 if (run) {
-                    uint64_t hl, hh;
-                    xxh3_128(key, hl, hh);
-                    const uint32_t hb = (uint32_t)(hh >> 48);
+                    // hash bits: 0..17 filter word and bits, 32..44 table slot, 48..63 pass range
+                    const uint64_t h = key_mix<NW>(key);
+                    const uint32_t hb = (uint32_t)(h >> 48);
                     if (hb >= hstart && hb < hend) {
-                        const uint32_t word = (uint32_t)(hl & (BW - 1));
-                        const uint64_t mask = bloom_mask(hl, hh);
-                        // relaxed pre-check, then fetch_or (insert_prepared, src/BloomFilter.h:225-234)
-                        unsigned long long old = s_bloom[word];
-                        if ((old & mask) != mask) old = atomicOr(&s_bloom[word], (unsigned long long)mask);
+                        const uint32_t word = (uint32_t)(h & (BW - 1));
+                        const uint64_t mask = bloom_mask(h, h >> 12);
+                        // test-and-set of the key's bits (src/BloomFilter.h:225-234); the
+                        // reference's relaxed pre-read is left out: it costs more than the
+                        // atomics it saves here
+                        const unsigned long long old = atomicOr(&s_bloom[word], (unsigned long long)mask);
                         first = (old & mask) != mask;
                         const uint32_t add = run - (first ? 1u : 0u);
                         if (add) {
                             bool claimed;
                             const int64_t s = table_find_or_claim<TW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
-                                                                     (hh >> 32), CountCfg<NW>::kProbe, to_table<NW>(key),
+                                                                     (h >> 32), CountCfg<NW>::kProbe, to_table<NW>(key),
                                                                      claimed);
                             if (s >= 0) atomicAdd(&s_tcnt[s], add);
                             else global_add<NW>(args, key, add);
@@ -1340,20 +1343,6 @@ k_ovf_scatter(const uint8_t* __restrict__ ovf, const unsigned long long* __restr
     }
 }
 
-// ------------------------------------------------------------------ debug hooks
-
-template <int NW>
-__global__ void k_debug_hash(const uint64_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ out)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Key<NW> key = load_key<NW>(keys, i);
-    uint64_t lo, hi;
-    xxh3_128(key, lo, hi);
-    out[2 * i] = lo;
-    out[2 * i + 1] = hi;
-}
-
 // ------------------------------------------------------------------ launchers
 
 #define KMC_LAUNCH_CHECK() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return e_; } while (0)
@@ -1452,15 +1441,6 @@ hipError_t launch_finalize(const SegArgs& sa, const CountArgs& ca, const uint64_
     return hipSuccess;
 }
 
-template <int NW>
-hipError_t launch_debug_hash(const uint64_t* keys, uint64_t n, uint64_t* out, hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_debug_hash<NW>), dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, keys, n, out);
-    KMC_LAUNCH_CHECK();
-    return hipSuccess;
-}
-
 template hipError_t launch_extract<1>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_extract<2>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_extract<4>(bool, const uint8_t*, uint64_t, uint32_t, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
@@ -1521,7 +1501,7 @@ k_hist_records(const uint8_t* __restrict__ recs, uint64_t nrec, HistArgs a)
 #pragma unroll
         for (int w = 0; w < NW; ++w) key.w[w] = r[w];
         bool claimed;
-        const int64_t s = table_find_or_claim<TableWords<NW>::value>(a.keys, a.cap, a.cap - 1, spill_start<NW>(key),
+        const int64_t s = table_find_or_claim<TableWords<NW>::value>(a.keys, a.cap, a.cap - 1, key_mix<NW>(key),
                                                                     kHistProbe, to_table<NW>(key), claimed);
         if (s >= 0) a.vals[s] = c;
         else atomicOr(a.error_flags, kErrHistFull);
@@ -1545,7 +1525,7 @@ k_hist_firsts(const uint64_t* __restrict__ firsts, uint64_t nfirst, HistArgs a)
         const Key<NW> key = load_key<NW>(firsts, i);
         const Key<TW> tkey = to_table<NW>(key);
         // probe until the key or an empty slot (the table is at most half full)
-        uint64_t s = spill_start<NW>(key) & (a.cap - 1);
+        uint64_t s = key_mix<NW>(key) & (a.cap - 1);
         bool found = false;
         for (uint32_t p = 0; p < kHistProbe; ++p) {
             const unsigned long long w0 = a.keys[s];
@@ -1633,8 +1613,5 @@ hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream
     return hipSuccess;
 }
 
-template hipError_t launch_debug_hash<1>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
-template hipError_t launch_debug_hash<2>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
-template hipError_t launch_debug_hash<4>(const uint64_t*, uint64_t, uint64_t*, hipStream_t);
 
 }  // namespace kmc

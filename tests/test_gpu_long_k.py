@@ -40,23 +40,6 @@ def all_ones_reads(k: int, copies: int) -> bytes:
     return b"\n".join(out * copies) + b"\n"
 
 
-def test_device_xxh3_matches_oracle_on_32_byte_keys():
-    rng = np.random.default_rng(11)
-    O = C.oracle()
-    keys = rng.integers(0, 2**63, size=(4096, 4), dtype=np.uint64) * 2 + 1
-    keys[:3] = 0
-    keys[3:6] = UMAX
-    keys[6] = (UMAX, 0, UMAX, 0)
-    got = K.debug_xxh3(keys, 4)
-    exp = np.zeros((keys.shape[0], 2), dtype=np.uint64)
-    for i in range(keys.shape[0]):
-        lo, hi = C.ctypes.c_uint64(), C.ctypes.c_uint64()
-        kk = np.ascontiguousarray(keys[i])
-        O.orc_xxh3_128(kk.ctypes.data, 4, C.ctypes.byref(lo), C.ctypes.byref(hi))
-        exp[i] = (lo.value, hi.value)
-    np.testing.assert_array_equal(got, exp)
-
-
 @COUNTERS
 @pytest.mark.parametrize("k", LONG_K)
 def test_long_k_random_stream(k, counter):

@@ -64,22 +64,6 @@ def check_outputs(firsts, recs, stats, k, exp_keys, exp_counts, exact=False):
         np.testing.assert_array_equal(recs["count"], np.where(in_firsts, c - 1, c))
 
 
-def test_device_xxh3_matches_oracle():
-    rng = np.random.default_rng(5)
-    O = C.oracle()
-    for nw in (1, 2):
-        keys = rng.integers(0, 2**63, size=(4096, nw), dtype=np.uint64) * 2 + 1
-        keys[:3] = 0
-        got = K.debug_xxh3(keys, nw)
-        exp = np.zeros((keys.shape[0], 2), dtype=np.uint64)
-        for i in range(keys.shape[0]):
-            lo, hi = C.ctypes.c_uint64(), C.ctypes.c_uint64()
-            kk = np.ascontiguousarray(keys[i])
-            O.orc_xxh3_128(kk.ctypes.data, nw, C.ctypes.byref(lo), C.ctypes.byref(hi))
-            exp[i] = (lo.value, hi.value)
-        np.testing.assert_array_equal(got, exp)
-
-
 @COUNTERS
 @pytest.mark.parametrize("k", [31, 17, 32, 33, 63, 5])
 def test_small_random_stream(k, counter):
