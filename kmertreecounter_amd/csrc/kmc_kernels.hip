@@ -523,6 +523,7 @@ __device__ __forceinline__ uint32_t run_head_length(const Key<NW>& key, bool val
         same = same && plo == lo && phi == hi;
     }
     const uint64_t heads = __ballot(!same);  // invalid lanes head empty runs, so no run crosses them
+    if (heads == ~0ULL) return valid ? 1u : 0u;  // no repeated neighbours in the wave (the usual case)
     if (!valid || same) return 0;
     const uint64_t above = lane == 63 ? 0ULL : (heads >> (lane + 1));
     return above ? (uint32_t)__ffsll((long long)above) : 64u - lane;
