@@ -236,6 +236,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         r["learned"] = {
             "plan_ms": plan_ms,
             "train_keys": st.learned_train_keys,
+            "split_buckets": st.learned_split_buckets,
             "train_ms": st.learned_train_ms,
             "loss_first": st.learned_loss_first,
             "loss_last": st.learned_loss_last,

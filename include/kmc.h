@@ -151,6 +151,7 @@ typedef struct kc_stats {
     uint64_t extra_passes;              /* counting passes beyond one per bucket, summed over buckets */
     uint64_t heavy_buckets;             /* buckets counted in chunks by the whole GPU */
     uint64_t rounds;                    /* bucket-range rounds of the last run (1 = one pass) */
+    uint64_t learned_split_buckets;     /* learned buckets the network left empty, refilled from a merged neighbour */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;

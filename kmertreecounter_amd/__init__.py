@@ -75,6 +75,7 @@ class Stats(ctypes.Structure):
         ("learned_sample_distinct_max_over_mean", ctypes.c_double),
         ("prefix_sample_distinct_max_over_mean", ctypes.c_double), ("extra_passes", ctypes.c_uint64),
         ("heavy_buckets", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+        ("learned_split_buckets", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

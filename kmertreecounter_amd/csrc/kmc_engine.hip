@@ -217,7 +217,8 @@ void free_work(kc_engine* h)
 // small arrays in d_small (unsigned long long units)
 constexpr uint64_t kSmCounts = 0, kSmCursor = 8 * 256, kSmCoarseOff = 16 * 256, kSmTileStart = kSmCoarseOff + 257,
                    kSmRootOvf = kSmTileStart + 257, kSmRootMain = kSmRootOvf + 256, kSmRootOff = kSmRootMain + 256,
-                   kSmOvfOff = kSmRootOff + 257, kSmOvfCursor = kSmOvfOff + 257, kSmTotal = kSmOvfCursor + 256;
+                   kSmOvfOff = kSmRootOff + 257, kSmOvfCursor = kSmOvfOff + 257, kSmScanPart = kSmOvfCursor + 256,
+                   kSmTotal = kSmScanPart + kScanGroups;
 
 // Buckets above heavy_keys keys are counted in chunks by the whole GPU.
 uint64_t heavy_keys(const kc_engine* h) { return h->cfg.heavy_keys ? h->cfg.heavy_keys : (1ULL << 20); }
@@ -482,7 +483,7 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
                 HIPX(mark(4));
                 HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
                 HIPX(mark(5));
-                HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, st));
+                HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, h->d_small + kSmScanPart, st));
                 HIPX(mark(6));
                 HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
                 HIPX(mark(7));
@@ -1057,6 +1058,7 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->ms_h2d = h->h2d_ms;
     if (h->planned) {
         s->learned_train_keys = h->plan.train_keys;
+        s->learned_split_buckets = h->plan.split_buckets;
         s->learned_sample_distinct_max_over_mean = h->plan.sample_distinct_learned;
         s->prefix_sample_distinct_max_over_mean = h->plan.sample_distinct_prefix;
         s->learned_train_ms = h->plan.train_ms;

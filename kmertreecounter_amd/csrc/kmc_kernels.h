@@ -132,8 +132,10 @@ hipError_t launch_extract(bool scatter, const uint8_t* seq, uint64_t padded_len,
 hipError_t launch_scan_coarse(const unsigned long long* counts, unsigned long long* coarse_off, unsigned long long* cursor,
                               unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st,
                               uint32_t rlo = 0, uint32_t rhi = 256);
+// fine bucket offsets and cursors; part: kScanGroups scratch words
+constexpr uint32_t kScanGroups = 64;  // up to 65536 fine buckets
 hipError_t launch_scan_fine(const unsigned long long* counts, uint32_t nf, unsigned long long* fine_off,
-                            unsigned long long* cursor, hipStream_t st);
+                            unsigned long long* cursor, unsigned long long* part, hipStream_t st);
 template <int NW>
 hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const unsigned long long* coarse_off,
                       const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,

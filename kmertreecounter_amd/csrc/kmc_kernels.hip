@@ -6,7 +6,7 @@
 //   P1  k_extract<scatter>  bytes -> keys, LDS counting sort by coarse bucket,
 //                           coalesced runs into A (bucket-major, XCD-minor)
 //   P2a k_sub<count>        per coarse bucket: fine-bucket histogram
-//   S1  k_scan_fine
+//   S1  k_scan_fine_local, k_scan_fine_add
 //   P2b k_sub<scatter>      A -> B grouped by fine bucket
 //   P3  k_count             one workgroup per fine bucket: LDS Bloom filter
 //                           (first occurrences -> firsts) and LDS open-
@@ -357,36 +357,55 @@ k_scan_coarse(const unsigned long long* __restrict__ counts, unsigned long long*
     if (b == 255) tile_start[256] = tall;
 }
 
-// counts[XCDS][nf] -> fine_off[nf+1], cursor[XCDS][nf] (nf multiple of 1024 or
-// smaller power of two; one workgroup of 1024 threads)
+// counts[XCDS][nf] -> fine_off[nf+1], cursor[XCDS][nf], in two launches of
+// ceil(nf / 1024) workgroups (at most kScanGroups).  First: workgroup g sums
+// fine buckets [1024 g, 1024 g + 1024) over the XCDs (one bucket per thread,
+// coalesced), writes each bucket's offset inside the group to fine_off and the
+// group's total to part[g].
 __global__ void __launch_bounds__(1024)
-k_scan_fine(const unsigned long long* __restrict__ counts, uint32_t nf,
-            unsigned long long* __restrict__ fine_off, unsigned long long* __restrict__ cursor)
+k_scan_fine_local(const unsigned long long* __restrict__ counts, uint32_t nf, unsigned long long* __restrict__ fine_off,
+                  unsigned long long* __restrict__ part)
 {
     __shared__ unsigned long long s_scan[17];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (nf + 1023) / 1024;
-    const uint32_t f0 = t * per;
+    const uint32_t f = blockIdx.x * 1024 + threadIdx.x;
     unsigned long long sum = 0;
-    for (uint32_t j = 0; j < per; ++j) {
-        const uint32_t f = f0 + j;
-        if (f < nf) {
-            for (int x = 0; x < kXcds; ++x) sum += counts[(size_t)x * nf + f];
-        }
+    if (f < nf) {
+#pragma unroll
+        for (int x = 0; x < kXcds; ++x) sum += counts[(size_t)x * nf + f];
     }
     unsigned long long all;
-    unsigned long long run = block_exclusive_scan<unsigned long long>(sum, s_scan, &all);
-    for (uint32_t j = 0; j < per; ++j) {
-        const uint32_t f = f0 + j;
-        if (f < nf) {
-            fine_off[f] = run;
-            for (int x = 0; x < kXcds; ++x) {
-                cursor[(size_t)x * nf + f] = run;
-                run += counts[(size_t)x * nf + f];
-            }
+    const unsigned long long off = block_exclusive_scan<unsigned long long>(sum, s_scan, &all);
+    if (f < nf) fine_off[f] = off;
+    if (threadIdx.x == 0) part[blockIdx.x] = all;
+}
+
+// Second: each group adds the totals of the groups before it, then writes the
+// per-XCD cursors of its buckets; the last group writes fine_off[nf].
+__global__ void __launch_bounds__(1024)
+k_scan_fine_add(const unsigned long long* __restrict__ counts, uint32_t nf, unsigned long long* __restrict__ fine_off,
+                unsigned long long* __restrict__ cursor, const unsigned long long* __restrict__ part)
+{
+    __shared__ unsigned long long s_base;
+    const uint32_t g = blockIdx.x;
+    if (threadIdx.x < kWave) {
+        unsigned long long v = 0;
+        for (uint32_t j = threadIdx.x; j < g; j += kWave) v += part[j];
+#pragma unroll
+        for (int off = kWave / 2; off; off >>= 1) v += __shfl_down(v, off, kWave);
+        if (threadIdx.x == 0) s_base = v;
+    }
+    __syncthreads();
+    const uint32_t f = g * 1024 + threadIdx.x;
+    if (f < nf) {
+        unsigned long long run = s_base + fine_off[f];
+        fine_off[f] = run;
+#pragma unroll
+        for (int x = 0; x < kXcds; ++x) {
+            cursor[(size_t)x * nf + f] = run;
+            run += counts[(size_t)x * nf + f];
         }
     }
-    if (t == 1023) fine_off[nf] = all;
+    if (g == gridDim.x - 1 && threadIdx.x == 0) fine_off[nf] = s_base + part[g];
 }
 
 // ------------------------------------------------------------------ P2
@@ -1372,9 +1391,13 @@ hipError_t launch_scan_coarse(const unsigned long long* counts, unsigned long lo
 }
 
 hipError_t launch_scan_fine(const unsigned long long* counts, uint32_t nf, unsigned long long* fine_off,
-                            unsigned long long* cursor, hipStream_t st)
+                            unsigned long long* cursor, unsigned long long* part, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_scan_fine, dim3(1), dim3(1024), 0, st, counts, nf, fine_off, cursor);
+    const uint32_t groups = (nf + 1023) / 1024;
+    if (groups == 0 || groups > kScanGroups) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_fine_local, dim3(groups), dim3(1024), 0, st, counts, nf, fine_off, part);
+    KMC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_fine_add, dim3(groups), dim3(1024), 0, st, counts, nf, fine_off, cursor, part);
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -67,6 +67,12 @@ int mlp_debug_gemm(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, 
 // [b[j], b[j+1]).
 void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std::vector<float>& pred,
                                  std::vector<uint64_t>& b);
+// The network can predict one bucket for the keys of two neighbouring
+// quantiles and none for the next: that bucket is empty and its neighbour holds
+// twice the mean.  Each empty bucket next to a bucket of more than 1.5x the mean
+// of the sorted distinct sample `keys` takes half of that bucket's keys
+// (boundary moved to its median).  Returns the number of buckets split.
+uint32_t split_merged_buckets(const std::vector<uint64_t>& keys, std::vector<uint64_t>& b);
 // Coarse (255) then per-coarse fine (256 x (2^sbits - 1)) routing boundaries
 // from the 4097 learned boundaries; fine ones interpolate inside a learned bucket.
 void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<uint64_t>& out);

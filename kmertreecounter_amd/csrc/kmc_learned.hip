@@ -731,6 +731,35 @@ void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std:
     }
 }
 
+uint32_t split_merged_buckets(const std::vector<uint64_t>& keys, std::vector<uint64_t>& b)
+{
+    const size_t n = keys.size();
+    std::vector<size_t> pos(kMlpC + 1);
+    for (uint32_t j = 0; j < kMlpC; ++j) pos[j] = std::lower_bound(keys.begin(), keys.end(), b[j]) - keys.begin();
+    pos[kMlpC] = n;
+    const double mean = (double)n / kMlpC;
+    uint32_t split = 0;
+    for (uint32_t e = 0; e < kMlpC; ++e) {
+        if (pos[e + 1] != pos[e]) continue;
+        const size_t below = e > 0 ? pos[e] - pos[e - 1] : 0;
+        const size_t above = e + 1 < kMlpC ? pos[e + 2] - pos[e + 1] : 0;
+        if (2.0 * (double)std::max(below, above) < 3.0 * mean) continue;
+        if (below >= above) {
+            // bucket e takes the upper half of bucket e - 1
+            const size_t m = pos[e - 1] + below / 2;
+            b[e] = keys[m];
+            pos[e] = m;
+        } else {
+            // bucket e takes the lower half of bucket e + 1
+            const size_t m = pos[e + 1] + above / 2;
+            b[e + 1] = keys[m];
+            pos[e + 1] = m;
+        }
+        ++split;
+    }
+    return split;
+}
+
 void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<uint64_t>& out)
 {
     const uint32_t F = 1u << sbits;

@@ -23,6 +23,7 @@ struct PlanResult {
     std::vector<uint64_t> learned_b;
     std::vector<float> loss_curve;      // training loss every 50 steps
     uint64_t train_keys = 0;            // sorted sample keys the model saw
+    uint32_t split_buckets = 0;         // empty learned buckets refilled by split_merged_buckets
     double train_ms = 0.0;
     double sample_max_over_mean_learned = 0.0;  // over the 4096 learned buckets, on the sample
     double sample_max_over_mean_prefix = 0.0;   // over the 4096 12-bit prefixes, on the sample

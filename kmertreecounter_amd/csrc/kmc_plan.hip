@@ -54,6 +54,13 @@ void bloom_capacity(const uint64_t* counts, uint64_t est_file_size, uint64_t* ca
 
 namespace {
 
+uint64_t splitmix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
 uint32_t key_words(uint32_t k) { return k <= 32 ? 1u : k <= 64 ? 2u : 4u; }
 
 uint64_t extract_tile(uint32_t nw)
@@ -177,12 +184,17 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
         err = std::string("learned plan: ") + hipGetErrorString(e);
         return KC_ERR_HIP;
     }
-    // word 0 of every key, at most 4M of them (evenly strided), sorted
+    // word 0 of the keys, at most about 4M of them, sorted.  A key is kept by
+    // its hash, so the sample does not depend on the order the extraction
+    // wrote the keys in, and a kept key keeps all its occurrences.
     const uint64_t cap = 4ULL << 20;
     const uint64_t stride = nkeys > cap ? (nkeys + cap - 1) / cap : 1;
     std::vector<uint64_t> sorted;
-    sorted.reserve(nkeys / stride + 1);
-    for (uint64_t i = 0; i < nkeys; i += stride) sorted.push_back(all[i * nw]);
+    sorted.reserve(nkeys / stride + nkeys / (4 * stride) + 1);
+    for (uint64_t i = 0; i < nkeys; ++i) {
+        const uint64_t w0 = all[i * nw];
+        if (stride == 1 || splitmix64(w0) % stride == 0) sorted.push_back(w0);
+    }
     std::vector<uint64_t>().swap(all);
     std::sort(sorted.begin(), sorted.end());
     MlpTrainOpts o;
@@ -210,6 +222,7 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     cleanup();
     if (rc) return KC_ERR_HIP;
     boundaries_from_predictions(train, pred, out.learned_b);
+    out.split_buckets = split_merged_buckets(uniq, out.learned_b);
     out.train_keys = train.size();
     std::vector<uint64_t> lb(out.learned_b.begin(), out.learned_b.begin() + kMlpC);
     out.sample_max_over_mean_learned = imbalance(sorted, lb);

@@ -102,6 +102,29 @@ def test_learned_partition_balances_skewed_input_and_stays_exact():
     assert sl.extra_passes <= sp.extra_passes
 
 
+def test_learned_plan_is_deterministic_and_balanced_on_uniform_input():
+    # workload 2 shape; the sample (24M keys) is above the 4M-key cap, so the
+    # plan keeps keys by hash: two plans see the same keys and give the same
+    # boundaries whatever order the extraction wrote the keys in
+    seq = K.generate(2, scale=10, seed=7, fmt="seq")
+    stats = []
+    for _ in range(2):
+        with K.Engine(31, partition=K.KC_PARTITION_LEARNED) as e:
+            e.sample_block(seq[: len(seq) // 10])
+            e.plan()
+            stats.append(e.stats())
+    a, b = stats
+    print(f"train keys {a.learned_train_keys}, split buckets {a.learned_split_buckets}, sample max/mean "
+          f"{a.learned_sample_max_over_mean:.3f} (distinct {a.learned_sample_distinct_max_over_mean:.3f}), "
+          f"prefix {a.prefix_sample_max_over_mean:.3f}")
+    assert 1_000_000 < a.learned_train_keys <= 4_500_000  # distinct keys of the ~4M kept occurrences
+    assert a.learned_train_keys == b.learned_train_keys
+    assert a.learned_sample_max_over_mean == b.learned_sample_max_over_mean
+    assert a.learned_split_buckets == b.learned_split_buckets
+    assert a.learned_sample_distinct_max_over_mean < 1.3
+    assert a.learned_sample_max_over_mean < 0.5 * a.prefix_sample_max_over_mean
+
+
 def test_learned_plan_falls_back_to_prefix_on_a_degenerate_sample():
     # one distinct key in the sample: nothing to learn, the prefix partition runs
     seq = (b"A" * 149 + b"\n") * 20000 + (b"ACGTTGCA" * 18 + b"\n") * 10
