@@ -224,28 +224,4 @@ struct Route {
     uint32_t rlo, rhi;
 };
 
-// Number of entries <= x in ascending b[0..255): branchless, eight fixed
-// steps, so the searches of several keys interleave.
-__device__ __forceinline__ uint32_t upper_count255(const uint64_t* b, uint64_t x)
-{
-    uint32_t pos = 0;
-#pragma unroll
-    for (int s = 7; s >= 0; --s) {
-        const uint32_t q = pos + (1u << s);
-        pos = (b[q - 1] <= x) ? q : pos;
-    }
-    return pos;
-}
-
-__device__ __forceinline__ uint32_t upper_count(const uint64_t* b, uint32_t n, uint64_t x)
-{
-    // number of entries <= x in ascending b[0..n)
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (b[mid] <= x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 }  // namespace kmc
