@@ -172,6 +172,19 @@ typedef enum kc_histogram_mode {
 } kc_histogram_mode;
 KC_API int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_freq, int64_t* hist);
 
+/* Replaces FastqReader<N>::read's line state machine (src/FastqReader.h:140-308):
+ * `data` is the next `len` bytes of a FASTQ file, cut anywhere (a record or a
+ * line may span calls).  The bytes are copied to the device as they are and
+ * the sequence lines (line index mod 4 == 1 from the first byte of the first
+ * call), each with its '\n', are appended to the resident sequence by device
+ * kernels.  Whole lines are kept together, which gives the k-mers of the
+ * reference's 32 KiB blocks for every k except 96, whose keys carry the
+ * bases before the window and so depend on where the reference cuts its
+ * blocks: k = 96 returns KC_ERR_ARG (push the host reader's blocks instead).
+ * kc_push_block after a FASTQ chunk that ends inside a record returns
+ * KC_ERR_STATE. */
+KC_API int kc_push_fastq(kc_engine* h, const char* data, uint64_t len);
+
 /* Keys callback: `keys` holds n keys of N u64 words each. */
 typedef int (*kc_keys_cb)(void* user, const uint64_t* keys, uint64_t n);
 
@@ -210,6 +223,11 @@ KC_API uint32_t kc_words_for_k(uint32_t k);
 
 /* Human-readable text of the last error on this handle (never NULL). */
 KC_API const char* kc_last_error(kc_engine* h);
+
+/* Test hook, no reference counterpart: copies up to `cap` bytes of the
+ * resident sequence (what the pushed blocks or FASTQ chunks left in HBM after
+ * kc_finish) to `out`; *len = its full length. */
+KC_API int kc_debug_resident(kc_engine* h, char* out, uint64_t cap, uint64_t* len);
 
 /* kc_debug_copy: one plain device copy of `bytes` (a multiple of 16) with a
  * 16-byte-per-lane kernel; *ms = its device time.  Reads `bytes` and writes

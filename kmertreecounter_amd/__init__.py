@@ -114,7 +114,7 @@ EXPORTED = [
     "kc_last_error", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
-    "kmc_write_histogram", "kc_autotune",
+    "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident",
 ]
 
 _lib = None
@@ -138,6 +138,8 @@ def lib() -> ctypes.CDLL:
     L.kc_sample_block.argtypes = [vp, cp, u64]
     L.kc_plan.argtypes = [vp]
     L.kc_push_block.argtypes = [vp, cp, u64]
+    L.kc_push_fastq.argtypes = [vp, cp, u64]
+    L.kc_debug_resident.argtypes = [vp, vp, u64, ctypes.POINTER(u64)]
     L.kc_finish.argtypes = [vp]
     L.kc_count_resident.argtypes = [vp]
     L.kc_emit_firsts.argtypes = [vp, KEYS_CB, vp]
@@ -277,6 +279,18 @@ class Engine:
 
     def push_block(self, data: bytes) -> None:
         self._check(self._L.kc_push_block(self._h, data, len(data)), "kc_push_block")
+
+    def push_fastq(self, data: bytes) -> None:
+        """Raw FASTQ bytes, parsed on the device (any cut points between calls)."""
+        self._check(self._L.kc_push_fastq(self._h, data, len(data)), "kc_push_fastq")
+
+    def resident(self) -> bytes:
+        """The resident sequence bytes after finish() (test hook)."""
+        n = ctypes.c_uint64()
+        self._check(self._L.kc_debug_resident(self._h, None, 0, ctypes.byref(n)), "kc_debug_resident")
+        buf = ctypes.create_string_buffer(max(1, n.value))
+        self._check(self._L.kc_debug_resident(self._h, buf, n.value, ctypes.byref(n)), "kc_debug_resident")
+        return buf.raw[: n.value]
 
     def finish(self) -> None:
         self._check(self._L.kc_finish(self._h), "kc_finish")

@@ -48,8 +48,15 @@ struct kc_engine {
     bool stage_busy[2] = {false, false};
     int cur_stage = 0;
     uint64_t stage_len = 0;
-    char last_byte = '\n';
     double h2d_ms = 0;
+    // FASTQ chunks (kc_push_fastq): staged like blocks, parsed on the device
+    bool stage_fastq = false;           // the current stage holds FASTQ bytes
+    uint8_t* d_raw = nullptr;           // one staged chunk as pushed
+    void* d_fq_scratch = nullptr;       // per-tile spans, phases and offsets
+    unsigned long long* d_fq_state = nullptr;  // [sequence length, line phase]
+    bool fq_state_live = false;         // d_fq_state holds the current stream state
+    uint64_t fq_unsettled = 0;          // FASTQ bytes parsed since seq_len was last read back
+    uint32_t fq_phase = 0;              // line phase after the last settled chunk
 
     // Stage 0
     std::vector<char> sample;
@@ -161,7 +168,9 @@ int ensure_seq_capacity(kc_engine* h, uint64_t need_len)
     }
     if (h->d_seq) {
         HIPX(hipStreamSynchronize(h->st));
-        HIPX(hipMemcpy(nd, h->d_seq, h->seq_len, hipMemcpyDeviceToDevice));
+        // parsed FASTQ output not yet read back is at most fq_unsettled bytes
+        const uint64_t used = std::min<uint64_t>(h->seq_alloc, h->seq_len + h->fq_unsettled);
+        HIPX(hipMemcpy(nd, h->d_seq, used, hipMemcpyDeviceToDevice));
         HIPX(hipFree(h->d_seq));
     }
     h->d_seq = nd;
@@ -189,6 +198,58 @@ int flush_stage(kc_engine* h)
     return KC_OK;
 }
 
+// The staged FASTQ chunk goes to d_raw and its sequence lines are appended to
+// d_seq by the parse kernels; the sequence length stays on the device until
+// settle_fastq reads it back.
+int flush_fastq(kc_engine* h)
+{
+    if (h->stage_len == 0) return KC_OK;
+    if (!h->d_raw) {
+        if (hipMalloc(&h->d_raw, kStageBytes + 64) != hipSuccess ||
+            hipMalloc(&h->d_fq_scratch, fastq_tile_scratch_bytes(kStageBytes)) != hipSuccess ||
+            hipMalloc(&h->d_fq_state, 2 * sizeof(unsigned long long)) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for FASTQ parsing failed");
+    }
+    // each raw byte yields at most one sequence byte
+    int rc = ensure_seq_capacity(h, h->seq_len + h->fq_unsettled + h->stage_len);
+    if (rc) return rc;
+    if (!h->fq_state_live) {
+        const unsigned long long init[2] = {h->seq_len, h->fq_phase};
+        HIPX(hipStreamSynchronize(h->st));
+        HIPX(hipMemcpy(h->d_fq_state, init, sizeof(init), hipMemcpyHostToDevice));
+        h->fq_state_live = true;
+    }
+    const int s = h->cur_stage;
+    HIPX(hipMemcpyAsync(h->d_raw, h->h_stage[s], h->stage_len, hipMemcpyHostToDevice, h->st));
+    HIPX(hipEventRecord(h->stage_ev[s], h->st));
+    h->stage_busy[s] = true;
+    HIPX(launch_fastq_parse(h->d_raw, h->stage_len, h->d_fq_scratch, h->d_fq_state, h->d_seq, h->st));
+    h->fq_unsettled += h->stage_len;
+    h->stage_len = 0;
+    h->cur_stage ^= 1;
+    const int n = h->cur_stage;
+    if (h->stage_busy[n]) {
+        HIPX(hipEventSynchronize(h->stage_ev[n]));
+        h->stage_busy[n] = false;
+    }
+    return KC_OK;
+}
+
+int flush_any(kc_engine* h) { return h->stage_fastq ? flush_fastq(h) : flush_stage(h); }
+
+// Reads the sequence length and line phase back after FASTQ chunks were parsed.
+int settle_fastq(kc_engine* h)
+{
+    if (!h->fq_unsettled) return KC_OK;
+    unsigned long long st[2] = {0, 0};
+    HIPX(hipStreamSynchronize(h->st));
+    HIPX(hipMemcpy(st, h->d_fq_state, sizeof(st), hipMemcpyDeviceToHost));
+    h->seq_len = st[0];
+    h->fq_phase = (uint32_t)st[1];
+    h->fq_unsettled = 0;
+    return KC_OK;
+}
+
 int append_bytes(kc_engine* h, const char* data, uint64_t len)
 {
     while (len) {
@@ -199,7 +260,7 @@ int append_bytes(kc_engine* h, const char* data, uint64_t len)
         data += n;
         len -= n;
         if (h->stage_len == kStageBytes) {
-            int rc = flush_stage(h);
+            int rc = flush_any(h);
             if (rc) return rc;
         }
     }
@@ -707,6 +768,9 @@ int kc_destroy(kc_engine* h)
     hipSetDevice(h->device);
     if (h->st) hipStreamSynchronize(h->st);
     hipFree(h->d_seq);
+    hipFree(h->d_raw);
+    hipFree(h->d_fq_scratch);
+    hipFree(h->d_fq_state);
     free_work(h);
     hipFree(h->d_small);
     hipFree(h->d_ctr);
@@ -765,6 +829,15 @@ int kc_push_block(kc_engine* h, const char* data, uint64_t len)
     if (!len) return KC_OK;
     HIPX(hipSetDevice(h->device));
     h->have_results = false;
+    if (h->stage_fastq) {
+        int rc = flush_fastq(h);
+        if (!rc) rc = settle_fastq(h);
+        if (rc) return rc;
+        // a FASTQ stream stopped inside a record would run into the block's lines
+        if (h->fq_phase != 0) return h->fail(KC_ERR_STATE, "kc_push_block after a FASTQ chunk that ends inside a record");
+        h->stage_fastq = false;
+        h->fq_state_live = false;  // seq_len moves on without the device state
+    }
     int rc = append_bytes(h, data, len);
     if (rc) return rc;
     // a block ending inside a line is followed by one that starts with its carry:
@@ -778,6 +851,36 @@ int kc_push_block(kc_engine* h, const char* data, uint64_t len)
     return KC_OK;
 }
 
+int kc_push_fastq(kc_engine* h, const char* data, uint64_t len)
+{
+    if (!h || (!data && len)) return KC_ERR_ARG;
+    if (!len) return KC_OK;
+    if (h->k == 96)
+        return h->fail(KC_ERR_ARG, "k = 96 keys depend on the reader's 32 KiB block seams: push the host reader's blocks");
+    HIPX(hipSetDevice(h->device));
+    h->have_results = false;
+    if (!h->stage_fastq) {
+        int rc = flush_stage(h);
+        if (rc) return rc;
+        h->stage_fastq = true;
+    }
+    return append_bytes(h, data, len);
+}
+
+int kc_debug_resident(kc_engine* h, char* out, uint64_t cap, uint64_t* len)
+{
+    if (!h || !len || (!out && cap)) return KC_ERR_ARG;
+    HIPX(hipSetDevice(h->device));
+    if (h->stage_len || h->fq_unsettled) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
+    *len = h->seq_len;
+    const uint64_t n = std::min<uint64_t>(cap, h->seq_len);
+    if (n) {
+        HIPX(hipStreamSynchronize(h->st));
+        HIPX(hipMemcpy(out, h->d_seq, n, hipMemcpyDeviceToHost));
+    }
+    return KC_OK;
+}
+
 int kc_finish(kc_engine* h)
 {
     if (!h) return KC_ERR_ARG;
@@ -786,7 +889,8 @@ int kc_finish(kc_engine* h)
     HIPX(hipEventCreate(&t0));
     HIPX(hipEventCreate(&t1));
     HIPX(hipEventRecord(t0, h->st));
-    int rc = flush_stage(h);
+    int rc = flush_any(h);
+    if (!rc) rc = settle_fastq(h);
     if (rc) return rc;
     HIPX(hipEventRecord(t1, h->st));
     HIPX(hipEventSynchronize(t1));
@@ -808,7 +912,7 @@ int kc_count_resident(kc_engine* h)
 {
     if (!h) return KC_ERR_ARG;
     HIPX(hipSetDevice(h->device));
-    if (h->stage_len) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
+    if (h->stage_len || h->fq_unsettled) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
     return run_pipeline(h);
 }
 
@@ -816,7 +920,7 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
 {
     if (!h) return KC_ERR_ARG;
     HIPX(hipSetDevice(h->device));
-    if (h->stage_len) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
+    if (h->stage_len || h->fq_unsettled) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
     if (!h->seq_len) return h->fail(KC_ERR_STATE, "no resident input: push blocks and call kc_finish first");
     // candidates: fine fan-out around the automatic one, both counters, two
     // heavy-bucket thresholds; each is timed on one pass over the resident input

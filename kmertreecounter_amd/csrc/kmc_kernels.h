@@ -154,6 +154,12 @@ hipError_t launch_debug_copy(const void* src, void* dst, uint64_t n16, hipStream
 // bytes of p set to v by kernels (the captured pipeline graph has no memset nodes)
 hipError_t launch_fill(void* p, uint64_t bytes, uint8_t v, hipStream_t st);
 
+// FASTQ chunk -> sequence lines appended at state[0] (state[1]: line phase at
+// the chunk's first byte); scratch of fastq_tile_scratch_bytes(n) bytes.
+uint64_t fastq_tile_scratch_bytes(uint64_t max_chunk);
+hipError_t launch_fastq_parse(const uint8_t* raw, uint64_t n, void* scratch, unsigned long long* state, uint8_t* seq,
+                              hipStream_t st);
+
 // Count histogram of the resident results (precise or approximate mode).
 constexpr uint32_t kHistProbe = 4096;
 struct HistArgs {
