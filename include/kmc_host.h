@@ -57,6 +57,8 @@ typedef struct kmc_tree_opts {
     const char* histogram_path; /* if set: write the precise count histogram here, as histogram_tool does */
     uint32_t hist_min;      /* histogram range (0 = 1 and 10000, the tool's defaults) */
     uint32_t hist_max;
+    uint32_t host_parse;    /* 1: the host reader's blocks (kc_push_block); 0: raw FASTQ chunks parsed on
+                               the device (kc_push_fastq).  k = 96 always uses the host reader. */
 } kmc_tree_opts;
 
 KC_API int kmc_tree_run(const kmc_tree_opts* opts, kc_stats* stats, double* wall_ms);

@@ -100,7 +100,7 @@ class TreeOpts(ctypes.Structure):
         ("n_thread", ctypes.c_uint32), ("min_count", ctypes.c_uint32), ("max_count", ctypes.c_uint32),
         ("partition", ctypes.c_uint32), ("verbose", ctypes.c_uint32), ("sample_bytes", ctypes.c_uint64),
         ("counter", ctypes.c_uint32), ("histogram_only", ctypes.c_uint32), ("histogram_path", ctypes.c_char_p),
-        ("hist_min", ctypes.c_uint32), ("hist_max", ctypes.c_uint32),
+        ("hist_min", ctypes.c_uint32), ("hist_max", ctypes.c_uint32), ("host_parse", ctypes.c_uint32),
     ]
 
 
@@ -421,10 +421,11 @@ def write_histogram(path: str, hist: np.ndarray, min_freq: int = 1) -> None:
 def tree_run(fastq_path: str, tmp_dir: str, k: int, n_thread: int = 8, min_count: int = 1, max_count: int = 0,
              partition: int = KC_PARTITION_PREFIX, verbose: bool = False, sample_bytes: int = 0,
              counter: int = KC_COUNTER_BLOOM, histogram_path: str | None = None, histogram_only: bool = False,
-             hist_min: int = 0, hist_max: int = 0) -> tuple[Stats, float]:
+             hist_min: int = 0, hist_max: int = 0, host_parse: bool = False) -> tuple[Stats, float]:
     o = TreeOpts(str(fastq_path).encode(), str(tmp_dir).encode(), k, n_thread, min_count, max_count, partition,
                  1 if verbose else 0, sample_bytes, counter, 1 if histogram_only else 0,
-                 str(histogram_path).encode() if histogram_path else None, hist_min, hist_max)
+                 str(histogram_path).encode() if histogram_path else None, hist_min, hist_max,
+                 1 if host_parse else 0)
     s = Stats()
     wall = ctypes.c_double()
     rc = lib().kmc_tree_run(ctypes.byref(o), ctypes.byref(s), ctypes.byref(wall))

@@ -198,6 +198,21 @@ int push_cb(void* user, const char* data, uint64_t len)
     return c->fn(c->h, data, len);
 }
 
+// The whole file, raw, to the device parser in 16 MiB reads.
+int push_fastq_file(kc_engine* h, const char* path)
+{
+    FileSource src;
+    if (src.open(path) != 0) return KC_ERR_ARG;
+    std::vector<char> buf(8 * kReadChunk);
+    for (;;) {
+        const int64_t n = src.read(buf.data(), buf.size());
+        if (n < 0) return KC_ERR_ARG;
+        if (n == 0) return KC_OK;
+        int rc = kc_push_fastq(h, buf.data(), (uint64_t)n);
+        if (rc) return rc;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -334,7 +349,12 @@ int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
     if (!rc) rc = kc_plan(h);
     const auto t1 = clk::now();
     PushCtx pc{h, 0, kc_push_block};
-    if (!rc) rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, 0, push_cb, &pc);
+    if (!rc) {
+        if (o->host_parse || o->k == 96)
+            rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, 0, push_cb, &pc);
+        else
+            rc = push_fastq_file(h, o->fastq_path);
+    }
     if (!rc) rc = kc_finish(h);
     const auto t2 = clk::now();
     if (!rc && !o->histogram_only) rc = kmc_write_outputs(h, o->tmp_dir);

@@ -64,6 +64,8 @@ int main(int argc, char** argv)
     o.histogram_only = (o.histogram_path && hist_only && std::strcmp(hist_only, "1") == 0) ? 1u : 0u;
     const char* counter = std::getenv("KMC_COUNTER");
     o.counter = (counter && std::strcmp(counter, "exact") == 0) ? KC_COUNTER_EXACT : KC_COUNTER_BLOOM;
+    const char* parse = std::getenv("KMC_PARSE");
+    o.host_parse = (parse && std::strcmp(parse, "host") == 0) ? 1u : 0u;
     o.verbose = 1;
     kc_stats st;
     double wall = 0;

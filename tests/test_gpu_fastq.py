@@ -128,3 +128,27 @@ def test_k96_and_mixed_pushes_are_refused():
         e.push_block(b"TTTTGGGGCCCCAAAATTTTGGGGCCCCAAAAT\n")
         e.finish()
         assert e.resident() == sequence_lines(rec) + b"TTTTGGGGCCCCAAAATTTTGGGGCCCCAAAAT\n"
+
+
+@pytest.mark.parametrize("gz", [False, True], ids=["plain", "gzip"])
+def test_tree_run_parses_on_the_device_like_the_host_reader(tmp_path, gz):
+    import gzip
+    fq = K.generate(4, scale=2000, seed=21)
+    path = tmp_path / ("in.fq.gz" if gz else "in.fq")
+    path.write_bytes(gzip.compress(fq, compresslevel=1) if gz else fq)
+    tables = []
+    for host in (True, False):
+        out = tmp_path / ("host" if host else "dev")
+        out.mkdir()
+        s, _ = K.tree_run(str(path), str(out), 31, host_parse=host)
+        nw = 1
+        recs = np.concatenate([np.fromfile(out / f"root_{i}.bin", dtype=K.record_dtype(nw)) for i in range(256)])
+        low = (out / "low.bin").read_bytes()
+        firsts = np.zeros(len(low) // 8, dtype=np.uint64)
+        C.oracle().orc_unpack_low(low, len(low) // 8, 31, nw, firsts.ctypes.data)
+        tables.append(C.merge_outputs(firsts, recs, nw))
+        assert s.total_kmers == int(tables[-1][1].sum())
+    keys, counts = C.count_table(C.extract_blocks(K.fastq_blocks(fq, 31), 31))
+    for got_k, got_c in tables:
+        np.testing.assert_array_equal(got_k, keys)
+        np.testing.assert_array_equal(got_c, counts)
