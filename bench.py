@@ -261,6 +261,33 @@ def summary(r: dict) -> dict:
     }
 
 
+def ingest_report(workload: int, scale: float, seed: int, k: int, threads: int) -> dict:
+    """End to end from a FASTQ file (kmc_tree_run, histogram only, no output
+    files): the host reader's blocks against raw chunks parsed on the device."""
+    import shutil
+    import tempfile
+    fq = K.generate(workload, scale=scale, seed=seed, fmt="fastq", threads=threads)
+    work = Path(tempfile.mkdtemp(prefix="kmc_ingest_"))
+    try:
+        path = work / "in.fq"
+        path.write_bytes(fq)
+        nbytes = len(fq)
+        del fq
+        res = {"fastq_bytes": nbytes}
+        for name, host in (("host_reader", True), ("device_parser", False)):
+            out = work / name
+            out.mkdir()
+            st, wall_ms = K.tree_run(str(path), str(out), k, histogram_only=True,
+                                     histogram_path=str(out / "hist.tsv"), host_parse=host)
+            res[name] = {"wall_s": wall_ms / 1e3, "device_count_ms": st.ms_total, "h2d_ms": st.ms_h2d,
+                         "kmers": st.total_kmers}
+            log(f"[ingest {name}] {nbytes / 1e9:.2f} GB FASTQ: wall {wall_ms / 1e3:.2f} s, "
+                f"device counting {st.ms_total:.1f} ms, K={st.total_kmers}")
+        return res
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +303,8 @@ def main() -> int:
                     help="replay the pipeline as a captured HIP graph (per-kernel times are then not recorded)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ingest", action="store_true",
+                    help="also time kmc_tree_run end to end from a FASTQ file, host reader vs device parser")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="after the run, copy this many GiB with k_copy16 (a plain copy whose "
                          "FETCH_SIZE/WRITE_SIZE readings scale those of the pipeline kernels)")
@@ -368,6 +397,8 @@ def main() -> int:
         n = int(args.calibration_copy_gib * (1 << 30)) // 16 * 16
         ms = K.debug_copy(n)  # k_copy16: reads n bytes, writes n bytes
         out["calibration_copy"] = {"bytes": n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
+    if args.ingest:
+        out["ingest"] = ingest_report(args.workload, args.scale, args.seed, k, threads)
     if "learned" in runs:
         out["learned"] = runs["learned"]["learned"]
     if len(runs) == 2:
