@@ -115,6 +115,8 @@ struct kc_engine {
     std::vector<unsigned long long> root_off;
     double phase_ms[7] = {};
     char* h_bounce = nullptr;
+    char* h_bounce2 = nullptr;             // second emit buffer: the next chunk copies while the callback runs
+    hipEvent_t emit_ev[2] = {nullptr, nullptr};
 
     int fail(int code, const std::string& what)
     {
@@ -708,6 +710,36 @@ int run_pipeline(kc_engine* h)
     }
 }
 
+// n items of `isz` bytes from device memory to the callback in pinned chunks
+// of at most kEmitChunk bytes; chunk c + 1 is copied while the callback
+// handles chunk c.
+template <typename F>
+int emit_chunks(kc_engine* h, const uint8_t* src, uint64_t n, uint64_t isz, F&& f)
+{
+    if (n == 0) return KC_OK;
+    const uint64_t per = kEmitChunk / isz;
+    char* buf[2] = {h->h_bounce, h->h_bounce2};
+    auto issue = [&](uint64_t c) -> hipError_t {
+        const uint64_t i = c * per, m = std::min(per, n - i);
+        hipError_t e = hipMemcpyAsync(buf[c & 1], src + i * isz, m * isz, hipMemcpyDeviceToHost, h->st);
+        if (e == hipSuccess) e = hipEventRecord(h->emit_ev[c & 1], h->st);
+        return e;
+    };
+    const uint64_t chunks = (n + per - 1) / per;
+    HIPX(issue(0));
+    for (uint64_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks) HIPX(issue(c + 1));
+        HIPX(hipEventSynchronize(h->emit_ev[c & 1]));
+        const uint64_t m = std::min(per, n - c * per);
+        const int rc = f(buf[c & 1], m);
+        if (rc) {
+            HIPX(hipStreamSynchronize(h->st));
+            return h->fail(rc, "emit callback returned an error");
+        }
+    }
+    return KC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -753,6 +785,9 @@ int kc_create(const kc_config* cfg, kc_engine** out)
         if (hipEventCreateWithFlags(&h->stage_ev[i], hipEventDisableTiming) != hipSuccess) return bail(KC_ERR_HIP);
     }
     if (hipHostMalloc((void**)&h->h_bounce, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+    if (hipHostMalloc((void**)&h->h_bounce2, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+    for (int i = 0; i < 2; ++i)
+        if (hipEventCreateWithFlags(&h->emit_ev[i], hipEventDisableTiming) != hipSuccess) return bail(KC_ERR_HIP);
     for (auto& e : h->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(KC_ERR_HIP);
     if (cfg->expected_bytes) {
@@ -792,6 +827,9 @@ int kc_destroy(kc_engine* h)
         if (h->stage_ev[i]) hipEventDestroy(h->stage_ev[i]);
     }
     if (h->h_bounce) hipHostFree(h->h_bounce);
+    if (h->h_bounce2) hipHostFree(h->h_bounce2);
+    for (int i = 0; i < 2; ++i)
+        if (h->emit_ev[i]) hipEventDestroy(h->emit_ev[i]);
     for (auto& e : h->ev)
         if (e) hipEventDestroy(e);
     if (h->st) hipStreamDestroy(h->st);
@@ -988,13 +1026,9 @@ int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
         }
         return KC_OK;
     }
-    for (uint64_t i = 0; i < h->n_first; i += per) {
-        const uint64_t n = std::min(per, h->n_first - i);
-        HIPX(hipMemcpy(h->h_bounce, h->res_firsts + i * h->nw, n * kb, hipMemcpyDeviceToHost));
-        int rc = cb(user, reinterpret_cast<const uint64_t*>(h->h_bounce), n);
-        if (rc) return h->fail(rc, "keys callback returned an error");
-    }
-    return KC_OK;
+    return emit_chunks(h, reinterpret_cast<const uint8_t*>(h->res_firsts), h->n_first, kb, [&](const void* p, uint64_t n) {
+        return cb(user, static_cast<const uint64_t*>(p), n);
+    });
 }
 
 int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
@@ -1019,12 +1053,8 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     // in-bucket records of the root, then its overflow records
     for (int part = 0; part < 2; ++part) {
         const uint64_t b = h->root_off[part * 257 + root_id], e = h->root_off[part * 257 + root_id + 1];
-        for (uint64_t i = b; i < e; i += per) {
-            const uint64_t n = std::min(per, e - i);
-            HIPX(hipMemcpy(h->h_bounce, h->res_recs + i * rb, n * rb, hipMemcpyDeviceToHost));
-            int rc = cb(user, h->h_bounce, n);
-            if (rc) return h->fail(rc, "records callback returned an error");
-        }
+        int rc = emit_chunks(h, h->res_recs + b * rb, e - b, rb, [&](const void* p, uint64_t n) { return cb(user, p, n); });
+        if (rc) return rc;
     }
     return KC_OK;
 }

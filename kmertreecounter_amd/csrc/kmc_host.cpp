@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kmc_host.h"
@@ -141,18 +142,40 @@ struct FileSource {
     }
 };
 
-bool write_all(int fd, const void* p, uint64_t n)
+bool pwrite_all(int fd, const void* p, uint64_t n, uint64_t off)
 {
-    const char* c = (const char*)p;
+    const char* c = static_cast<const char*>(p);
     while (n) {
-        const ssize_t w = ::write(fd, c, n);
-        if (w < 0) {
-            if (errno == EINTR) continue;
-            return false;
-        }
+        const ssize_t w = ::pwrite(fd, c, n, (off_t)off);
+        if (w <= 0) return false;
         c += w;
         n -= (uint64_t)w;
+        off += (uint64_t)w;
     }
+    return true;
+}
+
+// Output writers: each emitted chunk is split over kWriteThreads threads that
+// pack (low.bin) and pwrite their slice at its file offset, so the page-cache
+// copies run in parallel while the engine copies the next chunk from HBM.
+constexpr uint32_t kWriteThreads = 8;
+constexpr uint64_t kParallelBytes = 1ULL << 20;  // smaller chunks are written by the calling thread
+
+template <typename F>
+bool parallel_slices(uint64_t n, uint64_t bytes_per_item, F&& f)
+{
+    const uint32_t t = (n * bytes_per_item < kParallelBytes) ? 1u : kWriteThreads;
+    if (t == 1) return f(0, n, 0);
+    std::vector<std::thread> th;
+    std::vector<char> ok(t, 1);
+    const uint64_t per = (n + t - 1) / t;
+    for (uint32_t i = 0; i < t; ++i) {
+        const uint64_t b = std::min(n, i * per), e = std::min(n, b + per);
+        th.emplace_back([&, i, b, e] { ok[i] = f(b, e, i) ? 1 : 0; });
+    }
+    for (auto& x : th) x.join();
+    for (char c : ok)
+        if (!c) return false;
     return true;
 }
 
@@ -161,28 +184,44 @@ struct LowWriter {
     uint32_t k = 0;
     uint32_t nw = 1;
     uint64_t pb = 0;
-    std::vector<uint8_t> buf;
-    bool ok = true;
+    uint64_t at = 0;  // keys written so far
+    std::vector<uint8_t> buf[kWriteThreads];
 };
 
 int low_cb(void* user, const uint64_t* keys, uint64_t n)
 {
     LowWriter* w = (LowWriter*)user;
-    const uint64_t per = (4ULL << 20) / w->pb;
-    for (uint64_t i = 0; i < n; i += per) {
-        const uint64_t m = std::min(per, n - i);
-        w->buf.resize(m * w->pb);
-        kmc_pack_low(keys + i * w->nw, m, w->k, w->buf.data());
-        if (!write_all(w->fd, w->buf.data(), w->buf.size())) return -1;
-    }
-    return 0;
+    const bool ok = parallel_slices(n, w->pb, [&](uint64_t b, uint64_t e, uint32_t t) {
+        std::vector<uint8_t>& buf = w->buf[t];
+        const uint64_t per = (4ULL << 20) / w->pb;
+        for (uint64_t i = b; i < e; i += per) {
+            const uint64_t m = std::min(per, e - i);
+            buf.resize(m * w->pb);
+            kmc_pack_low(keys + i * w->nw, m, w->k, buf.data());
+            if (!pwrite_all(w->fd, buf.data(), buf.size(), (w->at + i) * w->pb)) return false;
+        }
+        return true;
+    });
+    w->at += n;
+    return ok ? 0 : -1;
 }
+
+struct RootWriter {
+    int fd = -1;
+    uint64_t rb = 0;  // bytes per ExportRecord<N>
+    uint64_t at = 0;  // records written so far
+};
 
 int root_cb(void* user, const void* recs, uint64_t n)
 {
-    const int fd = *(int*)user;
+    RootWriter* w = (RootWriter*)user;
     // ExportRecord<N> bytes straight from the device buffer (padding is zero)
-    return write_all(fd, recs, n * (8ULL * (uint64_t)((int*)user)[1] + 8)) ? 0 : -1;
+    const char* p = static_cast<const char*>(recs);
+    const bool ok = parallel_slices(n, w->rb, [&](uint64_t b, uint64_t e, uint32_t) {
+        return pwrite_all(w->fd, p + b * w->rb, (e - b) * w->rb, (w->at + b) * w->rb);
+    });
+    w->at += n;
+    return ok ? 0 : -1;
 }
 
 struct PushCtx {
@@ -292,12 +331,12 @@ int kmc_write_outputs(kc_engine* h, const char* tmp_dir)
     ::close(lw.fd);
     if (rc) return rc;
     for (uint32_t r = 0; r < 256; ++r) {
-        int ctx[2];
-        ctx[0] = ::open((dir + "root_" + std::to_string(r) + ".bin").c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
-        ctx[1] = (int)lw.nw;
-        if (ctx[0] < 0) return KC_ERR_ARG;
-        rc = kc_emit_root(h, r, root_cb, ctx);
-        ::close(ctx[0]);
+        RootWriter rw;
+        rw.fd = ::open((dir + "root_" + std::to_string(r) + ".bin").c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        rw.rb = 8ULL * lw.nw + 8;
+        if (rw.fd < 0) return KC_ERR_ARG;
+        rc = kc_emit_root(h, r, root_cb, &rw);
+        ::close(rw.fd);
         if (rc) return rc;
     }
     return KC_OK;
