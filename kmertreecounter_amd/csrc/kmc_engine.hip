@@ -1,13 +1,17 @@
 // kmc_engine.hip -- host side of the engine: device memory, block staging,
 // the pipeline driver and the C ABI declared in include/kmc.h.
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../include/kmc.h"
@@ -96,13 +100,59 @@ struct kc_engine {
     uint64_t heavy_buckets = 0;
     // bucket-range rounds: each round's outputs in host memory (empty when the
     // results of a single round stay on the device)
+    // Host memory kept across runs: pinned (hipHostMalloc), so device results
+    // copy into it at full link speed; pageable with huge pages if pinning fails.
+    struct HostBuf {
+        uint8_t* p = nullptr;
+        uint64_t cap = 0;
+        bool pinned = false;
+        uint64_t touched = 0;  // leading bytes already written once (their pages are mapped)
+        HostBuf() = default;
+        HostBuf(const HostBuf&) = delete;
+        HostBuf(HostBuf&& o) noexcept : p(o.p), cap(o.cap), pinned(o.pinned), touched(o.touched)
+        {
+            o.p = nullptr;
+            o.cap = 0;
+            o.touched = 0;
+        }
+        ~HostBuf() { release(); }
+        void release()
+        {
+            if (p && pinned) (void)hipHostFree(p);
+            else std::free(p);
+            p = nullptr;
+            cap = 0;
+            touched = 0;
+        }
+        bool ensure(uint64_t n)
+        {
+            if (n <= cap) return true;
+            release();
+            constexpr uint64_t kAlign = 2ULL << 20;
+            const uint64_t sz = std::max<uint64_t>(kAlign, (n + kAlign - 1) / kAlign * kAlign);
+            void* q = nullptr;
+            if (hipHostMalloc(&q, sz, hipHostMallocDefault) == hipSuccess && q) {
+                p = static_cast<uint8_t*>(q);
+                pinned = true;
+            } else {
+                (void)hipGetLastError();  // clear the failed pinning before the pageable fallback
+                p = static_cast<uint8_t*>(std::aligned_alloc(kAlign, sz));
+                pinned = false;
+                if (p) madvise(p, sz, MADV_HUGEPAGE);
+            }
+            cap = p ? sz : 0;
+            return p != nullptr;
+        }
+    };
     struct RoundOut {
-        std::unique_ptr<uint64_t[]> firsts;  // n_first x NW words (not zero-filled before the copy)
-        std::unique_ptr<uint8_t[]> recs;     // n_rec records
+        HostBuf firsts;  // n_first x NW words
+        HostBuf recs;    // n_rec records
         uint64_t n_first = 0, n_rec = 0;
         std::vector<unsigned long long> root_off;  // 2 x 257, as root_off
+        const uint64_t* first_words() const { return reinterpret_cast<const uint64_t*>(firsts.p); }
     };
-    std::vector<RoundOut> rounds;
+    std::vector<RoundOut> rounds;  // the first rounds_used hold the last run's outputs
+    uint32_t rounds_used = 0;
     uint32_t n_rounds = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured single-round pipeline (kc_config.graph)
     GraphKey graph_key{};
@@ -291,10 +341,55 @@ uint32_t heavy_chunk_keys(const kc_engine* h)
     return (uint32_t)std::min<uint64_t>(1ULL << 18, std::max<uint64_t>(1024, c));
 }
 
+// Device results to a host buffer kept across runs.  Pinned buffers, and
+// pageable ones whose pages were written before, take one copy (measured 53
+// and 17 GB/s).  Into fresh pageable pages a plain copy ran at 2.8 GB/s (page
+// faults), so those go in 64 MiB chunks through the two pinned staging buffers,
+// chunk c + 1 copied from HBM while 8 threads copy chunk c out (faulting the
+// pages in parallel).  The staging buffers are idle once pushes are flushed.
+int copy_to_host(kc_engine* h, kc_engine::HostBuf& hb, const void* src, uint64_t bytes)
+{
+    if (bytes == 0) return KC_OK;
+    void* dst = hb.p;
+    if (hb.pinned || hb.touched >= bytes) {
+        HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->st));
+        HIPX(hipStreamSynchronize(h->st));
+        return KC_OK;
+    }
+    hb.touched = std::max(hb.touched, bytes);
+    constexpr uint32_t kThreads = 8;
+    char* buf[2] = {h->h_stage[0], h->h_stage[1]};
+    const uint64_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](uint64_t c) -> hipError_t {
+        const uint64_t off = c * kStageBytes, m = std::min<uint64_t>(kStageBytes, bytes - off);
+        hipError_t e = hipMemcpyAsync(buf[c & 1], static_cast<const char*>(src) + off, m, hipMemcpyDeviceToHost, h->st);
+        if (e == hipSuccess) e = hipEventRecord(h->emit_ev[c & 1], h->st);
+        return e;
+    };
+    HIPX(issue(0));
+    for (uint64_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks) HIPX(issue(c + 1));
+        HIPX(hipEventSynchronize(h->emit_ev[c & 1]));
+        const uint64_t off = c * kStageBytes, m = std::min<uint64_t>(kStageBytes, bytes - off);
+        const uint64_t per = (m + kThreads - 1) / kThreads;
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < kThreads; ++t) {
+            const uint64_t b = std::min(m, t * per), e = std::min(m, b + per);
+            if (e > b)
+                th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + off + b, buf[c & 1] + b, e - b); });
+        }
+        for (auto& x : th) x.join();
+    }
+    return KC_OK;
+}
+
 int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 {
     const uint64_t kb = 8ULL * h->nw;
     if (h->key_cap < padded_len) {
+        if (std::getenv("KMC_TIMING"))
+            std::fprintf(stderr, "[kmc] key buffers %llu -> %llu keys\n", (unsigned long long)h->key_cap,
+                         (unsigned long long)padded_len);
         free_work(h);
         if (hipMalloc(&h->d_A, padded_len * kb) != hipSuccess || hipMalloc(&h->d_B, padded_len * kb) != hipSuccess ||
             hipMalloc(&h->d_R, padded_len * kb) != hipSuccess) {
@@ -402,9 +497,13 @@ int run_pipeline_t(kc_engine* h)
     const uint32_t sbits = choose_sbits(h, padded);
     const uint32_t nf = 256u << sbits;
     const uint64_t cap = round_capacity<NW>(h, padded > 0 ? padded : tile);
-    int rc = ensure_work(h, std::min<uint64_t>(cap, padded > 0 ? padded : tile), nf);
+    uint64_t want = std::min<uint64_t>(cap, padded > 0 ? padded : tile);
+    // when rounds are needed anyway, buffers within 10% of what fits are kept:
+    // reallocating hundreds of GB for a slightly larger round costs seconds
+    if (want < padded && h->key_cap < want && h->key_cap * 10 >= want * 9) want = h->key_cap;
+    int rc = ensure_work(h, want, nf);
     if (rc) return rc;
-    h->rounds.clear();
+    h->rounds_used = 0;
     h->n_rounds = 1;
 
     const bool learned = h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
@@ -457,22 +556,35 @@ int run_pipeline_t(kc_engine* h)
     ranges.emplace_back(start, 256u);
     uint64_t K = 0, nfirst = 0, nrec = 0, spilled = 0, maxb = 0, dist = 0, xp = 0, heavy = 0;
     float phase[7] = {};
+    // KMC_TIMING=1: host wall time of each round's count and output copy on stderr
+    const bool timing = std::getenv("KMC_TIMING") != nullptr;
+    auto t_last = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what, uint32_t i) {
+        if (!timing) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[kmc] round %u %s %.1f ms\n", i, what,
+                     std::chrono::duration<double, std::milli>(now - t_last).count());
+        t_last = now;
+    };
+    lap("ranges", 0);
     for (const auto& r : ranges) {
         rt.rlo = r.first;
         rt.rhi = r.second;
         rc = run_round_t<NW>(h, rt, sbits, nf, padded);
         if (rc) return rc;
-        kc_engine::RoundOut out;
+        lap("count", h->rounds_used);
+        if (h->rounds.size() <= h->rounds_used) h->rounds.emplace_back();
+        kc_engine::RoundOut& out = h->rounds[h->rounds_used];
         out.n_first = h->n_first;
         out.n_rec = h->n_rec;
-        out.firsts.reset(new (std::nothrow) uint64_t[std::max<uint64_t>(1, h->n_first * NW)]);
-        out.recs.reset(new (std::nothrow) uint8_t[std::max<uint64_t>(1, h->n_rec * rec_bytes(NW))]);
-        if (!out.firsts || !out.recs) return h->fail(KC_ERR_NOMEM, "host memory for a round's outputs");
+        if (!out.firsts.ensure(h->n_first * 8ULL * NW) || !out.recs.ensure(h->n_rec * rec_bytes(NW)))
+            return h->fail(KC_ERR_NOMEM, "host memory for a round's outputs");
         out.root_off = h->root_off;
-        if (h->n_first)
-            HIPX(hipMemcpy(out.firsts.get(), h->res_firsts, h->n_first * 8ULL * NW, hipMemcpyDeviceToHost));
-        if (h->n_rec) HIPX(hipMemcpy(out.recs.get(), h->res_recs, h->n_rec * rec_bytes(NW), hipMemcpyDeviceToHost));
-        h->rounds.push_back(std::move(out));
+        rc = copy_to_host(h, out.firsts, h->res_firsts, h->n_first * 8ULL * NW);
+        if (!rc) rc = copy_to_host(h, out.recs, h->res_recs, h->n_rec * rec_bytes(NW));
+        if (rc) return rc;
+        lap("copy", h->rounds_used);
+        ++h->rounds_used;
         K += h->K;
         nfirst += h->n_first;
         nrec += h->n_rec;
@@ -1016,11 +1128,12 @@ int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t kb = 8ULL * h->nw;
     const uint64_t per = kEmitChunk / kb;
-    if (!h->rounds.empty()) {  // bucket-range rounds: outputs in host memory
-        for (const auto& r : h->rounds) {
+    if (h->rounds_used) {  // bucket-range rounds: outputs in host memory
+        for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
+            const auto& r = h->rounds[ri];
             const uint64_t n = r.n_first;
             for (uint64_t i = 0; i < n; i += per) {
-                int rc = cb(user, r.firsts.get() + i * h->nw, std::min(per, n - i));
+                int rc = cb(user, r.first_words() + i * h->nw, std::min(per, n - i));
                 if (rc) return h->fail(rc, "keys callback returned an error");
             }
         }
@@ -1038,12 +1151,13 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t rb = rec_bytes(h->nw);
     const uint64_t per = kEmitChunk / rb;
-    if (!h->rounds.empty()) {  // bucket-range rounds: outputs in host memory
-        for (const auto& r : h->rounds) {
+    if (h->rounds_used) {  // bucket-range rounds: outputs in host memory
+        for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
+            const auto& r = h->rounds[ri];
             for (int part = 0; part < 2; ++part) {
                 const uint64_t b = r.root_off[part * 257 + root_id], e = r.root_off[part * 257 + root_id + 1];
                 for (uint64_t i = b; i < e; i += per) {
-                    int rc = cb(user, r.recs.get() + i * rb, std::min(per, e - i));
+                    int rc = cb(user, r.recs.p + i * rb, std::min(per, e - i));
                     if (rc) return h->fail(rc, "records callback returned an error");
                 }
             }
@@ -1149,14 +1263,15 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
     const bool precise = mode == KC_HISTOGRAM_PRECISE;
     std::fill(hist, hist + ((uint64_t)max_freq - min_freq + 1), (int64_t)0);
     const uint64_t buf_bytes = h->key_cap * 8ULL * h->nw;
-    if (h->rounds.empty())  // results in A and B; the record slices (R) are free
+    if (!h->rounds_used)  // results in A and B; the record slices (R) are free
         return histogram_pass(h, precise, min_freq, max_freq, h->res_recs, h->n_rec, h->res_firsts, h->n_first, h->d_R,
                               buf_bytes, hist);
     // bucket-range rounds: the key ranges are disjoint, so the rounds' histograms
     // add up; each round's outputs go back into R and A, and B holds the table
-    for (const auto& r : h->rounds) {
-        if (r.n_rec) HIPX(hipMemcpy(h->d_R, r.recs.get(), r.n_rec * rec_bytes(h->nw), hipMemcpyHostToDevice));
-        if (r.n_first) HIPX(hipMemcpy(h->d_A, r.firsts.get(), r.n_first * 8ULL * h->nw, hipMemcpyHostToDevice));
+    for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
+        const auto& r = h->rounds[ri];
+        if (r.n_rec) HIPX(hipMemcpy(h->d_R, r.recs.p, r.n_rec * rec_bytes(h->nw), hipMemcpyHostToDevice));
+        if (r.n_first) HIPX(hipMemcpy(h->d_A, r.firsts.p, r.n_first * 8ULL * h->nw, hipMemcpyHostToDevice));
         const int rc = histogram_pass(h, precise, min_freq, max_freq, reinterpret_cast<const uint8_t*>(h->d_R), r.n_rec,
                                       h->d_A, r.n_first, h->d_B, buf_bytes, hist);
         if (rc) return rc;
