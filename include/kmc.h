@@ -90,7 +90,7 @@ typedef struct kc_config {
     uint32_t min_count;       /* root record filter on the repeat count, inclusive */
     uint32_t max_count;       /*   (append_export_record, src/NewKmerTree.h:1159-1168); 0 = UINT32_MAX */
     uint64_t expected_bytes;  /* hint: total sequence bytes to be pushed (0 = grow on demand) */
-    uint32_t sub_bits;        /* fine fan-out bits below the root prefix: 0 = automatic, 1..8,
+    uint32_t sub_bits;        /* fine fan-out bits below the root prefix: 0 = automatic, 1..10,
                                  or KC_SUB_BITS_NONE for no sub-partition */
     uint32_t device;          /* HIP device ordinal */
     uint32_t classifier_num;  /* LPT owner count reported by kc_plan (reference: (n-2-n/8)/2) */

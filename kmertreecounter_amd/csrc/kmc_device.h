@@ -210,7 +210,8 @@ __device__ __forceinline__ uint64_t bloom_mask(uint64_t lo, uint64_t hi)
 //           learned CDF model (bucket = number of boundaries <= key).
 // learned routing tables: coarse table, 256 fine tables, 256 shifts
 constexpr uint32_t kLutCells = 4096;
-constexpr size_t kRouteLutBytes = kLutCells + 256 * (size_t)kLutCells + 256;
+// fine table entries are u8 up to sbits 8, u16 above (layout in kmc_learned.h)
+constexpr size_t kRouteLutBytes = kLutCells + 256 * (size_t)kLutCells * 2 + 256;
 
 struct Route {
     uint32_t mode;   // 0 prefix, 1 learned

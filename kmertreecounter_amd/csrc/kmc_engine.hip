@@ -192,7 +192,11 @@ uint64_t ext_tile(uint32_t nw)
 {
     return (nw == 1 ? ExtractCfg<1>::kThreads : nw == 2 ? ExtractCfg<2>::kThreads : ExtractCfg<4>::kThreads) * 16ULL;
 }
-uint64_t sub_tile(uint32_t nw) { return nw == 1 ? SubCfg<1>::kTile : nw == 2 ? SubCfg<2>::kTile : SubCfg<4>::kTile; }
+uint64_t sub_tile(uint32_t nw, uint32_t sbits)
+{
+    if (sbits > 8) return nw == 1 ? SubWideCfg<1>::kTile : nw == 2 ? SubWideCfg<2>::kTile : SubWideCfg<4>::kTile;
+    return nw == 1 ? SubCfg<1>::kTile : nw == 2 ? SubCfg<2>::kTile : SubCfg<4>::kTile;
+}
 uint64_t rec_bytes(uint32_t nw) { return 8ULL * nw + 8; }
 // words of a key in the open-addressing tables (k <= 128 keys take five 63-bit words)
 uint64_t table_words(uint32_t nw) { return nw == 4 ? (uint64_t)TableWords<4>::value : nw; }
@@ -443,10 +447,16 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 uint32_t choose_sbits(const kc_engine* h, uint64_t padded_len)
 {
     if (h->cfg.sub_bits == KC_SUB_BITS_NONE) return 0;
-    if (h->cfg.sub_bits) return std::min<uint32_t>(8, h->cfg.sub_bits);
+    if (h->cfg.sub_bits) return std::min<uint32_t>(kMaxSubBits, h->cfg.sub_bits);
     const double per_coarse = (double)padded_len / 256.0;
     uint32_t sb = 0;
     while (sb < 8 && per_coarse / (double)(1u << sb) > (double)kFineTarget) sb++;
+    // Past 256 fine buckets per coarse bucket the sub-partition scatter writes
+    // short runs and runs at about half speed, so it is only worth it where the
+    // counting kernel would otherwise pass over most buckets several times
+    // (measured: workload 4 -> 9 bits, workload 5 -> 10 bits, workload 2 stays at 8).
+    const double pass_keys = h->nw == 1 ? CountCfg<1>::kPassKeys : h->nw == 2 ? CountCfg<2>::kPassKeys : CountCfg<4>::kPassKeys;
+    while (sb >= 8 && sb < kMaxSubBits && per_coarse / (double)(1u << sb) > pass_keys) sb++;
     return sb;
 }
 
@@ -510,7 +520,7 @@ int run_pipeline_t(kc_engine* h)
     if (learned && h->bounds_sbits != sbits) {
         std::vector<uint64_t> rb;
         routing_bounds(h->plan.learned_b, sbits, rb);
-        if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * 255)));
+        if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * ((1u << kMaxSubBits) - 1))));
         HIPX(hipMemcpy(h->d_bounds, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice));
         std::vector<uint8_t> lut;
         routing_luts(rb, sbits, lut);
@@ -648,13 +658,13 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
             HIPX(mark(0));
             HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
             HIPX(mark(1));
-            HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st, rt.rlo, rt.rhi));
+            HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW, sbits), st, rt.rlo, rt.rhi));
             HIPX(mark(2));
             HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
             HIPX(mark(3));
             const unsigned long long* fine_off = coarse_off;
             if (sbits) {
-                const uint64_t max_tiles = padded / sub_tile(NW) + 256;
+                const uint64_t max_tiles = padded / sub_tile(NW, sbits) + 256;
                 HIPX(mark(4));
                 HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
                 HIPX(mark(5));
@@ -873,7 +883,7 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     *out = nullptr;
     const uint32_t nw = kc_words_for_k(cfg->k);
     if (!nw) return KC_ERR_ARG;
-    if (cfg->partition > 1 || (cfg->sub_bits > 8 && cfg->sub_bits != KC_SUB_BITS_NONE)) return KC_ERR_ARG;
+    if (cfg->partition > 1 || (cfg->sub_bits > kMaxSubBits && cfg->sub_bits != KC_SUB_BITS_NONE)) return KC_ERR_ARG;
     if (cfg->counter > KC_COUNTER_EXACT || (cfg->table_slots && cfg->table_slots < 16)) return KC_ERR_ARG;
     if (cfg->max_count && cfg->min_count > cfg->max_count) return KC_ERR_ARG;
     int ndev = 0;

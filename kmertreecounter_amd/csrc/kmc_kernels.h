@@ -27,6 +27,10 @@ template <int NW> struct SubCfg;
 template <> struct SubCfg<1> { static constexpr int kThreads = 512; static constexpr int kTile = 8192; };
 template <> struct SubCfg<2> { static constexpr int kThreads = 256; static constexpr int kTile = 4096; };
 template <> struct SubCfg<4> { static constexpr int kThreads = 256; static constexpr int kTile = 2048; };
+// sbits 9 and 10 (512 or 1024 fine buckets per coarse bucket): u16 bucket
+// indexes and larger histograms, so smaller tiles keep two workgroups per CU
+constexpr uint32_t kMaxSubBits = 10;
+template <int NW> struct SubWideCfg { static constexpr int kThreads = 256; static constexpr int kTile = 4096 / NW; };
 
 // P3: one workgroup per fine bucket (1 per CU).  LDS: Bloom words + count
 // table; buckets above kPassKeys keys are processed in hash-range passes.
@@ -133,7 +137,7 @@ hipError_t launch_scan_coarse(const unsigned long long* counts, unsigned long lo
                               unsigned long long* tile_start, uint32_t sub_tile, hipStream_t st,
                               uint32_t rlo = 0, uint32_t rhi = 256);
 // fine bucket offsets and cursors; part: kScanGroups scratch words
-constexpr uint32_t kScanGroups = 64;  // up to 65536 fine buckets
+constexpr uint32_t kScanGroups = 256;  // up to 262144 fine buckets (sbits 10)
 hipError_t launch_scan_fine(const unsigned long long* counts, uint32_t nf, unsigned long long* fine_off,
                             unsigned long long* cursor, unsigned long long* part, hipStream_t st);
 template <int NW>

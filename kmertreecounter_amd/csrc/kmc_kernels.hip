@@ -23,6 +23,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kmc_device.h"
 #include "kmc_kernels.h"
@@ -114,8 +115,10 @@ __device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s
 }
 
 // Learned fine bucket inside coarse bucket r (keys lo <= w0): the table of r
-// is indexed by (w0 - lo) >> shift, then bounds inside the cell are stepped over.
-__device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_fb, const uint8_t* s_lut, uint64_t lo,
+// is indexed by (w0 - lo) >> shift, then bounds inside the cell are stepped
+// over.  Table entries are u8 up to 256 fine buckets, u16 above.
+template <typename E>
+__device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_fb, const E* s_lut, uint64_t lo,
                                             uint32_t shift, uint64_t w0)
 {
     if (rt.sbits == 0) return 0;
@@ -410,30 +413,36 @@ k_scan_fine_add(const unsigned long long* __restrict__ counts, uint32_t nf, unsi
 
 // ------------------------------------------------------------------ P2
 
-// Tiles of SubCfg::kTile keys inside one coarse bucket of A.
-template <int NW, bool SCATTER>
-__global__ void __launch_bounds__(SubCfg<NW>::kThreads)
+// Tiles of kTile keys inside one coarse bucket of A.  WIDE: up to 1024 fine
+// buckets per coarse bucket (sbits 9 and 10), with u16 bucket indexes and
+// fine tables (SubWideCfg); otherwise up to 256.
+template <int NW, bool SCATTER, bool WIDE>
+__global__ void __launch_bounds__(WIDE ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads)
 k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coarse_off,
       const unsigned long long* __restrict__ tile_start, Route rt,
       unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
       uint64_t* __restrict__ B)
 {
-    constexpr int T = SubCfg<NW>::kThreads;
-    constexpr int TILE = SubCfg<NW>::kTile;
+    constexpr int T = WIDE ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads;
+    constexpr int TILE = WIDE ? SubWideCfg<NW>::kTile : SubCfg<NW>::kTile;
     constexpr int PER = TILE / T;
+    constexpr uint32_t NB = WIDE ? 1024u : 256u;       // fine buckets per coarse bucket, at most
+    constexpr uint32_t E = (NB + T - 1) / T;           // histogram entries per thread in the scan
+    using LutT = typename std::conditional<WIDE, uint16_t, uint8_t>::type;
+    using BktT = typename std::conditional<WIDE, uint16_t, uint8_t>::type;
     // fine bounds and the bucket's fine table are dead once every thread has
     // routed its keys (a barrier later): they share LDS with the staged keys
-    constexpr size_t kStage = 256 * sizeof(uint64_t) + kLutCells;
+    constexpr size_t kStage = NB * sizeof(uint64_t) + kLutCells * sizeof(LutT);
     constexpr size_t kKeys = SCATTER ? TILE * sizeof(Key<NW>) : 0;
     __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
     uint64_t* s_fb = reinterpret_cast<uint64_t*>(s_raw);
-    uint8_t* s_lut = s_raw + 256 * sizeof(uint64_t);
+    LutT* s_lut = reinterpret_cast<LutT*>(s_raw + NB * sizeof(uint64_t));
     Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
-    __shared__ uint32_t s_hist[256];
-    __shared__ uint32_t s_off[256];
-    __shared__ unsigned long long s_gbase[256];
+    __shared__ uint32_t s_hist[NB];
+    __shared__ uint32_t s_off[NB];
+    __shared__ unsigned long long s_gbase[NB];
     __shared__ uint32_t s_scan[T / kWave + 1];
-    __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
+    __shared__ BktT s_bkt[SCATTER ? TILE : 1];
 
     const int t = threadIdx.x;
     const unsigned long long tid = blockIdx.x;
@@ -453,16 +462,16 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     const uint32_t nsub = 1u << rt.sbits;
     const uint32_t nf = 256u << rt.sbits;
 
-    if (t < 256) s_hist[t] = 0;
+    for (uint32_t i = t; i < NB; i += T) s_hist[i] = 0;
     uint64_t flo = 0;
     uint32_t shift = 0;
     if (rt.mode == 1 && rt.sbits) {
-        if (t < 255) s_fb[t] = t < (int)(nsub - 1) ? rt.fine_bounds[(size_t)r * (nsub - 1) + t] : ~0ULL;
-        const uint8_t* tab = rt.lut + kLutCells + (size_t)r * kLutCells;
-        for (int i = t; i < (int)(kLutCells / 16); i += T)
+        for (uint32_t i = t; i + 1 < NB; i += T) s_fb[i] = i + 1 < nsub ? rt.fine_bounds[(size_t)r * (nsub - 1) + i] : ~0ULL;
+        const uint8_t* tab = rt.lut + kLutCells + (size_t)r * kLutCells * sizeof(LutT);
+        for (int i = t; i < (int)(kLutCells * sizeof(LutT) / 16); i += T)
             reinterpret_cast<uint4*>(s_lut)[i] = reinterpret_cast<const uint4*>(tab)[i];
         flo = r ? rt.coarse_bounds[r - 1] : 0;
-        shift = rt.lut[kLutCells + 256 * (size_t)kLutCells + r];
+        shift = rt.lut[kLutCells + 256 * (size_t)kLutCells * sizeof(LutT) + r];
     }
     __syncthreads();
 
@@ -486,18 +495,31 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     __syncthreads();
 
     if constexpr (!SCATTER) {
-        if (t < (int)nsub) {
-            const uint32_t h = s_hist[t];
-            if (h) atomicAdd(&counts[(size_t)xcd * nf + ((size_t)r << rt.sbits) + t], (unsigned long long)h);
+        for (uint32_t i = t; i < nsub; i += T) {
+            const uint32_t h = s_hist[i];
+            if (h) atomicAdd(&counts[(size_t)xcd * nf + ((size_t)r << rt.sbits) + i], (unsigned long long)h);
         }
         return;
     } else {
-        uint32_t h = (t < (int)nsub) ? s_hist[t] : 0u;
+        // entries [E t, E t + E) per thread: local sums, then one workgroup scan
+        uint32_t h[E];
+        uint32_t mine = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < E; ++q) {
+            const uint32_t i = E * t + q;
+            h[q] = i < nsub ? s_hist[i] : 0u;
+            mine += h[q];
+        }
         uint32_t total;
-        const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
-        if (t < (int)nsub) {
-            s_off[t] = off;
-            if (h) s_gbase[t] = atomicAdd(&cursor[(size_t)xcd * nf + ((size_t)r << rt.sbits) + t], (unsigned long long)h);
+        uint32_t off = block_exclusive_scan<uint32_t>(mine, s_scan, &total);
+#pragma unroll
+        for (uint32_t q = 0; q < E; ++q) {
+            const uint32_t i = E * t + q;
+            if (i < nsub) {
+                s_off[i] = off;
+                if (h[q]) s_gbase[i] = atomicAdd(&cursor[(size_t)xcd * nf + ((size_t)r << rt.sbits) + i], (unsigned long long)h[q]);
+            }
+            off += h[q];
         }
         __syncthreads();
 #pragma unroll
@@ -506,7 +528,7 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
             if (idx < n) {
                 const uint32_t slot = s_off[sb[j]] + rank[j];
                 s_keys[slot] = keys[j];
-                s_bkt[slot] = (uint8_t)sb[j];
+                s_bkt[slot] = (BktT)sb[j];
             }
         }
         __syncthreads();
@@ -1407,12 +1429,18 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
                       const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,
                       unsigned long long* cursor, uint64_t* B, hipStream_t st)
 {
-    constexpr int T = SubCfg<NW>::kThreads;
     if (max_tiles == 0) return hipSuccess;
-    if (scatter)
-        hipLaunchKernelGGL((k_sub<NW, true>), dim3((uint32_t)max_tiles), dim3(T), 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
-    else
-        hipLaunchKernelGGL((k_sub<NW, false>), dim3((uint32_t)max_tiles), dim3(T), 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+    if (rt.sbits > kMaxSubBits) return hipErrorInvalidValue;
+    const dim3 g((uint32_t)max_tiles);
+    if (rt.sbits > 8) {
+        const dim3 b(SubWideCfg<NW>::kThreads);
+        if (scatter) hipLaunchKernelGGL((k_sub<NW, true, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+        else hipLaunchKernelGGL((k_sub<NW, false, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+    } else {
+        const dim3 b(SubCfg<NW>::kThreads);
+        if (scatter) hipLaunchKernelGGL((k_sub<NW, true, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+        else hipLaunchKernelGGL((k_sub<NW, false, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+    }
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -79,7 +79,8 @@ void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<
 // Lookup tables that start each routing search next to its answer:
 // [0, 4096): coarse table, entry p = number of coarse bounds <= p << 52;
 // then per coarse bucket r a 4096-entry fine table over r's key range
-// [lo_r, hi_r): entry c = number of r's fine bounds <= lo_r + (c << shift_r);
+// [lo_r, hi_r): entry c = number of r's fine bounds <= lo_r + (c << shift_r)
+// (u8 entries up to 256 fine buckets, little-endian u16 above);
 // then the 256 shifts.  `rb` is the output of routing_bounds.
 void routing_luts(const std::vector<uint64_t>& rb, uint32_t sbits, std::vector<uint8_t>& lut);
 

@@ -786,7 +786,8 @@ void routing_luts(const std::vector<uint64_t>& rb, uint32_t sbits, std::vector<u
     const uint64_t* cb = rb.data();
     for (uint32_t p = 0; p < kLutCells; ++p)
         lut[p] = (uint8_t)(std::upper_bound(cb, cb + 255, (uint64_t)p << 52) - cb);
-    uint8_t* shifts = lut.data() + kLutCells + 256 * (size_t)kLutCells;
+    const size_t esz = F > 256 ? 2 : 1;  // fine table entry bytes
+    uint8_t* shifts = lut.data() + kLutCells + 256 * (size_t)kLutCells * esz;
     for (uint32_t r = 0; r < 256; ++r) {
         // keys of coarse bucket r: lo <= w0 <= last
         const uint64_t lo = r ? cb[r - 1] : 0;
@@ -795,15 +796,20 @@ void routing_luts(const std::vector<uint64_t>& rb, uint32_t sbits, std::vector<u
         const uint32_t bw = span ? 64 - (uint32_t)__builtin_clzll(span) : 0;
         const uint32_t shift = bw > 12 ? bw - 12 : 0;
         shifts[r] = (uint8_t)shift;
-        uint8_t* t = lut.data() + kLutCells + (size_t)r * kLutCells;
+        uint8_t* t = lut.data() + kLutCells + (size_t)r * kLutCells * esz;
         if (F == 1) continue;
         const uint64_t* fb = rb.data() + 255 + (size_t)r * (F - 1);
         for (uint32_t c = 0; c < kLutCells; ++c) {
             const uint64_t off = (uint64_t)c << shift;
+            uint32_t v = F - 1;  // cell past the bucket's last key: never looked up
             if (shift < 64 && (off >> shift) == c && off <= span)
-                t[c] = (uint8_t)(std::upper_bound(fb, fb + (F - 1), lo + off) - fb);
-            else
-                t[c] = (uint8_t)(F - 1);  // cell past the bucket's last key: never looked up
+                v = (uint32_t)(std::upper_bound(fb, fb + (F - 1), lo + off) - fb);
+            if (esz == 2) {
+                const uint16_t v16 = (uint16_t)v;
+                std::memcpy(t + 2 * (size_t)c, &v16, 2);
+            } else {
+                t[c] = (uint8_t)v;
+            }
         }
     }
 }

@@ -88,13 +88,36 @@ def test_empty_and_degenerate_inputs():
             check_outputs(f, r, s, 31, keys, counts)
 
 
-@pytest.mark.parametrize("sub_bits", [K.KC_SUB_BITS_NONE, 3, 8])
+@pytest.mark.parametrize("sub_bits", [K.KC_SUB_BITS_NONE, 3, 8, 9, 10])
 def test_fanout_does_not_change_table(sub_bits):
     seq = K.generate(1, scale=20, seed=3, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
     f, r, s = run_engine([seq], 31, sub_bits=sub_bits)
     assert s.fine_buckets == 256 << (0 if sub_bits == K.KC_SUB_BITS_NONE else sub_bits)
     check_outputs(f, r, s, 31, keys, counts)
+
+
+@pytest.mark.parametrize("k", [31, 63, 100])
+@pytest.mark.parametrize("sub_bits", [9, 10])
+def test_wide_fanout_with_learned_routing(k, sub_bits):
+    # 512 or 1024 fine buckets per coarse bucket: u16 bucket indexes and fine
+    # tables in the sub-partition kernel, for both partitions
+    seq = K.generate(2, scale=200, seed=5 + k, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, k))
+    for part in (K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED):
+        with K.Engine(k, partition=part, sub_bits=sub_bits) as e:
+            if part == K.KC_PARTITION_LEARNED:
+                e.sample_block(seq[: len(seq) // 5])
+                e.plan()
+            e.push_block(seq)
+            e.finish()
+            f, r, s = e.firsts(), e.records(), e.stats()
+            assert s.fine_buckets == 256 << sub_bits
+            check_outputs(f, r, s, k, keys, counts)
+            for rid in (0, 99, 255):
+                rr = e.root(rid)
+                if len(rr):
+                    assert (rr["key"][:, 0] >> np.uint64(56) == rid).all()
 
 
 def test_block_carry_path_matches():
