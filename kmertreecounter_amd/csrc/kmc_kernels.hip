@@ -110,7 +110,13 @@ __device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s
 {
     if (rt.mode == 0) return (uint32_t)(w0 >> 56);
     uint32_t b = s_lut[w0 >> 52];
-    while (b < 255 && s_cb[b] <= w0) ++b;
+    // the first bound check without a loop, so the checks of a thread's keys
+    // overlap; only a key past that bound steps on (a cell rarely holds two)
+    const bool past = b < 255 && s_cb[b < 255 ? b : 254] <= w0;
+    if (past) {
+        ++b;
+        while (b < 255 && s_cb[b] <= w0) ++b;
+    }
     return b;
 }
 
@@ -125,7 +131,11 @@ __device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_f
     const uint32_t nb = (1u << rt.sbits) - 1u;
     if (rt.mode == 0) return (uint32_t)((w0 >> (56 - rt.sbits)) & nb);
     uint32_t f = s_lut[(w0 - lo) >> shift];
-    while (f < nb && s_fb[f] <= w0) ++f;
+    const bool past = f < nb && s_fb[f < nb ? f : nb - 1] <= w0;
+    if (past) {
+        ++f;
+        while (f < nb && s_fb[f] <= w0) ++f;
+    }
     return f;
 }
 
