@@ -285,19 +285,27 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     uint32_t bk[16];
     uint32_t rank[16];
     uint32_t okm = 0;
+    if constexpr (SCATTER) {
+        // every key routed first, then the histogram atomics: the routing table
+        // reads of a thread's 16 keys overlap instead of waiting behind atomics
+        // (measured faster in the scatter, slower in the count pass)
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        Key<NW> key;
-        const bool ok = window_key<NW>(c, v, i, k, key);
-        keys[i] = key;
-        if (ok) {
-            const uint32_t b = coarse_of(rt, s_cb, s_lut, key.w[0]);
+        for (int i = 0; i < 16; ++i) {
+            Key<NW> key;
+            const bool ok = window_key<NW>(c, v, i, k, key);
+            keys[i] = key;
+            bk[i] = ok ? coarse_of(rt, s_cb, s_lut, key.w[0]) : 0u;
             // the scatter of a bucket-range round keeps its own buckets only
-            if (!SCATTER || (b >= rt.rlo && b < rt.rhi)) {
-                okm |= 1u << i;
-                bk[i] = b;
-                rank[i] = atomicAdd(&s_hist[b], 1u);
-            }
+            if (ok && bk[i] >= rt.rlo && bk[i] < rt.rhi) okm |= 1u << i;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (okm & (1u << i)) rank[i] = atomicAdd(&s_hist[bk[i]], 1u);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            Key<NW> key;
+            if (window_key<NW>(c, v, i, k, key)) atomicAdd(&s_hist[coarse_of(rt, s_cb, s_lut, key.w[0])], 1u);
         }
     }
     __syncthreads();
@@ -494,13 +502,17 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
         const uint32_t idx = t + j * T;
         if (idx < n) keys[j] = load_key<NW>(A, beg + idx);
     }
+    if constexpr (SCATTER) {
+        // routed first, then the atomics (as in k_extract's scatter)
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        const uint32_t idx = t + j * T;
-        if (idx < n) {
-            sb[j] = fine_of(rt, s_fb, s_lut, flo, shift, keys[j].w[0]);
-            rank[j] = atomicAdd(&s_hist[sb[j]], 1u);
-        }
+        for (int j = 0; j < PER; ++j) sb[j] = (t + j * T < n) ? fine_of(rt, s_fb, s_lut, flo, shift, keys[j].w[0]) : 0u;
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (t + j * T < n) rank[j] = atomicAdd(&s_hist[sb[j]], 1u);
+    } else {
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (t + j * T < n) atomicAdd(&s_hist[fine_of(rt, s_fb, s_lut, flo, shift, keys[j].w[0])], 1u);
     }
     __syncthreads();
 
