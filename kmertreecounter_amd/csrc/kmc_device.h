@@ -193,15 +193,6 @@ __device__ __forceinline__ Key<NW> from_table(const Key<TableWords<NW>::value>& 
     }
 }
 
-// Bloom probe rule of ConcurrentBloomFilter (src/BloomFilter.h:207-214,
-// :248-258): word = low64 & (cap-1); three bits (low64 + i*(high64|1)) & 63.
-// The counting kernels feed it two words of their own 64-bit key hash.
-__device__ __forceinline__ uint64_t bloom_mask(uint64_t lo, uint64_t hi)
-{
-    const uint64_t h2 = hi | 1ULL;
-    return (1ULL << (lo & 63)) | (1ULL << ((lo + h2) & 63)) | (1ULL << ((lo + 2 * h2) & 63));
-}
-
 // ---------------------------------------------------------------- routing
 // Coarse bucket (0..255) and fine sub-bucket (0..2^sbits-1).
 //   prefix: coarse = top 8 key bits (the reference's root prefix, ROOT_BASES=4,

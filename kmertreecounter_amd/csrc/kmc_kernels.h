@@ -40,7 +40,7 @@ template <int NW> struct CountCfg;
 // CU measured slower: the extra passes cost more than the occupancy gains.)
 template <> struct CountCfg<1> {
     static constexpr int kThreads = 1024;
-    static constexpr uint32_t kBloomWords = 4096;
+    static constexpr uint32_t kBloomWords = 8192;  // 32-bit filter words (32 KiB)
     static constexpr uint32_t kTableSlots = 8192;
     static constexpr uint32_t kPassKeys = 98304;
     static constexpr uint32_t kPassDistinct = 5000;  // target distinct keys per pass (61% table load)
@@ -50,7 +50,7 @@ template <> struct CountCfg<1> {
 };
 template <> struct CountCfg<2> {
     static constexpr int kThreads = 1024;
-    static constexpr uint32_t kBloomWords = 4096;
+    static constexpr uint32_t kBloomWords = 8192;  // 32-bit filter words (32 KiB)
     static constexpr uint32_t kTableSlots = 4096;
     static constexpr uint32_t kPassKeys = 49152;
     static constexpr uint32_t kPassDistinct = 2500;
@@ -61,7 +61,7 @@ template <> struct CountCfg<2> {
 // k 65..128: a table slot is five 63-bit words plus its count (44 bytes)
 template <> struct CountCfg<4> {
     static constexpr int kThreads = 1024;
-    static constexpr uint32_t kBloomWords = 4096;
+    static constexpr uint32_t kBloomWords = 8192;  // 32-bit filter words (32 KiB)
     static constexpr uint32_t kTableSlots = 2048;
     static constexpr uint32_t kPassKeys = 24576;
     static constexpr uint32_t kPassDistinct = 1250;
