@@ -152,3 +152,26 @@ def test_tree_run_parses_on_the_device_like_the_host_reader(tmp_path, gz):
     for got_k, got_c in tables:
         np.testing.assert_array_equal(got_k, keys)
         np.testing.assert_array_equal(got_c, counts)
+
+
+@pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT], ids=["bloom", "exact"])
+def test_device_parsing_with_rounds_graph_and_filters(counter):
+    # a parsed input counted in bucket-range rounds, then replayed as a graph,
+    # with the record filter on: the merged table stays the oracle's
+    fq = K.generate(2, scale=200, seed=17)
+    keys, counts = C.count_table(C.extract(sequence_lines(fq), 31))
+    with K.Engine(31, counter=counter, max_round_keys=600_000) as e:
+        push_in_chunks(e, fq, range(1 << 20, len(fq), 1 << 20))
+        e.finish()
+        s = e.stats()
+        assert s.rounds >= 2
+        check_outputs(e.firsts(), e.records(), s, 31, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+    with K.Engine(31, counter=counter, graph=True, min_count=2, max_count=50) as e:
+        push_in_chunks(e, fq, range(1 << 20, len(fq), 1 << 20))
+        e.finish()
+        for _ in range(2):
+            e.count_resident()
+            r, s = e.records(), e.stats()
+            assert len(r) > 0 and ((r["count"] >= 2) & (r["count"] <= 50)).all()
+            assert s.records == len(r) <= s.records_unfiltered
+            assert s.total_kmers == int(counts.sum())
