@@ -1,6 +1,6 @@
 // kmc_device.h -- device-side building blocks shared by the engine kernels:
-// base encoding, canonical k-mer construction, XXH3-128 for 8/16/32-byte keys,
-// Bloom probe and key routing.  gfx950 (CDNA4), wave64.
+// base encoding, canonical k-mer construction, table key encoding and
+// routing tables.  gfx950 (CDNA4), wave64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,32 +21,54 @@ struct Key {
 // ASCII -> 2-bit code: A/a=0, C/c=1, G/g=2, T/t/U/u=3 (reference table,
 // src/GetKmer.h:22-36).  ((c >> 1) & 3) maps A,C,G,T to 0,1,3,2; xor with
 // its high bit swaps G and T.  Validity is the case-folded byte being one of
-// A C G T U (src/FastqParser.h:211-236); everything else resets a window.
-__device__ __forceinline__ void encode_word(uint32_t w, int first_idx, uint32_t& code, uint32_t& valid)
+// A C G T U (src/FastqParser.h:194-236); everything else resets a window.
+//
+// Four bytes per 32-bit operation: the codes of all four bytes at once, and
+// validity as "the case-folded byte is the letter its code stands for" (a
+// byte lookup of A C G T by code with v_perm_b32; for code 3, U is folded
+// onto T by clearing bit 0), then an exact zero-byte test.  Outputs: the
+// four codes in the low byte (first byte in bits 7..6) and a nibble of
+// invalid flags (first byte in bit 3).
+__device__ __forceinline__ void encode_word(uint32_t w, uint32_t& codes, uint32_t& bad)
 {
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const uint32_t ch = (w >> (8 * b)) & 0xFFu;
-        const uint32_t u = ch & 0xDFu;
-        const uint32_t ok = (u == 'A') | (u == 'C') | (u == 'G') | (u == 'T') | (u == 'U');
-        uint32_t cd = (ch >> 1) & 3u;
-        cd ^= cd >> 1;
-        const int idx = first_idx + b;
-        code |= cd << (30 - 2 * idx);
-        valid |= ok << (15 - idx);
-    }
+    const uint32_t t = (w >> 1) & 0x03030303u;
+    const uint32_t cd = t ^ ((t >> 1) & 0x01010101u);  // code of each byte, 0..3
+    codes = (cd << 6) | (cd >> 4) | (cd >> 14) | (cd >> 24);
+    constexpr uint32_t kLetters = 0x54474341u;  // 'T' 'G' 'C' 'A' in bytes 3..0
+    const uint32_t expect = __builtin_amdgcn_perm(kLetters, kLetters, cd);
+    const uint32_t u_as_t = cd & (cd >> 1) & 0x01010101u;  // bit 0 of code-3 bytes
+    const uint32_t d = (w & 0xDFDFDFDFu & ~u_as_t) ^ expect;  // zero byte: valid
+    const uint32_t nz = (((d & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | d) & 0x80808080u;
+    const uint32_t g = nz >> 7;  // bytes 0..3 -> bits 0, 8, 16, 24
+    bad = ((g << 3) | (g >> 6) | (g >> 15) | (g >> 24)) & 0xFu;
 }
 
 // 16 sequence bytes -> 16 codes (MSB-first, first byte in bits 31..30) and a
 // 16-bit validity mask (first byte in bit 15).
 __device__ __forceinline__ void encode16(uint4 q, uint32_t& code, uint32_t& valid)
 {
-    code = 0;
-    valid = 0;
-    encode_word(q.x, 0, code, valid);
-    encode_word(q.y, 4, code, valid);
-    encode_word(q.z, 8, code, valid);
-    encode_word(q.w, 12, code, valid);
+    uint32_t c0, c1, c2, c3, b0, b1, b2, b3;
+    encode_word(q.x, c0, b0);
+    encode_word(q.y, c1, b1);
+    encode_word(q.z, c2, b2);
+    encode_word(q.w, c3, b3);
+    // low bytes of c0..c3 into bytes 3..0 (v_perm_b32: selectors 4..7 pick the
+    // first operand's bytes, 0..3 the second's, 0x0C a zero byte)
+    code = __builtin_amdgcn_perm(c0, c1, 0x04000C0Cu) | __builtin_amdgcn_perm(c2, c3, 0x0C0C0400u);
+    valid = ~((b0 << 12) | (b1 << 8) | (b2 << 4) | b3) & 0xFFFFu;
+}
+
+// reverse the order of the 16 two-bit groups of x
+__device__ __forceinline__ uint32_t rev16(uint32_t x)
+{
+    x = __builtin_bitreverse32(x);
+    return ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+}
+
+// bits [s, s + 32) of the 64-bit value hi:lo (s in 0..31; v_alignbit_b32)
+__device__ __forceinline__ uint32_t funnel32(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
 }
 
 // reverse the order of the 32 two-bit groups of x
@@ -54,22 +76,6 @@ __device__ __forceinline__ uint64_t rev2(uint64_t x)
 {
     x = __builtin_bitreverse64(x);
     return ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
-}
-
-// Canonical key of the window whose forward 2-bit pack (first base most
-// significant, right aligned, 2k bits) is F.  The reference keeps
-//   seq = pack(reverse(window))    (src/GetKmer.h:71-73, shift right, new base on top)
-//   rev = pack(complement(window)) (src/GetKmer.h:75-77, shift left, complement at the low slot)
-// left aligned in N words and takes the smaller (src/GetKmer.h:181-206).
-__device__ __forceinline__ Key<1> canon1(uint64_t F, uint32_t k)
-{
-    const uint64_t mask = (k == 32) ? ~0ULL : ((1ULL << (2 * k)) - 1ULL);
-    F &= mask;
-    const uint64_t rev = (~F & mask) << (64 - 2 * k);
-    const uint64_t seq = rev2(F);
-    Key<1> r;
-    r.w[0] = seq < rev ? seq : rev;
-    return r;
 }
 
 // 33 <= k <= 64: F = (fhi, flo) is a 2k-bit value
@@ -92,8 +98,8 @@ __device__ __forceinline__ Key<2> canon2(uint64_t fhi, uint64_t flo, uint32_t k)
 
 // 65 <= k <= 128: S = (s0, s1, s2, s3) holds the codes of the last 128 bases
 // ending at the window's last base (oldest on top, 2 bits each).  As in
-// canon1/canon2, seq = pack(reverse(window)) and rev = pack(complement(window)),
-// left aligned in four words.  For k = 96 the reference never masks word 2 of
+// canon2 (and the reference, src/GetKmer.h:71-77), seq = pack(reverse(window))
+// and rev = pack(complement(window)), left aligned in four words.  For k = 96 the reference never masks word 2 of
 // seq (kmer_last_index 2, remainder 0, src/GetKmer.h:38-47,:71-73), so the
 // bases shifted out of the window stay in word 3 of seq: the 32 bases before
 // the window, newest first, zero where they precede the current run.  `pre`

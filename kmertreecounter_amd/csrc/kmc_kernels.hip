@@ -120,6 +120,40 @@ __device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s
     return b;
 }
 
+// coarse_of for N keys at once: every table read of the N keys is issued
+// before the first result is used (a key's two reads depend on each other,
+// different keys' do not), and only a key past its first bound steps on.
+template <int N>
+__device__ __forceinline__ void coarse_of_all(const Route& rt, const uint64_t* s_cb, const uint8_t* s_lut,
+                                              const uint64_t* w0, uint32_t* b)
+{
+    if (rt.mode == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[i] = (uint32_t)(w0[i] >> 56);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] = s_lut[w0[i] >> 52];
+    // the first two bounds at or above the cell's count (a cell rarely holds two)
+    uint64_t b1[N], b2[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        b1[i] = s_cb[b[i] < 255 ? b[i] : 254];
+        b2[i] = s_cb[b[i] < 254 ? b[i] + 1 : 254];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (b[i] < 255 && b1[i] <= w0[i]) {
+            uint32_t x = b[i] + 1;
+            if (x < 255 && b2[i] <= w0[i]) {
+                ++x;
+                while (x < 255 && s_cb[x] <= w0[i]) ++x;
+            }
+            b[i] = x;
+        }
+    }
+}
+
 // Learned fine bucket inside coarse bucket r (keys lo <= w0): the table of r
 // is indexed by (w0 - lo) >> shift, then bounds inside the cell are stepped
 // over.  Table entries are u8 up to 256 fine buckets, u16 above.
@@ -147,20 +181,65 @@ __device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_f
 template <int NW>
 __device__ __forceinline__ bool window_key(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<NW>& key);
 
+// The 16 windows ending in a thread's word.  k > 32: window_key per window.
+template <int NW>
+struct Windows {
+    const uint32_t* c;
+    const uint32_t* v;
+    uint32_t k;
+    __device__ __forceinline__ Windows(const uint32_t* c_, const uint32_t* v_, uint32_t k_) : c(c_), v(v_), k(k_) {}
+    __device__ __forceinline__ bool key(int i, Key<NW>& out) const { return window_key<NW>(c, v, i, k, out); }
+};
+
+// k <= 32, one key word.  The reference keeps, left aligned,
+//   seq = pack(reverse(window))    (src/GetKmer.h:71-73, shift right, new base on top)
+//   rev = pack(complement(window)) (src/GetKmer.h:75-77, shift left, complement at the low slot)
+// and takes the smaller (src/GetKmer.h:181-206).  Per thread, once: the
+// 96-bit code string c0:c1:c2 (base i of c2 at bits 2(15-i)+1..2(15-i)), the
+// same words with their bases in reverse order (r2:r1:r0, base i of c2 at bits
+// 64+2i+1..64+2i), and a mask of the windows whose k bases are all valid (the
+// validity string ANDed with itself shifted, doubling the span up to k).
+// Per window: rev is a constant funnel shift of the forward string moved to
+// the top and complemented, seq a constant funnel shift of the reversed
+// string, both cut to the top 2k bits.
 template <>
-__device__ __forceinline__ bool window_key<1>(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<1>& key)
-{
-    // 96-bit code string c[0]:c[1]:c[2], base i of c[2] at bit 2*(15-i)
-    const int s = 2 * (15 - i);
-    const uint64_t lo = ((uint64_t)c[1] << 32) | c[2];
-    const uint64_t hi = c[0];
-    const uint64_t F = s ? ((lo >> s) | (hi << (64 - s))) : lo;
-    const uint64_t V = ((uint64_t)v[0] << 32) | ((uint64_t)v[1] << 16) | (uint64_t)v[2];
-    const uint64_t km = (k == 64) ? ~0ULL : ((1ULL << k) - 1ULL);
-    const bool ok = ((V >> (15 - i)) & km) == km;
-    key = canon1(F, k);
-    return ok;
-}
+struct Windows<1> {
+    uint32_t c0, c1, c2, r0, r1, r2;
+    uint32_t okm;    // bit 15 - i: the window ending at base i is all valid
+    uint32_t shl;    // 64 - 2k
+    uint64_t top;    // the top 2k bits
+    __device__ __forceinline__ Windows(const uint32_t* c, const uint32_t* v, uint32_t k)
+    {
+        c0 = c[0];
+        c1 = c[1];
+        c2 = c[2];
+        r0 = rev16(c0);
+        r1 = rev16(c1);
+        r2 = rev16(c2);
+        // base i of the thread's word at bit 15 - i, the 32 bases before it above
+        uint64_t p = ((uint64_t)v[0] << 32) | ((uint64_t)v[1] << 16) | (uint64_t)v[2];
+        uint32_t span = 1;  // bit b of p: bases at bits b .. b + span - 1 all valid
+        while (2 * span <= k) {
+            p &= p >> span;
+            span *= 2;
+        }
+        p &= p >> (k - span);  // k - span <= span: the two spans cover k bases
+        okm = (uint32_t)p & 0xFFFFu;
+        shl = 64 - 2 * k;
+        top = ~0ULL << shl;
+    }
+    __device__ __forceinline__ bool key(int i, Key<1>& out) const
+    {
+        const uint32_t s = 2 * (15 - i);  // 0..30, a constant once unrolled
+        const uint64_t fwd = ((uint64_t)funnel32(c0, c1, s) << 32) | funnel32(c1, c2, s);
+        const uint64_t rev = ~(fwd << shl) & top;
+        const uint32_t sr = 2 * i + 2;  // 2..32
+        const uint64_t seq = (sr == 32 ? (((uint64_t)r2 << 32) | r1)
+                                       : (((uint64_t)funnel32(r2, r1, sr) << 32) | funnel32(r1, r0, sr))) & top;
+        out.w[0] = seq < rev ? seq : rev;
+        return (okm >> (15 - i)) & 1u;
+    }
+};
 
 template <>
 __device__ __forceinline__ bool window_key<2>(const uint32_t* c, const uint32_t* v, int i, uint32_t k, Key<2>& key)
@@ -218,6 +297,8 @@ __device__ __forceinline__ bool window_key<4>(const uint32_t* c, const uint32_t*
     key = canon4(s0, s1, s2, s3, k, pre);
     return ok;
 }
+
+constexpr int kRouteGroup = 4;
 
 template <int NW, bool SCATTER>
 __global__ void __launch_bounds__(ExtractCfg<NW>::kThreads)
@@ -280,24 +361,46 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         v[j] = s_valid[t + j];
     }
     (void)padded_len;
+    const Windows<NW> win(c, v, k);
 
-    Key<NW> keys[16];
-    uint32_t bk[16];
-    uint32_t rank[16];
+    Key<NW> keys[SCATTER ? 16 : 1];
+    uint32_t bk[SCATTER ? 16 : 1];
+    uint32_t rank[SCATTER ? 16 : 1];
     uint32_t okm = 0;
     if constexpr (SCATTER) {
-        // every key routed first, then the histogram atomics: the routing table
-        // reads of a thread's 16 keys overlap instead of waiting behind atomics
-        // (measured faster in the scatter, slower in the count pass)
+        if (rt.mode == 0) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            Key<NW> key;
-            const bool ok = window_key<NW>(c, v, i, k, key);
-            keys[i] = key;
-            bk[i] = ok ? coarse_of(rt, s_cb, s_lut, key.w[0]) : 0u;
-            // the scatter of a bucket-range round keeps its own buckets only
-            if (ok && bk[i] >= rt.rlo && bk[i] < rt.rhi) okm |= 1u << i;
+            for (int i = 0; i < 16; ++i) {
+                Key<NW> key;
+                if (win.key(i, key)) okm |= 1u << i;
+                keys[i] = key;
+                bk[i] = (uint32_t)(key.w[0] >> 56);
+            }
+        } else {
+            // learned routing: keys in two halves of eight, each half's
+            // buckets looked up together (coarse_of_all).  Measured: 11.1 ms
+            // key by key, 10.1 ms this way at 2.33e9 keys; the count pass and
+            // the prefix scatter, with less latency to hide or more waves to
+            // hide it, were faster key by key.
+#pragma unroll
+            for (int part = 0; part < 16 / kRouteGroup; ++part) {
+                uint64_t w0[kRouteGroup];
+#pragma unroll
+                for (int j = 0; j < kRouteGroup; ++j) {
+                    const int i = kRouteGroup * part + j;
+                    Key<NW> key;
+                    if (win.key(i, key)) okm |= 1u << i;
+                    w0[j] = key.w[0];
+                    keys[i] = key;
+                }
+                coarse_of_all<kRouteGroup>(rt, s_cb, s_lut, w0, bk + kRouteGroup * part);
+            }
         }
+        // the scatter of a bucket-range round keeps its own buckets only, then
+        // the histogram atomics (after every key is routed)
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (bk[i] < rt.rlo || bk[i] >= rt.rhi) okm &= ~(1u << i);
 #pragma unroll
         for (int i = 0; i < 16; ++i)
             if (okm & (1u << i)) rank[i] = atomicAdd(&s_hist[bk[i]], 1u);
@@ -305,7 +408,7 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             Key<NW> key;
-            if (window_key<NW>(c, v, i, k, key)) atomicAdd(&s_hist[coarse_of(rt, s_cb, s_lut, key.w[0])], 1u);
+            if (win.key(i, key)) atomicAdd(&s_hist[coarse_of(rt, s_cb, s_lut, key.w[0])], 1u);
         }
     }
     __syncthreads();
@@ -318,12 +421,14 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         return;
     } else {
         uint32_t h = (t < 256) ? s_hist[t] : 0u;
+        // the cursor reservation is issued first and its result stored only
+        // after the scan and the staging: its round trip overlaps them (the
+        // barriers in between wait for LDS operations only)
+        unsigned long long gb = 0;
+        if (h) gb = atomicAdd(&cursor[(size_t)xcd * 256 + t], (unsigned long long)h);
         uint32_t total;
         const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
-        if (t < 256) {
-            s_off[t] = off;
-            if (h) s_gbase[t] = atomicAdd(&cursor[(size_t)xcd * 256 + t], (unsigned long long)h);
-        }
+        if (t < 256) s_off[t] = off;
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -333,6 +438,7 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
                 s_bkt[slot] = (uint8_t)bk[i];
             }
         }
+        if (h) s_gbase[t] = gb;
         __syncthreads();
         for (uint32_t slot = t; slot < total; slot += T) {
             const uint32_t b = s_bkt[slot];
@@ -502,6 +608,11 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
         const uint32_t idx = t + j * T;
         if (idx < n) keys[j] = load_key<NW>(A, beg + idx);
     }
+    // one wait for all of them on every path (vmcnt(0), expcnt and lgkmcnt
+    // left open): the loads sit in branches, and without a wait that
+    // dominates the later uses each use of a key would wait for every global
+    // operation in flight, the cursor reservations below included
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     if constexpr (SCATTER) {
         // routed first, then the atomics (as in k_extract's scatter)
 #pragma unroll
@@ -532,15 +643,21 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
             h[q] = i < nsub ? s_hist[i] : 0u;
             mine += h[q];
         }
+        // cursor reservations issued before the scan, stored after the staging
+        // (their round trips overlap both, as in k_extract)
+        unsigned long long gb[E];
+#pragma unroll
+        for (uint32_t q = 0; q < E; ++q) {
+            const uint32_t i = E * t + q;
+            gb[q] = 0;
+            if (h[q]) gb[q] = atomicAdd(&cursor[(size_t)xcd * nf + ((size_t)r << rt.sbits) + i], (unsigned long long)h[q]);
+        }
         uint32_t total;
         uint32_t off = block_exclusive_scan<uint32_t>(mine, s_scan, &total);
 #pragma unroll
         for (uint32_t q = 0; q < E; ++q) {
             const uint32_t i = E * t + q;
-            if (i < nsub) {
-                s_off[i] = off;
-                if (h[q]) s_gbase[i] = atomicAdd(&cursor[(size_t)xcd * nf + ((size_t)r << rt.sbits) + i], (unsigned long long)h[q]);
-            }
+            if (i < nsub) s_off[i] = off;
             off += h[q];
         }
         __syncthreads();
@@ -553,6 +670,9 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
                 s_bkt[slot] = (BktT)sb[j];
             }
         }
+#pragma unroll
+        for (uint32_t q = 0; q < E; ++q)
+            if (h[q]) s_gbase[E * t + q] = gb[q];
         __syncthreads();
         for (uint32_t slot = t; slot < total; slot += T) {
             const uint32_t b = s_bkt[slot];

@@ -77,6 +77,23 @@ def test_small_random_stream(k, counter):
     check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
 
 
+@pytest.mark.parametrize("k", [1, 2, 5, 15, 16, 17, 31, 32, 33, 64, 65, 128])
+def test_every_byte_value(k):
+    # runs of bases (both cases, U) broken by every byte value 0..255, each
+    # one at several offsets inside the 16-byte words the kernels encode:
+    # only A C G T U in either case are bases (src/FastqParser.h:194-236),
+    # including bytes one bit away from a letter ('@', '`', 'V', 0xC1, 0xE1)
+    rng = np.random.default_rng(500 + k)
+    n = 240_000
+    seq = rng.choice(np.frombuffer(b"ACGTacgtUu", np.uint8), size=n)
+    pos = rng.choice(n, size=256 * 24, replace=False)
+    seq[pos] = np.tile(np.arange(256, dtype=np.uint8), 24)
+    seq = seq.tobytes()
+    keys, counts = C.count_table(C.extract(seq, k))
+    f, r, s = run_engine([seq], k)
+    check_outputs(f, r, s, k, keys, counts)
+
+
 def test_empty_and_degenerate_inputs():
     for blocks in ([], [b"\n"], [b"NNNNNNNNNN\n"], [b"ACGT\n"], [b"A" * 31 + b"\n"]):
         f, r, s = run_engine(blocks, 31)
