@@ -404,11 +404,28 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
 #pragma unroll
         for (int i = 0; i < 16; ++i)
             if (okm & (1u << i)) rank[i] = atomicAdd(&s_hist[bk[i]], 1u);
-    } else {
+    } else if (rt.mode == 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             Key<NW> key;
-            if (win.key(i, key)) atomicAdd(&s_hist[coarse_of(rt, s_cb, s_lut, key.w[0])], 1u);
+            if (win.key(i, key)) atomicAdd(&s_hist[(uint32_t)(key.w[0] >> 56)], 1u);
+        }
+    } else {
+#pragma unroll
+        for (int part = 0; part < 16 / kRouteGroup; ++part) {
+            uint64_t w0[kRouteGroup];
+            uint32_t b[kRouteGroup];
+            uint32_t ok = 0;
+#pragma unroll
+            for (int j = 0; j < kRouteGroup; ++j) {
+                Key<NW> key;
+                if (win.key(kRouteGroup * part + j, key)) ok |= 1u << j;
+                w0[j] = key.w[0];
+            }
+            coarse_of_all<kRouteGroup>(rt, s_cb, s_lut, w0, b);
+#pragma unroll
+            for (int j = 0; j < kRouteGroup; ++j)
+                if (ok & (1u << j)) atomicAdd(&s_hist[b[j]], 1u);
         }
     }
     __syncthreads();
