@@ -948,35 +948,44 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                 if (i < hi) nb[u] = load_key<NW>(B, i);
             }
             uint64_t fbal[kLoads];  // first occurrences of each key slot, one reservation per step
+            // per key slot: run length (a run of r equal keys is one occurrence
+            // followed by r - 1 repeats: the head lane tests the filter once and
+            // counts the rest), hash (bits 0..27 filter word and bits, 32..44
+            // table slot, 48..63 pass range) and filter bits (0: not tested)
+            uint32_t run[kLoads], fmask[kLoads], fold[kLoads];
+            uint64_t hk[kLoads];
+            // k <= 64: the step's filter atomics issued back to back; k > 64
+            // (more key registers) one key at a time
+            constexpr bool kBatch = NW <= 2;
+            auto filter = [&](int u) {
+                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                run[u] = run_head_length<NW>(kb[u], i < hi);
+                hk[u] = count_mix<NW>(kb[u]);
+                const uint32_t hb = (uint32_t)(hk[u] >> 48);
+                fmask[u] = (run[u] && hb >= hstart && hb < hend) ? bloom_mask32(hk[u]) : 0u;
+                // test-and-set of the key's bits (src/BloomFilter.h:225-234; an
+                // empty mask leaves the word as it is); the reference's relaxed
+                // pre-read is left out: it costs more than the atomics it saves here
+                fold[u] = atomicOr(&s_bloom[(uint32_t)(hk[u] & (BW - 1))], fmask[u]);
+            };
+            if constexpr (kBatch) {
+#pragma unroll
+                for (int u = 0; u < kLoads; ++u) filter(u);
+            }
 #pragma unroll
             for (int u = 0; u < kLoads; ++u) {
-                const unsigned long long i = lo + (it * kLoads + u) * T + t;
+                if constexpr (!kBatch) filter(u);
                 bool first = false;
-                const Key<NW>& key = kb[u];
-                // a run of r equal keys is one occurrence followed by r - 1
-                // repeats: the head lane tests the filter once and counts the rest
-                const uint32_t run = run_head_length<NW>(key, i < hi);
-                if (run) {
-                    // hash bits: 0..27 filter word and bits, 32..44 table slot, 48..63 pass range
-                    const uint64_t h = count_mix<NW>(key);
-                    const uint32_t hb = (uint32_t)(h >> 48);
-                    if (hb >= hstart && hb < hend) {
-                        const uint32_t word = (uint32_t)(h & (BW - 1));
-                        const uint32_t mask = bloom_mask32(h);
-                        // test-and-set of the key's bits (src/BloomFilter.h:225-234); the
-                        // reference's relaxed pre-read is left out: it costs more than the
-                        // atomics it saves here
-                        const uint32_t old = atomicOr(&s_bloom[word], mask);
-                        first = (old & mask) != mask;
-                        const uint32_t add = run - (first ? 1u : 0u);
-                        if (add) {
-                            bool claimed;
-                            const int64_t s = table_find_or_claim<TW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
-                                                                     (h >> 32), CountCfg<NW>::kProbe, to_table<NW>(key),
-                                                                     claimed);
-                            if (s >= 0) atomicAdd(&s_tcnt[s], add);
-                            else global_add<NW>(args, key, add);
-                        }
+                if (fmask[u]) {
+                    first = (fold[u] & fmask[u]) != fmask[u];
+                    const uint32_t add = run[u] - (first ? 1u : 0u);
+                    if (add) {
+                        bool claimed;
+                        const int64_t s = table_find_or_claim<TW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
+                                                                 (hk[u] >> 32), CountCfg<NW>::kProbe,
+                                                                 to_table<NW>(kb[u]), claimed);
+                        if (s >= 0) atomicAdd(&s_tcnt[s], add);
+                        else global_add<NW>(args, kb[u], add);
                     }
                 }
                 fbal[u] = __ballot(first);
