@@ -323,7 +323,7 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_off[256];
-    __shared__ unsigned long long s_gbase[256];
+    __shared__ unsigned long long s_dst[256];  // global index of the tile's slot 0 for bucket b
     __shared__ uint32_t s_scan[T / kWave + 1];
     __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
 
@@ -447,20 +447,28 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
         if (t < 256) s_off[t] = off;
         __syncthreads();
+        // prefix buckets are the keys' top byte: no bucket byte per slot then
+        const bool stage_bucket = rt.mode != 0;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             if (okm & (1u << i)) {
                 const uint32_t slot = s_off[bk[i]] + rank[i];
                 s_keys[slot] = keys[i];
-                s_bkt[slot] = (uint8_t)bk[i];
+                if (stage_bucket) s_bkt[slot] = (uint8_t)bk[i];
             }
         }
-        if (h) s_gbase[t] = gb;
+        // bucket b's keys go to gb + (slot - off): one base per bucket (the
+        // difference wraps modulo 2^64 and the sum with slot is exact)
+        if (h) s_dst[t] = gb - (unsigned long long)off;
         __syncthreads();
-        for (uint32_t slot = t; slot < total; slot += T) {
-            const uint32_t b = s_bkt[slot];
-            const uint64_t dst = s_gbase[b] + (slot - s_off[b]);
-            store_key<NW>(out, dst, s_keys[slot]);
+        if (stage_bucket) {
+            for (uint32_t slot = t; slot < total; slot += T)
+                store_key<NW>(out, s_dst[s_bkt[slot]] + slot, s_keys[slot]);
+        } else {
+            for (uint32_t slot = t; slot < total; slot += T) {
+                const Key<NW> key = s_keys[slot];
+                store_key<NW>(out, s_dst[(uint32_t)(key.w[0] >> 56)] + slot, key);
+            }
         }
     }
 }
@@ -581,7 +589,7 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
     __shared__ uint32_t s_hist[NB];
     __shared__ uint32_t s_off[NB];
-    __shared__ unsigned long long s_gbase[NB];
+    __shared__ unsigned long long s_dst[NB];  // global index of the tile's slot 0 for fine bucket f
     __shared__ uint32_t s_scan[T / kWave + 1];
     __shared__ BktT s_bkt[SCATTER ? TILE : 1];
 
@@ -671,30 +679,40 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
         }
         uint32_t total;
         uint32_t off = block_exclusive_scan<uint32_t>(mine, s_scan, &total);
+        uint32_t offq[E];
 #pragma unroll
         for (uint32_t q = 0; q < E; ++q) {
             const uint32_t i = E * t + q;
+            offq[q] = off;
             if (i < nsub) s_off[i] = off;
             off += h[q];
         }
         __syncthreads();
+        // prefix fine buckets are key bits: no bucket index per slot then
+        const bool stage_bucket = rt.mode != 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t idx = t + j * T;
             if (idx < n) {
                 const uint32_t slot = s_off[sb[j]] + rank[j];
                 s_keys[slot] = keys[j];
-                s_bkt[slot] = (BktT)sb[j];
+                if (stage_bucket) s_bkt[slot] = (BktT)sb[j];
             }
         }
+        // fine bucket f's keys go to gb + (slot - off), as in k_extract
 #pragma unroll
         for (uint32_t q = 0; q < E; ++q)
-            if (h[q]) s_gbase[E * t + q] = gb[q];
+            if (h[q]) s_dst[E * t + q] = gb[q] - (unsigned long long)offq[q];
         __syncthreads();
-        for (uint32_t slot = t; slot < total; slot += T) {
-            const uint32_t b = s_bkt[slot];
-            const uint64_t dst = s_gbase[b] + (slot - s_off[b]);
-            store_key<NW>(B, dst, s_keys[slot]);
+        if (stage_bucket) {
+            for (uint32_t slot = t; slot < total; slot += T)
+                store_key<NW>(B, s_dst[s_bkt[slot]] + slot, s_keys[slot]);
+        } else {
+            for (uint32_t slot = t; slot < total; slot += T) {
+                const Key<NW> key = s_keys[slot];
+                const uint32_t f = rt.sbits ? (uint32_t)((key.w[0] >> (56 - rt.sbits)) & (nsub - 1u)) : 0u;
+                store_key<NW>(B, s_dst[f] + slot, key);
+            }
         }
     }
 }
