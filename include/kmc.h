@@ -72,7 +72,7 @@ typedef enum kc_partition {
 
 /* Repeat counting inside a bucket.  Both give the same merged table (the
  * low.bin keys plus the root records).  BLOOM (the default) runs the
- * reference's first-occurrence Bloom test (XXH3-128, three bits per key) and
+ * reference's first-occurrence Bloom test (three bits per key) and
  * counts only the repeats, so a Bloom false positive moves a key's first
  * occurrence into its record, as in the reference.  EXACT counts every
  * occurrence in the bucket's table and splits each key into one low.bin entry

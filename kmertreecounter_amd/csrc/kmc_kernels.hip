@@ -103,26 +103,12 @@ __device__ __forceinline__ void store_key(uint64_t* p, uint64_t i, const Key<NW>
     }
 }
 
-// Learned coarse bucket = number of the 255 bounds <= w0.  The table entry of
-// w0's 12-bit cell counts the bounds at or below the cell's first key, so only
-// bounds inside the cell remain to be stepped over (usually none).
-__device__ __forceinline__ uint32_t coarse_of(const Route& rt, const uint64_t* s_cb, const uint8_t* s_lut, uint64_t w0)
-{
-    if (rt.mode == 0) return (uint32_t)(w0 >> 56);
-    uint32_t b = s_lut[w0 >> 52];
-    // the first bound check without a loop, so the checks of a thread's keys
-    // overlap; only a key past that bound steps on (a cell rarely holds two)
-    const bool past = b < 255 && s_cb[b < 255 ? b : 254] <= w0;
-    if (past) {
-        ++b;
-        while (b < 255 && s_cb[b] <= w0) ++b;
-    }
-    return b;
-}
-
-// coarse_of for N keys at once: every table read of the N keys is issued
-// before the first result is used (a key's two reads depend on each other,
-// different keys' do not), and only a key past its first bound steps on.
+// Coarse buckets of N keys.  Prefix routing: the top byte.  Learned routing:
+// the number of the 255 bounds <= w0; the table entry of w0's 12-bit cell
+// counts the bounds at or below the cell's first key, so only bounds inside
+// the cell remain to be stepped over (usually none, rarely two).  Every table
+// read of the N keys is issued before the first result is used (a key's reads
+// depend on each other, different keys' do not).
 template <int N>
 __device__ __forceinline__ void coarse_of_all(const Route& rt, const uint64_t* s_cb, const uint8_t* s_lut,
                                               const uint64_t* w0, uint32_t* b)
