@@ -332,10 +332,12 @@ def main() -> int:
 
     alg = algorithmic_bytes(st, seq_bytes + (-(seq_bytes) % (512 * 16)))
     dominant = max(per_phase, key=per_phase.get)
-    # a graph replay has no per-kernel times: no kernel roofline then
-    achieved = alg[dominant] / (per_phase[dominant] * 1e-3) / 1e9 if per_phase[dominant] > 0 else None
+    # a graph replay has no per-kernel times: no kernel roofline then (no
+    # dominant kernel, no achieved rate, no traffic)
+    timed = per_phase[dominant] > 0
+    achieved = alg[dominant] / (per_phase[dominant] * 1e-3) / 1e9 if timed else None
     total_alg = sum(alg.values())
-    traffic = traffic_from_profiles(dominant, args.workload, main_run["partition"])
+    traffic = traffic_from_profiles(dominant, args.workload, main_run["partition"]) if timed else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -375,7 +377,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": names[dominant],
+            "kernel": names[dominant] if timed else None,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
