@@ -311,7 +311,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the k-mer exchange between "
+        print("bench.py measures one GPU: the engine has no multi-GPU path (the k-mer exchange between "
               "GPUs is a design note only); run with --gpus 1", file=sys.stderr)
         return 2
 

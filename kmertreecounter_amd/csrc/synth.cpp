@@ -2,7 +2,7 @@
 //
 // Every random draw comes from SplitMix64 streams keyed by (seed, purpose,
 // index), so a read's bytes depend only on the seed and its index: the output
-// is identical for any thread count and on any x86-64 host running this build.
+// is identical for any thread count and on any x86-64 host.
 //
 // Workloads (genome length G, read count R at scale 1; `scale` divides G and R):
 //   1  G=2 Mbp i.i.d. uniform; 400k x 150 bp; 1% substitutions; no N; quality 'I'.
