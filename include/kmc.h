@@ -113,6 +113,10 @@ typedef struct kc_config {
                                  (one-round runs; the graph is captured again when buffers or
                                  options change); the stats then carry the total time only */
     uint32_t reserved2;
+    uint64_t est_file_size;   /* Stage 0: the reference's estimated FASTQ size, input of the Bloom
+                                 sizing (FastqPreReader::get_estimated_raw_fastq_file_size: the size
+                                 on disk, four times it for gzip; src/main.cpp:88-108); 0 = expected_bytes,
+                                 or the sample size when that is 0 too */
 } kc_config;
 
 typedef struct kc_stats {
@@ -218,6 +222,11 @@ typedef struct kc_tune_result {
 } kc_tune_result;
 KC_API int kc_autotune(kc_engine* h, kc_tune_result* out);
 
+/* Free and total device memory of a device in bytes (hipMemGetInfo); for
+ * choosing max_round_keys, and for tests that check nothing stays allocated.
+ * No reference counterpart. */
+KC_API int kc_device_memory(uint32_t device, uint64_t* free_bytes, uint64_t* total_bytes);
+
 /* Number of u64 words per key for k (1, 2 or 4), 0 if k is 0 or above 128. */
 KC_API uint32_t kc_words_for_k(uint32_t k);
 
@@ -233,6 +242,18 @@ KC_API int kc_debug_resident(kc_engine* h, char* out, uint64_t cap, uint64_t* le
  * 16-byte-per-lane kernel; *ms = its device time.  Reads `bytes` and writes
  * `bytes` of HBM: the known-size reference for hardware traffic counters. */
 KC_API int kc_debug_copy(uint64_t bytes, uint32_t device, float* ms);
+
+/* Fault hook for tests, no reference counterpart (the reference exits on the
+ * same failures: src/FastqReader.h:78-90, src/ExportWriter.h:595-599, or throws
+ * std::bad_alloc, src/BloomFilter.h:45-48).  kind 1: device and pinned-host
+ * allocations of the engine; kind 2: its host<->device and device copies.
+ * The nth such call from now on fails once (nth <= 0: none), as if the runtime
+ * had refused it; the entry point that made it returns KC_ERR_NOMEM or
+ * KC_ERR_HIP, frees what it allocated and leaves the handle usable.  Returns
+ * the calls of that kind made since the hook was last set (-1: bad kind).
+ * KMC_FAULT_ALLOC=n / KMC_FAULT_COPY=n in the environment set it at the first
+ * kc_create of the process. */
+KC_API int64_t kc_debug_fault(uint32_t kind, int64_t nth);
 
 /* Test hooks of the learned partitioner, no reference counterpart.
  * kc_debug_gemm_bf16: C[M][N] = A[M][K] . Bt[N][K]^T on the MFMA product

@@ -28,6 +28,16 @@ KC_API int kmc_fastq_blocks(const char* fastq, uint64_t len, uint32_t k, uint64_
 KC_API int kmc_fastq_blocks_file(const char* path, uint32_t k, uint64_t block_size,
                                  uint64_t max_bytes, kmc_block_cb cb, void* user);
 
+/* Stage-0 sample of a FASTQ file as the reference's FastqPreReader<N>::pre_read
+ * reads it (src/FastqPreReader.h:128-300): min(size, 256 MiB) bytes, as three
+ * record-aligned parts of a plain file or the head of a gzip stream, cut into
+ * sequence-only blocks like kmc_fastq_blocks.  *est_file_size (may be NULL)
+ * receives the reference's estimated FASTQ size
+ * (get_estimated_raw_fastq_file_size: the size on disk, four times it for
+ * gzip), the Bloom sizing input of kc_plan. */
+KC_API int kmc_stage0_sample_file(const char* path, uint32_t k, uint64_t block_size, kmc_block_cb cb, void* user,
+                                  uint64_t* est_file_size);
+
 /* low.bin record packing (src/ExportWriter.h:522-550): per key k/32 raw
  * little-endian words, then the top ceil(2*(k%32)/8) bytes of the masked tail
  * word.  out needs n * kmc_packed_bytes(k) bytes. */
@@ -40,7 +50,7 @@ KC_API void kmc_pack_low(const uint64_t* keys, uint64_t n, uint32_t k, uint8_t* 
 KC_API int kmc_write_outputs(kc_engine* h, const char* tmp_dir);
 
 /* The `Tree` pipeline (src/main.cpp:111-312) on the GPU engine: Stage 0 on the
- * first sample_bytes of FASTQ, then the whole file, then outputs in tmp_dir.
+ * reference's sample of the file, then the whole file, then outputs in tmp_dir.
  * Fills *stats.  Prints the reference's summary lines when verbose. */
 typedef struct kmc_tree_opts {
     const char* fastq_path;
@@ -51,7 +61,8 @@ typedef struct kmc_tree_opts {
     uint32_t max_count;     /* 0 = UINT32_MAX */
     uint32_t partition;     /* kc_partition */
     uint32_t verbose;
-    uint64_t sample_bytes;  /* Stage-0 sample size (0 = 256 MiB, the reference's cap) */
+    uint64_t sample_bytes;  /* Stage-0 sample: 0 = the reference's (kmc_stage0_sample_file); otherwise
+                               the first sample_bytes of the file */
     uint32_t counter;       /* kc_counter */
     uint32_t histogram_only; /* 1: skip low.bin and the root files (needs histogram_path) */
     const char* histogram_path; /* if set: write the precise count histogram here, as histogram_tool does */

@@ -34,7 +34,9 @@ KC_HISTOGRAM_APPROXIMATE = 1
 
 
 class KmcError(RuntimeError):
-    pass
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
 
 
 class Config(ctypes.Structure):
@@ -45,7 +47,7 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
         ("heavy_keys", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
-        ("graph", ctypes.c_uint32), ("reserved2", ctypes.c_uint32),
+        ("graph", ctypes.c_uint32), ("reserved2", ctypes.c_uint32), ("est_file_size", ctypes.c_uint64),
     ]
 
 
@@ -114,7 +116,8 @@ EXPORTED = [
     "kc_last_error", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
-    "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident",
+    "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident", "kmc_stage0_sample_file",
+    "kc_debug_fault", "kc_device_memory",
 ]
 
 _lib = None
@@ -151,6 +154,9 @@ def lib() -> ctypes.CDLL:
     L.kc_last_error.argtypes = [vp]
     L.kc_last_error.restype = cp
     L.kc_debug_copy.argtypes = [u64, u32, ctypes.POINTER(ctypes.c_float)]
+    L.kc_debug_fault.argtypes = [u32, ctypes.c_int64]
+    L.kc_debug_fault.restype = ctypes.c_int64
+    L.kc_device_memory.argtypes = [u32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L.kc_histogram.argtypes = [vp, u32, u32, u32, vp]
     L.kc_autotune.argtypes = [vp, vp]
     L.kmc_write_histogram.argtypes = [cp, vp, u32, u32]
@@ -162,6 +168,7 @@ def lib() -> ctypes.CDLL:
     L.kcs_free.argtypes = [vp]
     L.kmc_fastq_blocks.argtypes = [cp, u64, u32, u64, BLOCK_CB, vp]
     L.kmc_fastq_blocks_file.argtypes = [cp, u32, u64, u64, BLOCK_CB, vp]
+    L.kmc_stage0_sample_file.argtypes = [cp, u32, u64, BLOCK_CB, vp, ctypes.POINTER(u64)]
     L.kmc_packed_bytes.argtypes = [u32]
     L.kmc_packed_bytes.restype = u64
     L.kmc_pack_low.argtypes = [vp, u64, u32, vp]
@@ -220,6 +227,23 @@ def fastq_blocks(fastq: bytes, k: int, block_size: int = 32 * 1024) -> list[byte
     return blocks
 
 
+def stage0_sample_file(path: str | os.PathLike, k: int, block_size: int = 32 * 1024) -> tuple[list[bytes], int]:
+    """Stage-0 sample blocks of a FASTQ file as the reference's FastqPreReader reads
+    them, and the reference's estimated file size (the Bloom sizing input)."""
+    blocks: list[bytes] = []
+
+    @BLOCK_CB
+    def cb(_user, data, n):
+        blocks.append(ctypes.string_at(data, n))
+        return 0
+
+    est = ctypes.c_uint64()
+    rc = lib().kmc_stage0_sample_file(str(path).encode(), k, block_size, cb, None, ctypes.byref(est))
+    if rc != 0:
+        raise KmcError(f"kmc_stage0_sample_file failed: {rc}")
+    return blocks, est.value
+
+
 def pack_low(keys: np.ndarray, k: int) -> bytes:
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
     nw = words_for_k(k)
@@ -237,22 +261,22 @@ class Engine:
     def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
                  pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0,
-                 max_round_keys: int = 0, graph: bool = False):
+                 max_round_keys: int = 0, graph: bool = False, est_file_size: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
-                     counter, table_slots, heavy_keys, 0, max_round_keys, 1 if graph else 0, 0)
+                     counter, table_slots, heavy_keys, 0, max_round_keys, 1 if graph else 0, 0, est_file_size)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
-            raise KmcError(f"kc_create failed with status {rc}")
+            raise KmcError(f"kc_create failed with status {rc}", rc)
         self._h = h
 
     def _check(self, rc: int, what: str) -> None:
         if rc != 0:
             msg = self._L.kc_last_error(self._h)
-            raise KmcError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}")
+            raise KmcError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}", rc)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -357,6 +381,25 @@ def count_sequence(seq: bytes, k: int, **kw) -> tuple[np.ndarray, np.ndarray, St
         e.push_block(seq)
         e.finish()
         return e.firsts(), e.records(), e.stats()
+
+
+def device_memory(device: int = 0) -> tuple[int, int]:
+    """(free, total) bytes of a device's memory."""
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().kc_device_memory(device, ctypes.byref(f), ctypes.byref(t))
+    if rc != 0:
+        raise KmcError(f"kc_device_memory failed: {rc}", rc)
+    return f.value, t.value
+
+
+FAULT_ALLOC = 1
+FAULT_COPY = 2
+
+
+def debug_fault(kind: int, nth: int) -> int:
+    """Make the nth engine allocation (kind 1) or copy (kind 2) from now on fail
+    once (nth <= 0: none); returns the calls of that kind since the last call."""
+    return int(lib().kc_debug_fault(kind, nth))
 
 
 def debug_copy(nbytes: int, device: int = 0) -> float:

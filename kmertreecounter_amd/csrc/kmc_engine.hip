@@ -4,6 +4,7 @@
 #include <sys/mman.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -11,15 +12,35 @@
 #include <cstring>
 #include <string>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "../../include/kmc.h"
+#include "kmc_fault.h"
 #include "kmc_kernels.h"
 #include "kmc_learned.h"
 #include "kmc_plan.h"
 
 using namespace kmc;
+
+namespace kmc {
+
+namespace {
+// fault hook: calls seen per kind since the hook was last set, and the call
+// (1-based) that fails; 0 = none
+std::atomic<int64_t> g_fault_calls[3];
+std::atomic<int64_t> g_fault_at[3];
+}  // namespace
+
+bool fault_hit(uint32_t kind)
+{
+    const int64_t c = g_fault_calls[kind].fetch_add(1, std::memory_order_relaxed) + 1;
+    const int64_t at = g_fault_at[kind].load(std::memory_order_relaxed);
+    return at > 0 && c == at;
+}
+
+}  // namespace kmc
 
 namespace {
 
@@ -131,7 +152,7 @@ struct kc_engine {
             constexpr uint64_t kAlign = 2ULL << 20;
             const uint64_t sz = std::max<uint64_t>(kAlign, (n + kAlign - 1) / kAlign * kAlign);
             void* q = nullptr;
-            if (hipHostMalloc(&q, sz, hipHostMallocDefault) == hipSuccess && q) {
+            if (dhost_malloc(&q, sz, hipHostMallocDefault) == hipSuccess && q) {
                 p = static_cast<uint8_t*>(q);
                 pinned = true;
             } else {
@@ -216,18 +237,22 @@ int ensure_seq_capacity(kc_engine* h, uint64_t need_len)
     if (want <= h->seq_alloc) return KC_OK;
     uint64_t alloc = std::max<uint64_t>(want, h->seq_alloc * 2);
     uint8_t* nd = nullptr;
-    hipError_t e = hipMalloc(&nd, alloc);
+    hipError_t e = dmalloc(&nd, alloc);
     if (e != hipSuccess) {
         alloc = want;
-        e = hipMalloc(&nd, alloc);
+        e = dmalloc(&nd, alloc);
         if (e != hipSuccess) return h->fail(KC_ERR_NOMEM, "device allocation for the sequence buffer failed");
     }
     if (h->d_seq) {
-        HIPX(hipStreamSynchronize(h->st));
         // parsed FASTQ output not yet read back is at most fq_unsettled bytes
         const uint64_t used = std::min<uint64_t>(h->seq_alloc, h->seq_len + h->fq_unsettled);
-        HIPX(hipMemcpy(nd, h->d_seq, used, hipMemcpyDeviceToDevice));
-        HIPX(hipFree(h->d_seq));
+        hipError_t ce = hipStreamSynchronize(h->st);
+        if (ce == hipSuccess) ce = dcopy(nd, h->d_seq, used, hipMemcpyDeviceToDevice);
+        if (ce != hipSuccess) {
+            (void)hipFree(nd);  // the old buffer stays in place
+            return h->hip_fail(ce, "copy of the sequence buffer");
+        }
+        (void)hipFree(h->d_seq);
     }
     h->d_seq = nd;
     h->seq_alloc = alloc;
@@ -240,7 +265,7 @@ int flush_stage(kc_engine* h)
     int rc = ensure_seq_capacity(h, h->seq_len + h->stage_len);
     if (rc) return rc;
     const int s = h->cur_stage;
-    HIPX(hipMemcpyAsync(h->d_seq + h->seq_len, h->h_stage[s], h->stage_len, hipMemcpyHostToDevice, h->st));
+    HIPX(dcopy_async(h->d_seq + h->seq_len, h->h_stage[s], h->stage_len, hipMemcpyHostToDevice, h->st));
     HIPX(hipEventRecord(h->stage_ev[s], h->st));
     h->stage_busy[s] = true;
     h->seq_len += h->stage_len;
@@ -260,11 +285,18 @@ int flush_stage(kc_engine* h)
 int flush_fastq(kc_engine* h)
 {
     if (h->stage_len == 0) return KC_OK;
-    if (!h->d_raw) {
-        if (hipMalloc(&h->d_raw, kStageBytes + 64) != hipSuccess ||
-            hipMalloc(&h->d_fq_scratch, fastq_tile_scratch_bytes(kStageBytes)) != hipSuccess ||
-            hipMalloc(&h->d_fq_state, 2 * sizeof(unsigned long long)) != hipSuccess)
+    if (!h->d_raw || !h->d_fq_scratch || !h->d_fq_state) {
+        dfree(h->d_raw);
+        dfree(h->d_fq_scratch);
+        dfree(h->d_fq_state);
+        if (dmalloc(&h->d_raw, kStageBytes + 64) != hipSuccess ||
+            dmalloc(&h->d_fq_scratch, fastq_tile_scratch_bytes(kStageBytes)) != hipSuccess ||
+            dmalloc(&h->d_fq_state, 2 * sizeof(unsigned long long)) != hipSuccess) {
+            dfree(h->d_raw);
+            dfree(h->d_fq_scratch);
+            dfree(h->d_fq_state);
             return h->fail(KC_ERR_NOMEM, "device allocation for FASTQ parsing failed");
+        }
     }
     // each raw byte yields at most one sequence byte
     int rc = ensure_seq_capacity(h, h->seq_len + h->fq_unsettled + h->stage_len);
@@ -272,11 +304,11 @@ int flush_fastq(kc_engine* h)
     if (!h->fq_state_live) {
         const unsigned long long init[2] = {h->seq_len, h->fq_phase};
         HIPX(hipStreamSynchronize(h->st));
-        HIPX(hipMemcpy(h->d_fq_state, init, sizeof(init), hipMemcpyHostToDevice));
+        HIPX(dcopy(h->d_fq_state, init, sizeof(init), hipMemcpyHostToDevice));
         h->fq_state_live = true;
     }
     const int s = h->cur_stage;
-    HIPX(hipMemcpyAsync(h->d_raw, h->h_stage[s], h->stage_len, hipMemcpyHostToDevice, h->st));
+    HIPX(dcopy_async(h->d_raw, h->h_stage[s], h->stage_len, hipMemcpyHostToDevice, h->st));
     HIPX(hipEventRecord(h->stage_ev[s], h->st));
     h->stage_busy[s] = true;
     HIPX(launch_fastq_parse(h->d_raw, h->stage_len, h->d_fq_scratch, h->d_fq_state, h->d_seq, h->st));
@@ -299,7 +331,7 @@ int settle_fastq(kc_engine* h)
     if (!h->fq_unsettled) return KC_OK;
     unsigned long long st[2] = {0, 0};
     HIPX(hipStreamSynchronize(h->st));
-    HIPX(hipMemcpy(st, h->d_fq_state, sizeof(st), hipMemcpyDeviceToHost));
+    HIPX(dcopy(st, h->d_fq_state, sizeof(st), hipMemcpyDeviceToHost));
     h->seq_len = st[0];
     h->fq_phase = (uint32_t)st[1];
     h->fq_unsettled = 0;
@@ -325,10 +357,29 @@ int append_bytes(kc_engine* h, const char* data, uint64_t len)
 
 void free_work(kc_engine* h)
 {
-    hipFree(h->d_A); h->d_A = nullptr;
-    hipFree(h->d_B); h->d_B = nullptr;
-    hipFree(h->d_R); h->d_R = nullptr;
+    dfree(h->d_A);
+    dfree(h->d_B);
+    dfree(h->d_R);
     h->key_cap = 0;
+}
+
+void free_fine(kc_engine* h)
+{
+    dfree(h->d_fcounts);
+    dfree(h->d_fcursor);
+    dfree(h->d_fine_off);
+    dfree(h->d_foff);
+    dfree(h->d_roff);
+    dfree(h->d_bucket_u32);
+    h->nf_alloc = 0;
+}
+
+void free_spill(kc_engine* h)
+{
+    dfree(h->d_spill_keys);
+    dfree(h->d_spill_cnt);
+    h->spill_cap = 0;
+    h->spill_dirty = true;
 }
 
 // small arrays in d_small (unsigned long long units)
@@ -356,7 +407,7 @@ int copy_to_host(kc_engine* h, kc_engine::HostBuf& hb, const void* src, uint64_t
     if (bytes == 0) return KC_OK;
     void* dst = hb.p;
     if (hb.pinned || hb.touched >= bytes) {
-        HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->st));
+        HIPX(dcopy_async(dst, src, bytes, hipMemcpyDeviceToHost, h->st));
         HIPX(hipStreamSynchronize(h->st));
         return KC_OK;
     }
@@ -366,7 +417,7 @@ int copy_to_host(kc_engine* h, kc_engine::HostBuf& hb, const void* src, uint64_t
     const uint64_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
     auto issue = [&](uint64_t c) -> hipError_t {
         const uint64_t off = c * kStageBytes, m = std::min<uint64_t>(kStageBytes, bytes - off);
-        hipError_t e = hipMemcpyAsync(buf[c & 1], static_cast<const char*>(src) + off, m, hipMemcpyDeviceToHost, h->st);
+        hipError_t e = dcopy_async(buf[c & 1], static_cast<const char*>(src) + off, m, hipMemcpyDeviceToHost, h->st);
         if (e == hipSuccess) e = hipEventRecord(h->emit_ev[c & 1], h->st);
         return e;
     };
@@ -395,51 +446,62 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
             std::fprintf(stderr, "[kmc] key buffers %llu -> %llu keys\n", (unsigned long long)h->key_cap,
                          (unsigned long long)padded_len);
         free_work(h);
-        if (hipMalloc(&h->d_A, padded_len * kb) != hipSuccess || hipMalloc(&h->d_B, padded_len * kb) != hipSuccess ||
-            hipMalloc(&h->d_R, padded_len * kb) != hipSuccess) {
+        if (dmalloc(&h->d_A, padded_len * kb) != hipSuccess || dmalloc(&h->d_B, padded_len * kb) != hipSuccess ||
+            dmalloc(&h->d_R, padded_len * kb) != hipSuccess) {
             free_work(h);
             return h->fail(KC_ERR_NOMEM, "device allocation for the key buffers failed");
         }
         h->key_cap = padded_len;
     }
-    if (!h->d_small) {
-        HIPX(hipMalloc(&h->d_small, sizeof(unsigned long long) * kSmTotal));
-        HIPX(hipMalloc(&h->d_ctr, sizeof(unsigned long long) * kCtrCount));
+    // Every buffer group is allocated whole or not at all: a failed call leaves
+    // null pointers and zero capacities, so the next call allocates again.
+    auto nomem = [&](const char* what) { return h->fail(KC_ERR_NOMEM, std::string("device allocation for ") + what + " failed"); };
+    if (!h->d_small || !h->d_ctr) {
+        dfree(h->d_small);
+        dfree(h->d_ctr);
+        if (dmalloc(&h->d_small, sizeof(unsigned long long) * kSmTotal) != hipSuccess ||
+            dmalloc(&h->d_ctr, sizeof(unsigned long long) * kCtrCount) != hipSuccess) {
+            dfree(h->d_small);
+            dfree(h->d_ctr);
+            return nomem("the bucket counters");
+        }
     }
     if (h->nf_alloc < nf) {
-        hipFree(h->d_fcounts); hipFree(h->d_fcursor); hipFree(h->d_fine_off);
-        hipFree(h->d_foff); hipFree(h->d_roff); hipFree(h->d_bucket_u32);
-        HIPX(hipMalloc(&h->d_fcounts, sizeof(unsigned long long) * kXcds * nf));
-        HIPX(hipMalloc(&h->d_fcursor, sizeof(unsigned long long) * kXcds * nf));
-        HIPX(hipMalloc(&h->d_fine_off, sizeof(unsigned long long) * (nf + 1)));
-        HIPX(hipMalloc(&h->d_foff, sizeof(unsigned long long) * (nf + 1)));
-        HIPX(hipMalloc(&h->d_roff, sizeof(unsigned long long) * (nf + 1)));
-        HIPX(hipMalloc(&h->d_bucket_u32, sizeof(uint32_t) * 3 * nf));
+        free_fine(h);
+        if (dmalloc(&h->d_fcounts, sizeof(unsigned long long) * kXcds * nf) != hipSuccess ||
+            dmalloc(&h->d_fcursor, sizeof(unsigned long long) * kXcds * nf) != hipSuccess ||
+            dmalloc(&h->d_fine_off, sizeof(unsigned long long) * (nf + 1)) != hipSuccess ||
+            dmalloc(&h->d_foff, sizeof(unsigned long long) * (nf + 1)) != hipSuccess ||
+            dmalloc(&h->d_roff, sizeof(unsigned long long) * (nf + 1)) != hipSuccess ||
+            dmalloc(&h->d_bucket_u32, sizeof(uint32_t) * 3 * nf) != hipSuccess) {
+            free_fine(h);
+            return nomem("the fine-bucket arrays");
+        }
         h->nf_alloc = nf;
     }
     if (h->ovf_cap == 0) {
         const uint64_t cap = 1ULL << 20;
-        if (hipMalloc(&h->d_ovf, cap * rec_bytes(h->nw)) != hipSuccess)
-            return h->fail(KC_ERR_NOMEM, "device allocation for the overflow records failed");
+        dfree(h->d_ovf);
+        if (dmalloc(&h->d_ovf, cap * rec_bytes(h->nw)) != hipSuccess) return nomem("the overflow records");
         h->ovf_cap = cap;
     }
     // every listed chunk but the last of a bucket is full: at most K/chunk + nf chunks
     const uint64_t want_heavy = h->key_cap / heavy_chunk_keys(h) + nf + 1;
     if (h->heavy_cap < want_heavy) {
-        hipFree(h->d_heavy);
-        h->d_heavy = nullptr;
+        dfree(h->d_heavy);
         h->heavy_cap = 0;
-        HIPX(hipMalloc(&h->d_heavy, sizeof(unsigned long long) * 2 * want_heavy));
+        if (dmalloc(&h->d_heavy, sizeof(unsigned long long) * 2 * want_heavy) != hipSuccess) return nomem("the heavy-bucket list");
         h->heavy_cap = want_heavy;
     }
     const uint64_t want_spill = std::min<uint64_t>(1ULL << 26, std::max<uint64_t>(1ULL << 16, pow2_at_least(h->key_cap / 512)));
     if (h->spill_cap < want_spill) {
-        hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
-        if (hipMalloc(&h->d_spill_keys, want_spill * 8ULL * table_words(h->nw)) != hipSuccess ||
-            hipMalloc(&h->d_spill_cnt, want_spill * 4ULL) != hipSuccess)
-            return h->fail(KC_ERR_NOMEM, "device allocation for the spill table failed");
+        free_spill(h);
+        if (dmalloc(&h->d_spill_keys, want_spill * 8ULL * table_words(h->nw)) != hipSuccess ||
+            dmalloc(&h->d_spill_cnt, want_spill * 4ULL) != hipSuccess) {
+            free_spill(h);
+            return nomem("the spill table");
+        }
         h->spill_cap = want_spill;
-        h->spill_dirty = true;
     }
     return KC_OK;
 }
@@ -520,12 +582,14 @@ int run_pipeline_t(kc_engine* h)
     if (learned && h->bounds_sbits != sbits) {
         std::vector<uint64_t> rb;
         routing_bounds(h->plan.learned_b, sbits, rb);
-        if (!h->d_bounds) HIPX(hipMalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * ((1u << kMaxSubBits) - 1))));
-        HIPX(hipMemcpy(h->d_bounds, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice));
+        if (!h->d_bounds && dmalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * ((1u << kMaxSubBits) - 1))) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the learned bounds failed");
+        HIPX(dcopy(h->d_bounds, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice));
         std::vector<uint8_t> lut;
         routing_luts(rb, sbits, lut);
-        if (!h->d_lut) HIPX(hipMalloc(&h->d_lut, kRouteLutBytes));
-        HIPX(hipMemcpy(h->d_lut, lut.data(), kRouteLutBytes, hipMemcpyHostToDevice));
+        if (!h->d_lut && dmalloc(&h->d_lut, kRouteLutBytes) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the routing tables failed");
+        HIPX(dcopy(h->d_lut, lut.data(), kRouteLutBytes, hipMemcpyHostToDevice));
         h->bounds_sbits = sbits;
     }
     Route rt{};
@@ -547,7 +611,7 @@ int run_pipeline_t(kc_engine* h)
     HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * kSmTotal, st));
     HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, nullptr, nullptr, st));
     std::vector<unsigned long long> hc((size_t)kXcds * 256);
-    HIPX(hipMemcpyAsync(hc.data(), counts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st));
+    HIPX(dcopy_async(hc.data(), counts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
     std::vector<std::pair<uint32_t, uint32_t>> ranges;
     uint64_t acc = 0;
@@ -751,38 +815,39 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
                 h->graph_key = key;
             }
             HIPX(hipEventRecord(h->ev[0], st));
+            // the spill table counts as written until this run's counters are read back
+            h->spill_dirty = true;
             HIPX(hipGraphLaunch(h->graph_exec, st));
             HIPX(hipEventRecord(h->ev[10], st));
-            h->spill_dirty = false;
         } else {
             const int erc = enqueue(false);
+            h->spill_dirty = true;
             if (erc) return erc;
             HIPX(hipEventRecord(h->ev[10], st));
-            h->spill_dirty = false;
         }
         HIPX(hipStreamSynchronize(st));
 
         unsigned long long ctr[kCtrCount];
-        HIPX(hipMemcpy(ctr, h->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+        HIPX(dcopy(ctr, h->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
         unsigned long long coff256 = 0;
-        HIPX(hipMemcpy(&coff256, coarse_off + 256, sizeof(coff256), hipMemcpyDeviceToHost));
+        HIPX(dcopy(&coff256, coarse_off + 256, sizeof(coff256), hipMemcpyDeviceToHost));
         h->spill_dirty = ctr[kCtrSpilled] != 0;
         if (ctr[kCtrErr] & kErrSpillFull) {
             if (h->spill_cap >= (1ULL << 31)) return h->fail(KC_ERR_CAPACITY, "spill table cannot grow further");
             const uint64_t ncap = std::min<uint64_t>(1ULL << 31, std::max<uint64_t>(h->spill_cap * 4, pow2_at_least(ctr[kCtrSpilled] * 4)));
-            hipFree(h->d_spill_keys); hipFree(h->d_spill_cnt);
-            h->d_spill_keys = nullptr; h->d_spill_cnt = nullptr; h->spill_cap = 0;
-            if (hipMalloc(&h->d_spill_keys, ncap * 8ULL * TableWords<NW>::value) != hipSuccess || hipMalloc(&h->d_spill_cnt, ncap * 4ULL) != hipSuccess)
+            free_spill(h);
+            if (dmalloc(&h->d_spill_keys, ncap * 8ULL * TableWords<NW>::value) != hipSuccess || dmalloc(&h->d_spill_cnt, ncap * 4ULL) != hipSuccess) {
+                free_spill(h);
                 return h->fail(KC_ERR_NOMEM, "spill table growth failed");
+            }
             h->spill_cap = ncap;
-            h->spill_dirty = true;
             continue;
         }
         if (ctr[kCtrErr] & kErrRecFull) {
             const uint64_t ncap = std::max<uint64_t>(h->ovf_cap * 4, ctr[kCtrOvf] + 1024);
-            hipFree(h->d_ovf);
-            h->d_ovf = nullptr; h->ovf_cap = 0;
-            if (hipMalloc(&h->d_ovf, ncap * rec_bytes(NW)) != hipSuccess)
+            dfree(h->d_ovf);
+            h->ovf_cap = 0;
+            if (dmalloc(&h->d_ovf, ncap * rec_bytes(NW)) != hipSuccess)
                 return h->fail(KC_ERR_NOMEM, "overflow record buffer growth failed");
             h->ovf_cap = ncap;
             continue;
@@ -801,7 +866,7 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         h->res_firsts = src;
         h->res_recs = reinterpret_cast<const uint8_t*>(slices);
         h->root_off.assign(2 * 257, 0);
-        HIPX(hipMemcpy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
+        HIPX(dcopy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
         // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total
         // (a graph replay records the total only)
         const int pairs[6][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {9, 10}};
@@ -843,7 +908,7 @@ int emit_chunks(kc_engine* h, const uint8_t* src, uint64_t n, uint64_t isz, F&& 
     char* buf[2] = {h->h_bounce, h->h_bounce2};
     auto issue = [&](uint64_t c) -> hipError_t {
         const uint64_t i = c * per, m = std::min(per, n - i);
-        hipError_t e = hipMemcpyAsync(buf[c & 1], src + i * isz, m * isz, hipMemcpyDeviceToHost, h->st);
+        hipError_t e = dcopy_async(buf[c & 1], src + i * isz, m * isz, hipMemcpyDeviceToHost, h->st);
         if (e == hipSuccess) e = hipEventRecord(h->emit_ev[c & 1], h->st);
         return e;
     };
@@ -877,8 +942,36 @@ const char* kc_last_error(kc_engine* h)
     return (h && !h->err.empty()) ? h->err.c_str() : "";
 }
 
+int kc_device_memory(uint32_t device, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return KC_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_ERR_NODEV;
+    if ((int)device >= ndev) return KC_ERR_ARG;
+    if (hipSetDevice((int)device) != hipSuccess) return KC_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return KC_ERR_HIP;
+    *free_bytes = f;
+    *total_bytes = t;
+    return KC_OK;
+}
+
+int64_t kc_debug_fault(uint32_t kind, int64_t nth)
+{
+    if (kind != kFaultAlloc && kind != kFaultCopy) return -1;
+    g_fault_at[kind].store(nth > 0 ? nth : 0, std::memory_order_relaxed);
+    return g_fault_calls[kind].exchange(0, std::memory_order_relaxed);
+}
+
 int kc_create(const kc_config* cfg, kc_engine** out)
 {
+    // KMC_FAULT_ALLOC=n / KMC_FAULT_COPY=n: the n-th allocation / copy after the
+    // first kc_create of the process fails (the hook of kc_debug_fault)
+    static std::once_flag env_hook;
+    std::call_once(env_hook, [] {
+        if (const char* v = std::getenv("KMC_FAULT_ALLOC")) kc_debug_fault(kFaultAlloc, std::atoll(v));
+        if (const char* v = std::getenv("KMC_FAULT_COPY")) kc_debug_fault(kFaultCopy, std::atoll(v));
+    });
     if (!cfg || !out) return KC_ERR_ARG;
     *out = nullptr;
     const uint32_t nw = kc_words_for_k(cfg->k);
@@ -903,11 +996,11 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     h->cu_count = (uint32_t)cus;  // k_heavy: one persistent workgroup per CU
     if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return bail(KC_ERR_HIP);
     for (int i = 0; i < 2; ++i) {
-        if (hipHostMalloc((void**)&h->h_stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+        if (dhost_malloc((void**)&h->h_stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
         if (hipEventCreateWithFlags(&h->stage_ev[i], hipEventDisableTiming) != hipSuccess) return bail(KC_ERR_HIP);
     }
-    if (hipHostMalloc((void**)&h->h_bounce, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
-    if (hipHostMalloc((void**)&h->h_bounce2, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+    if (dhost_malloc((void**)&h->h_bounce, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
+    if (dhost_malloc((void**)&h->h_bounce2, kEmitChunk, hipHostMallocDefault) != hipSuccess) return bail(KC_ERR_NOMEM);
     for (int i = 0; i < 2; ++i)
         if (hipEventCreateWithFlags(&h->emit_ev[i], hipEventDisableTiming) != hipSuccess) return bail(KC_ERR_HIP);
     for (auto& e : h->ev)
@@ -1036,7 +1129,7 @@ int kc_debug_resident(kc_engine* h, char* out, uint64_t cap, uint64_t* len)
     const uint64_t n = std::min<uint64_t>(cap, h->seq_len);
     if (n) {
         HIPX(hipStreamSynchronize(h->st));
-        HIPX(hipMemcpy(out, h->d_seq, n, hipMemcpyDeviceToHost));
+        HIPX(dcopy(out, h->d_seq, n, hipMemcpyDeviceToHost));
     }
     return KC_OK;
 }
@@ -1204,7 +1297,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
         hipFree(d_hist); hipFree(d_err); free_table();
         d_hist = nullptr; d_err = nullptr;
     };
-    if (hipMalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || hipMalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) {
+    if (dmalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || dmalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) {
         release();
         return h->fail(KC_ERR_NOMEM, "histogram allocation failed");
     }
@@ -1220,7 +1313,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
                 d_vals = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(scratch) + kbytes);
             } else {
                 own_table = true;
-                if (hipMalloc(&d_keys, kbytes) != hipSuccess || hipMalloc(&d_vals, cap * 4ULL) != hipSuccess) {
+                if (dmalloc(&d_keys, kbytes) != hipSuccess || dmalloc(&d_vals, cap * 4ULL) != hipSuccess) {
                     release();
                     return h->fail(KC_ERR_NOMEM, "histogram table allocation failed");
                 }
@@ -1243,7 +1336,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
                 : h->nw == 2 ? launch_histogram<2>(recs, nrec, firsts, nfirst, a, h->st)
                              : launch_histogram<4>(recs, nrec, firsts, nfirst, a, h->st);
         unsigned long long err = 0;
-        if (e == hipSuccess) e = hipMemcpyAsync(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, h->st);
+        if (e == hipSuccess) e = dcopy_async(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, h->st);
         if (e == hipSuccess) e = hipStreamSynchronize(h->st);
         if (e != hipSuccess) {
             release();
@@ -1253,7 +1346,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
             cap *= 2;
             continue;
         }
-        e = hipMemcpy(part.data(), d_hist, sizeof(long long) * nb, hipMemcpyDeviceToHost);
+        e = dcopy(part.data(), d_hist, sizeof(long long) * nb, hipMemcpyDeviceToHost);
         release();
         if (e != hipSuccess) return h->fail(KC_ERR_HIP, std::string("histogram: ") + hipGetErrorString(e));
         for (uint64_t b = 0; b < nb; ++b) hist[b] += part[b];
@@ -1280,8 +1373,8 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
     // add up; each round's outputs go back into R and A, and B holds the table
     for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
         const auto& r = h->rounds[ri];
-        if (r.n_rec) HIPX(hipMemcpy(h->d_R, r.recs.p, r.n_rec * rec_bytes(h->nw), hipMemcpyHostToDevice));
-        if (r.n_first) HIPX(hipMemcpy(h->d_A, r.firsts.p, r.n_first * 8ULL * h->nw, hipMemcpyHostToDevice));
+        if (r.n_rec) HIPX(dcopy(h->d_R, r.recs.p, r.n_rec * rec_bytes(h->nw), hipMemcpyHostToDevice));
+        if (r.n_first) HIPX(dcopy(h->d_A, r.firsts.p, r.n_first * 8ULL * h->nw, hipMemcpyHostToDevice));
         const int rc = histogram_pass(h, precise, min_freq, max_freq, reinterpret_cast<const uint8_t*>(h->d_R), r.n_rec,
                                       h->d_A, r.n_first, h->d_B, buf_bytes, hist);
         if (rc) return rc;
@@ -1342,8 +1435,8 @@ int kc_debug_copy(uint64_t bytes, uint32_t device, float* ms)
     if (bytes == 0 || bytes % 16) return KC_ERR_ARG;
     if (hipSetDevice((int)device) != hipSuccess) return KC_ERR_NODEV;
     void *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, bytes) != hipSuccess) return KC_ERR_NOMEM;
-    if (hipMalloc(&b, bytes) != hipSuccess) { hipFree(a); return KC_ERR_NOMEM; }
+    if (dmalloc(&a, bytes) != hipSuccess) return KC_ERR_NOMEM;
+    if (dmalloc(&b, bytes) != hipSuccess) { hipFree(a); return KC_ERR_NOMEM; }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = KC_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = KC_ERR_HIP;

@@ -7,6 +7,7 @@
 #include <unistd.h>
 #include <zlib.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -71,6 +72,9 @@ public:
     }
 
     int finish() { return publish(); }
+
+    // back to the header line (the Stage-0 sampler's jump to a record start)
+    void reset_line_state() { state_ = 0; }
 
 private:
     static constexpr int kSeq = 1, kPlus = 2;
@@ -252,9 +256,90 @@ int push_fastq_file(kc_engine* h, const char* path)
     }
 }
 
+// Stage-0 sample of a FASTQ file, as FastqPreReader<N>::pre_read reads it
+// (src/FastqPreReader.h:110-165, :128-300): need = min(size on disk, 256 MiB).
+// Plain files: three parts of need/3 bytes at 0, size/2 - part/2 and
+// size - part, each but the first moved to the next "\n@" (:330-347), read
+// max(2 MiB, part) bytes at a time.  The read that completes a part's bytes
+// resets the line state and moves to the next part before its own bytes are
+// parsed (:193-207), so the third part is parsed on from the second part's
+// line state, and a sequence line cut at a part's end runs on into the next
+// part's first sequence line.  Gzip: 2 MiB reads of the decompressed stream
+// while fewer than need bytes were read.  The estimated file size is the size
+// on disk, four times that for gzip (:300-310).
+int stage0_sample(const char* path, uint32_t k, uint64_t block_size, kmc_block_cb cb, void* user, uint64_t* est_size)
+{
+    struct stat sb;
+    if (::stat(path, &sb) != 0) return KC_ERR_ARG;
+    const uint64_t fsize = (uint64_t)sb.st_size;
+    FileSource src;
+    if (src.open(path) != 0) return KC_ERR_ARG;
+    const bool gz = src.gz != nullptr;
+    if (est_size) *est_size = gz ? fsize * 4 : fsize;
+    const uint64_t need = std::min<uint64_t>(fsize, 256ULL << 20);
+    const uint64_t part = need / 3 ? need / 3 : 1;
+    const uint64_t chunk = (part > kReadChunk && !gz) ? part : kReadChunk;
+    FastqChunker ch(k, block_size, cb, user);
+    std::vector<char> buf(chunk);
+    uint64_t have = 0;
+    if (gz) {
+        while (have < need) {
+            const int64_t n = src.read(buf.data(), chunk);
+            if (n < 0) return KC_ERR_ARG;
+            if (n == 0) break;
+            have += (uint64_t)n;
+            const int rc = ch.feed(buf.data(), (uint64_t)n);
+            if (rc) return rc;
+        }
+        return ch.finish();
+    }
+    const uint64_t offs[3] = {0, fsize / 2 >= part / 2 ? fsize / 2 - part / 2 : 0, fsize >= part ? fsize - part : 0};
+    // the first "\n@" in the 64 KiB from `start` on (its '@'), else `start`
+    auto record_start = [&](uint64_t start) -> uint64_t {
+        char look[65536];
+        const ssize_t n = ::pread(src.fd, look, sizeof(look), (off_t)start);
+        if (n <= 1) return start;
+        for (ssize_t i = 1; i < n; ++i)
+            if (look[i - 1] == '\n' && look[i] == '@') return start + (uint64_t)i;
+        return start;
+    };
+    uint64_t pos = 0, seg_read = 0;
+    int seg = 0;
+    for (;;) {
+        uint64_t n = 0;
+        while (n < chunk) {  // one read() of a regular file: full unless at the end
+            const ssize_t r = ::pread(src.fd, buf.data() + n, chunk - n, (off_t)(pos + n));
+            if (r < 0) return KC_ERR_ARG;
+            if (r == 0) break;
+            n += (uint64_t)r;
+        }
+        if (n == 0) break;
+        pos += n;
+        seg_read += n;
+        have += n;
+        if (seg_read >= part && seg < 2) {
+            ch.reset_line_state();
+            ++seg;
+            seg_read = 0;
+            pos = record_start(offs[seg]);
+        }
+        const int rc = ch.feed(buf.data(), n);
+        if (rc) return rc;
+        if (have >= need) break;
+    }
+    return ch.finish();
+}
+
 }  // namespace
 
 extern "C" {
+
+int kmc_stage0_sample_file(const char* path, uint32_t k, uint64_t block_size, kmc_block_cb cb, void* user,
+                           uint64_t* est_file_size)
+{
+    if (!path || !cb || block_size <= 128 || k == 0 || k > 128) return KC_ERR_ARG;
+    return stage0_sample(path, k, block_size, cb, user, est_file_size);
+}
 
 int kmc_fastq_blocks(const char* fastq, uint64_t len, uint32_t k, uint64_t block_size, kmc_block_cb cb, void* user)
 {
@@ -366,25 +451,60 @@ int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
         std::fprintf(stderr, "Failed to open file\n");
         return KC_ERR_ARG;
     }
+
+    // Stage 0 first, into host memory: the reference's sample (or the first
+    // sample_bytes) and its estimated file size
+    struct Sample {
+        std::vector<char> bytes;
+        std::vector<uint64_t> lens;
+    } sample;
+    auto keep = [](void* user, const char* data, uint64_t len) -> int {
+        Sample* sp = static_cast<Sample*>(user);
+        sp->bytes.insert(sp->bytes.end(), data, data + len);
+        sp->lens.push_back(len);
+        return 0;
+    };
+    uint64_t est_file = 0;
+    int rc = o->sample_bytes ? kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, o->sample_bytes, keep, &sample)
+                             : kmc_stage0_sample_file(o->fastq_path, o->k, 32ULL << 10, keep, &sample, &est_file);
+    if (rc) return rc;
+    if (!est_file) est_file = (uint64_t)sb.st_size;
+    // sequence bytes per FASTQ byte, from the head of the file: the resident
+    // sequence buffer is sized from it (it still grows if the estimate is short)
+    uint64_t head_seq = 0;
+    auto count_seq = [](void* user, const char*, uint64_t len) -> int {
+        *static_cast<uint64_t*>(user) += len;
+        return 0;
+    };
+    const uint64_t head = 4ULL << 20;
+    rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, head, count_seq, &head_seq);
+    if (rc) return rc;
+    const double ratio = std::min(1.0, (double)head_seq / (double)std::max<uint64_t>(1, std::min(head, est_file)));
+
     kc_config cfg{};
     cfg.k = o->k;
     cfg.partition = o->partition;
     cfg.counter = o->counter;
     cfg.min_count = o->min_count ? o->min_count : 1;
     cfg.max_count = o->max_count;
-    cfg.expected_bytes = (uint64_t)sb.st_size;
+    cfg.expected_bytes = (uint64_t)(ratio * (double)est_file * 1.02) + (1ULL << 20);
+    cfg.est_file_size = est_file;
     // classifier thread count of the reference split (src/main.cpp:114-117)
     const uint32_t n = o->n_thread ? o->n_thread : 8;
     const uint32_t parsers = n / 8 > 0 ? n / 8 : 1;
     const uint32_t budget = n > 2 + parsers ? n - 2 - parsers : 1;
     cfg.classifier_num = budget / 2 ? budget / 2 : 1;
     kc_engine* h = nullptr;
-    int rc = kc_create(&cfg, &h);
+    rc = kc_create(&cfg, &h);
     if (rc) return rc;
-    // Stage 0: head of the file (the reference samples up to 256 MiB, src/FastqPreReader.h:110)
-    PushCtx sc{h, 0, kc_sample_block};
-    const uint64_t sample = o->sample_bytes ? o->sample_bytes : (256ULL << 20);
-    rc = kmc_fastq_blocks_file(o->fastq_path, o->k, 32ULL << 10, sample, push_cb, &sc);
+    {
+        uint64_t at = 0;
+        for (uint64_t len : sample.lens) {
+            if (!rc) rc = kc_sample_block(h, sample.bytes.data() + at, len);
+            at += len;
+        }
+        Sample().bytes.swap(sample.bytes);
+    }
     if (!rc) rc = kc_plan(h);
     const auto t1 = clk::now();
     PushCtx pc{h, 0, kc_push_block};
@@ -418,6 +538,18 @@ int kmc_tree_run(const kmc_tree_opts* o, kc_stats* stats, double* wall_ms)
     }
     auto us = [](clk::duration d) { return (long long)std::chrono::duration_cast<std::chrono::microseconds>(d).count(); };
     if (o->verbose) {
+        // Stage-0 plan lines as the reference prints them (src/main.cpp:77-82, :106, :186)
+        uint64_t sum = 0, max_cap = 0;
+        std::vector<uint64_t> loads(cfg.classifier_num, 0);
+        for (int p = 0; p < 256; ++p) {
+            sum += st.prefix_counts[p];
+            max_cap = std::max<uint64_t>(max_cap, st.bloom_capacity[p]);
+            if (st.prefix_owners[p] < loads.size()) loads[st.prefix_owners[p]] += st.prefix_counts[p];
+        }
+        std::printf("Average prefix count: %llu\n", (unsigned long long)(sum / 256));
+        for (uint32_t c = 0; c < cfg.classifier_num; ++c)
+            std::printf("Classifier %u total prefix count: %llu\n", c, (unsigned long long)loads[c]);
+        std::printf("Max Bloom Filter capacity: %llu\n", (unsigned long long)max_cap);
         std::printf("Total read k-mer count: %llu\n", (unsigned long long)st.total_kmers);
         std::printf("Init elapsed us: %lld\n", us(t1 - t0));
         std::printf("Mid elapsed us: %lld\n", us(t2 - t1));

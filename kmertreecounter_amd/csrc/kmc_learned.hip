@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "kmc_fault.h"
 #include "kmc_learned.h"
 
 namespace kmc {
@@ -440,7 +441,7 @@ struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { if (p) hipFree(p); }
     template <typename T> T* as() { return reinterpret_cast<T*>(p); }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+    hipError_t alloc(size_t bytes) { return dmalloc(&p, bytes); }
 };
 
 #define LCHK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) { err = std::string(#expr) + ": " + hipGetErrorString(e_); return -2; } } while (0)
@@ -544,7 +545,7 @@ static int mlp_loss_grad(MlpWork& w, uint32_t nb, hipStream_t st, std::string& e
 
 static int upload_params(MlpWork& w, const float* params, hipStream_t st, std::string& err)
 {
-    LCHK(hipMemcpyAsync(w.params.p, params, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice, st));
+    LCHK(dcopy_async(w.params.p, params, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_to_bf16_copies, dim3((kMlpNParam + 255) / 256), dim3(256), 0, st, w.params.as<float>(),
                        (uint32_t)kMlpNParam, w.w1b.as<__bf16>(), w.w2b.as<__bf16>(), w.w3b.as<__bf16>(),
                        w.w3tb.as<__bf16>());
@@ -578,13 +579,13 @@ int mlp_debug_loss_grad(const float* params, const uint64_t* keys, const uint32_
     if (w.init(nb, err)) return -3;
     hipStream_t st = nullptr;
     if (upload_params(w, params, st, err)) return -2;
-    LCHK(hipMemcpy(w.keys.p, keys, 8ULL * nb, hipMemcpyHostToDevice));
-    LCHK(hipMemcpy(w.labels.p, labels, 4ULL * nb, hipMemcpyHostToDevice));
+    LCHK(dcopy(w.keys.p, keys, 8ULL * nb, hipMemcpyHostToDevice));
+    LCHK(dcopy(w.labels.p, labels, 4ULL * nb, hipMemcpyHostToDevice));
     LCHK(hipMemset(w.grad.p, 0, sizeof(float) * kMlpNParam));
     if (mlp_loss_grad(w, nb, st, err)) return -2;
     LCHK(hipDeviceSynchronize());
-    LCHK(hipMemcpy(grad, w.grad.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost));
-    LCHK(hipMemcpy(loss, w.loss.p, 4, hipMemcpyDeviceToHost));
+    LCHK(dcopy(grad, w.grad.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost));
+    LCHK(dcopy(loss, w.loss.p, 4, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -595,11 +596,11 @@ int mlp_debug_gemm(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, 
     LCHK(a.alloc(2ULL * M * K));
     LCHK(b.alloc(2ULL * N * K));
     LCHK(c.alloc(4ULL * M * N));
-    LCHK(hipMemcpy(a.p, A, 2ULL * M * K, hipMemcpyHostToDevice));
-    LCHK(hipMemcpy(b.p, Bt, 2ULL * N * K, hipMemcpyHostToDevice));
+    LCHK(dcopy(a.p, A, 2ULL * M * K, hipMemcpyHostToDevice));
+    LCHK(dcopy(b.p, Bt, 2ULL * N * K, hipMemcpyHostToDevice));
     LCHK(gemm_nt(a.as<__bf16>(), b.as<__bf16>(), c.as<float>(), M, N, K, K, K, N, nullptr));
     LCHK(hipDeviceSynchronize());
-    LCHK(hipMemcpy(C, c.p, 4ULL * M * N, hipMemcpyDeviceToHost));
+    LCHK(dcopy(C, c.p, 4ULL * M * N, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -630,12 +631,12 @@ int mlp_train(const uint64_t* d_sorted, uint64_t n, const MlpTrainOpts& o, hipSt
         LCHK(hipGetLastError());
         if (step % 50 == 0 || step + 1 == o.steps) {
             float l = 0;
-            LCHK(hipMemcpyAsync(&l, w.loss.p, 4, hipMemcpyDeviceToHost, st));
+            LCHK(dcopy_async(&l, w.loss.p, 4, hipMemcpyDeviceToHost, st));
             LCHK(hipStreamSynchronize(st));
             loss_curve.push_back(l);
         }
     }
-    LCHK(hipMemcpyAsync(params.data(), w.params.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost, st));
+    LCHK(dcopy_async(params.data(), w.params.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -653,7 +654,7 @@ int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, bool ex
         const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, n - i);
         const uint32_t mp = (m + 63) / 64 * 64;
         LCHK(hipMemsetAsync(w.keys.p, 0, 8ULL * mp, st));
-        LCHK(hipMemcpyAsync(w.keys.p, d_keys + i, 8ULL * m, hipMemcpyDeviceToDevice, st));
+        LCHK(dcopy_async(w.keys.p, d_keys + i, 8ULL * m, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(k_mlp_hidden, dim3(mp / kSpb), dim3(256), 0, st, w.keys.as<uint64_t>(), mp,
                            w.params.as<float>(), w.w1b.as<__bf16>(), w.w2b.as<__bf16>(), w.h1.as<float>(),
                            w.h2.as<__bf16>(), (__bf16*)nullptr, mp);
@@ -669,7 +670,7 @@ int mlp_predict(const float* params, const uint64_t* d_keys, uint64_t n, bool ex
         LCHK(hipGetLastError());
     }
     pred.resize(n);
-    LCHK(hipMemcpyAsync(pred.data(), out.p, 4ULL * n, hipMemcpyDeviceToHost, st));
+    LCHK(dcopy_async(pred.data(), out.p, 4ULL * n, hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -681,7 +682,7 @@ int mlp_debug_fit(const uint64_t* sorted, uint64_t n, const MlpTrainOpts& o, uin
     if (o.distinct) keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     DevBuf d;
     LCHK(d.alloc(8ULL * keys.size()));
-    LCHK(hipMemcpy(d.p, keys.data(), 8ULL * keys.size(), hipMemcpyHostToDevice));
+    LCHK(dcopy(d.p, keys.data(), 8ULL * keys.size(), hipMemcpyHostToDevice));
     std::vector<float> params, curve;
     int rc = mlp_train(d.as<uint64_t>(), keys.size(), o, nullptr, params, curve, err);
     if (rc) return rc;

@@ -9,6 +9,7 @@
 
 #include <chrono>
 
+#include "kmc_fault.h"
 #include "kmc_kernels.h"
 #include "kmc_learned.h"
 
@@ -91,19 +92,19 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
         uint8_t* d_seq = nullptr;
         unsigned long long* d_counts = nullptr;
         auto cleanup = [&]() { hipFree(d_seq); hipFree(d_counts); };
-        if (hipMalloc(&d_seq, padded + 256) != hipSuccess || hipMalloc(&d_counts, sizeof(unsigned long long) * kXcds * 256) != hipSuccess) {
+        if (dmalloc(&d_seq, padded + 256) != hipSuccess || dmalloc(&d_counts, sizeof(unsigned long long) * kXcds * 256) != hipSuccess) {
             cleanup();
             err = "stage 0: device allocation failed";
             return KC_ERR_NOMEM;
         }
-        hipError_t e = hipMemcpyAsync(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
+        hipError_t e = dcopy_async(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
         if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
         if (e == hipSuccess) e = hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * kXcds * 256, st);
         Route rt{};
         rt.rhi = 256;  // every coarse bucket
         if (e == hipSuccess) e = extract(nw, false, d_seq, padded, k, rt, d_counts, nullptr, nullptr, st);
         std::vector<unsigned long long> h(kXcds * 256);
-        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_counts, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = dcopy_async(h.data(), d_counts, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         cleanup();
         if (e != hipSuccess) {
@@ -119,7 +120,8 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
     }
     clamp_prefix_counts(out.prefix_counts);
     lpt_assign(out.prefix_counts, cfg.classifier_num ? cfg.classifier_num : 1, out.owners);
-    bloom_capacity(out.prefix_counts, cfg.expected_bytes ? cfg.expected_bytes : sample.size(), out.bloom_capacity);
+    const uint64_t est = cfg.est_file_size ? cfg.est_file_size : cfg.expected_bytes ? cfg.expected_bytes : sample.size();
+    bloom_capacity(out.prefix_counts, est, out.bloom_capacity);
     return KC_OK;
 }
 
@@ -153,8 +155,8 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     unsigned long long* d_small = nullptr;
     uint64_t* d_keys = nullptr;
     auto cleanup = [&]() { hipFree(d_seq); hipFree(d_small); hipFree(d_keys); };
-    if (hipMalloc(&d_seq, padded + 256) != hipSuccess || hipMalloc(&d_small, sizeof(unsigned long long) * (16 * 256 + 600)) != hipSuccess ||
-        hipMalloc(&d_keys, padded * 8ULL * nw) != hipSuccess) {
+    if (dmalloc(&d_seq, padded + 256) != hipSuccess || dmalloc(&d_small, sizeof(unsigned long long) * (16 * 256 + 600)) != hipSuccess ||
+        dmalloc(&d_keys, padded * 8ULL * nw) != hipSuccess) {
         cleanup();
         err = "learned plan: device allocation failed";
         return KC_ERR_NOMEM;
@@ -165,19 +167,19 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     unsigned long long* tstart = coff + 257;
     Route rt{};
     rt.rhi = 256;  // every coarse bucket
-    hipError_t e = hipMemcpyAsync(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
+    hipError_t e = dcopy_async(d_seq, sample.data(), sample.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess && padded > sample.size()) e = hipMemsetAsync(d_seq + sample.size(), '\n', padded - sample.size(), st);
     if (e == hipSuccess) e = hipMemsetAsync(d_small, 0, sizeof(unsigned long long) * (16 * 256 + 600), st);
     if (e == hipSuccess) e = extract(nw, false, d_seq, padded, k, rt, counts, cursor, nullptr, st);
     if (e == hipSuccess) e = launch_scan_coarse(counts, coff, cursor, tstart, 4096, st);
     if (e == hipSuccess) e = extract(nw, true, d_seq, padded, k, rt, counts, cursor, d_keys, st);
     unsigned long long nkeys = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&nkeys, coff + 256, sizeof(nkeys), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = dcopy_async(&nkeys, coff + 256, sizeof(nkeys), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     std::vector<uint64_t> all;
     if (e == hipSuccess && nkeys) {
         all.resize(nkeys * nw);
-        e = hipMemcpy(all.data(), d_keys, nkeys * 8ULL * nw, hipMemcpyDeviceToHost);
+        e = dcopy(all.data(), d_keys, nkeys * 8ULL * nw, hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) {
         cleanup();
@@ -209,7 +211,7 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
         out.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return KC_OK;
     }
-    e = hipMemcpy(d_keys, train.data(), train.size() * 8, hipMemcpyHostToDevice);
+    e = dcopy(d_keys, train.data(), train.size() * 8, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         cleanup();
         err = std::string("learned plan: ") + hipGetErrorString(e);

@@ -28,9 +28,15 @@
  *                        reference Tree low.bin round trip.
  *   orc_histogram_*    - reference histogram_tool KATs ({1,1,1,0}, {1,0,0},
  *                        {0,1,0}, {1,-1,1,-1}) + closed form from the table.
- *   orc_lpt/orc_bloom_capacity - hand-computed cases; tie order of the
- *                        reference std::sort is not pinned (affects ownership
- *                        only, never counts).
+ *   orc_pre_read, orc_prefix_counts, orc_clamp_prefix_counts, orc_lpt,
+ *   orc_bloom_capacity - the Stage-0 lines the reference Tree prints (average
+ *                        clamped prefix count, every classifier's LPT load,
+ *                        largest Bloom capacity) on plain, gzip and small
+ *                        inputs (tests/golden/stage0_runs.json); prefix
+ *                        counts also against numpy bincount and sum == K;
+ *                        hand-computed LPT and capacity cases.  The
+ *                        reference's unstable std::sort leaves the owner of
+ *                        equal-count prefixes open, never the set of loads.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -120,71 +126,172 @@ static void ob_store_overlap(orc_blocks_t* b, uint64_t overlap)
 }
 
 /*
- * Convert FASTQ bytes into the reader's sequence-only blocks.  The 4-line state
- * machine (src/FastqReader.h:293-308) keeps only line 2 of each record; every
- * sequence line ends with '\n' inside the block; a line split at a full block
- * carries its last k-1 bytes (or the bytes after the last '\n', if nearer) into
- * the next block.  The reference reads the file in 2 MiB chunks; the output does
- * not depend on the chunk size, so the whole input is one chunk here.
- * Returns 0, or -1 when out/lens capacity is too small.
+ * The reader's line state machine over one buffer of FASTQ bytes
+ * (src/FastqReader.h:140-308).  The 4-line state machine (:293-308) keeps only
+ * line 2 of each record; every sequence line ends with '\n' inside the block;
+ * a line split at a full block carries its last k-1 bytes (or the bytes after
+ * the last '\n', if nearer) into the next block.  *state (0 header,
+ * 1 sequence, 2 plus, 3 quality) persists across buffers, as it does across
+ * the reference's 2 MiB reads.  Returns 0, or -1 when out/lens are too small.
  */
-ORC_API int orc_fastq_blocks(const char* fq, uint64_t fq_len, uint32_t k, uint64_t block_size,
-                             char* out, uint64_t out_cap, uint64_t* lens, uint64_t lens_cap,
-                             uint64_t* n_blocks, uint64_t* out_len)
+static int ob_feed(orc_blocks_t* b, const char* fq, uint64_t fq_len, int* state, uint64_t overlap)
 {
-    orc_blocks_t b;
-    memset(&b, 0, sizeof(b));
-    b.out = out; b.out_cap = out_cap; b.lens = lens; b.lens_cap = lens_cap;
-    b.block_size = block_size; b.last_nl = ORC_NO_NL;
-    const uint64_t overlap = k > 1 ? (uint64_t)(k - 1) : 0;
-    int state = 0; /* 0 header, 1 sequence, 2 plus, 3 quality */
     uint64_t pos = 0;
-    for (;;) {
-        if (!b.has_block) ob_acquire(&b);
-        if (b.overflow) return -1;
-        if (pos >= fq_len) { ob_publish(&b); break; }
-        if (state != 1) {
+    while (pos < fq_len) {
+        if (!b->has_block) ob_acquire(b);
+        if (b->overflow) return -1;
+        if (*state != 1) {
             const char* nl = memchr(fq + pos, '\n', fq_len - pos);
             if (!nl) { pos = fq_len; continue; }
             pos = (uint64_t)(nl - fq) + 1;
-            state = (state + 1) & 3;
+            *state = (*state + 1) & 3;
             continue;
         }
         const char* nl = memchr(fq + pos, '\n', fq_len - pos);
         uint64_t seq_len = nl ? (uint64_t)(nl - (fq + pos)) : fq_len - pos;
         uint64_t copied = 0;
         while (copied < seq_len) {
-            if (b.write_size == block_size) {
-                ob_store_overlap(&b, overlap);
-                ob_publish(&b);
-                ob_acquire(&b);
-                if (b.overflow) return -1;
+            if (b->write_size == b->block_size) {
+                ob_store_overlap(b, overlap);
+                ob_publish(b);
+                ob_acquire(b);
+                if (b->overflow) return -1;
             }
-            uint64_t room = block_size - b.write_size;
+            uint64_t room = b->block_size - b->write_size;
             uint64_t n = seq_len - copied < room ? seq_len - copied : room;
-            if (b.cur_start + b.write_size + n > out_cap) return -1;
-            memcpy(out + b.cur_start + b.write_size, fq + pos + copied, n);
-            b.write_size += n;
+            if (b->cur_start + b->write_size + n > b->out_cap) return -1;
+            memcpy(b->out + b->cur_start + b->write_size, fq + pos + copied, n);
+            b->write_size += n;
             copied += n;
         }
         pos += seq_len;
         if (!nl) continue;
-        if (b.write_size == block_size) {
-            b.carry_len = 0;
-            ob_publish(&b);
-            ob_acquire(&b);
-            if (b.overflow) return -1;
+        if (b->write_size == b->block_size) {
+            b->carry_len = 0;
+            ob_publish(b);
+            ob_acquire(b);
+            if (b->overflow) return -1;
         }
-        if (b.cur_start + b.write_size + 1 > out_cap) return -1;
-        b.last_nl = b.write_size;
-        out[b.cur_start + b.write_size++] = '\n';
+        if (b->cur_start + b->write_size + 1 > b->out_cap) return -1;
+        b->last_nl = b->write_size;
+        b->out[b->cur_start + b->write_size++] = '\n';
         pos++;
-        state = (state + 1) & 3;
+        *state = (*state + 1) & 3;
     }
-    if (b.overflow) return -1;
-    *n_blocks = b.n_blocks;
-    *out_len = b.out_len;
     return 0;
+}
+
+static int ob_finish(orc_blocks_t* b, uint64_t* n_blocks, uint64_t* out_len)
+{
+    if (!b->has_block) ob_acquire(b);
+    ob_publish(b);
+    if (b->overflow) return -1;
+    *n_blocks = b->n_blocks;
+    *out_len = b->out_len;
+    return 0;
+}
+
+static void ob_init(orc_blocks_t* b, uint64_t block_size, char* out, uint64_t out_cap, uint64_t* lens, uint64_t lens_cap)
+{
+    memset(b, 0, sizeof(*b));
+    b->out = out; b->out_cap = out_cap; b->lens = lens; b->lens_cap = lens_cap;
+    b->block_size = block_size; b->last_nl = ORC_NO_NL;
+}
+
+/*
+ * Convert FASTQ bytes into the reader's sequence-only blocks
+ * (FastqReader<N>::read, src/FastqReader.h:140-290).  The reference reads the
+ * file in 2 MiB chunks; the output does not depend on the chunk size, so the
+ * whole input is one buffer here.
+ */
+ORC_API int orc_fastq_blocks(const char* fq, uint64_t fq_len, uint32_t k, uint64_t block_size,
+                             char* out, uint64_t out_cap, uint64_t* lens, uint64_t lens_cap,
+                             uint64_t* n_blocks, uint64_t* out_len)
+{
+    orc_blocks_t b;
+    ob_init(&b, block_size, out, out_cap, lens, lens_cap);
+    int state = 0;
+    if (ob_feed(&b, fq, fq_len, &state, k > 1 ? (uint64_t)(k - 1) : 0)) return -1;
+    return ob_finish(&b, n_blocks, out_len);
+}
+
+/* find_next_record_start (src/FastqPreReader.h:330-347): the first "\n@" in
+ * the 64 KiB from `start` on; the record starts at its '@' (a '\n' at
+ * `start` - 1 is not looked at).  No match: `start` itself. */
+static uint64_t pre_record_start(const char* fq, uint64_t fq_len, uint64_t start)
+{
+    if (start >= fq_len) return start;
+    uint64_t n = fq_len - start < 65536 ? fq_len - start : 65536;
+    if (n <= 1) return start;
+    for (uint64_t i = 1; i < n; ++i)
+        if (fq[start + i - 1] == '\n' && fq[start + i] == '@') return start + i;
+    return start;
+}
+
+/*
+ * Stage-0 sample blocks: FastqPreReader<N>::pre_read (src/FastqPreReader.h:
+ * 110-165, :128-300).  `fq` is the whole (decompressed) input.  need =
+ * min(file_size, 256 MiB), where file_size is the size on disk (the compressed
+ * size for gzip, as fstat gives it, :79-86).
+ *   plain: three parts of need/3 bytes (at least 1) at 0, file_size/2 -
+ *     part/2 and file_size - part (clamped at 0), each but the first moved to
+ *     the next "\n@".  Reads are max(2 MiB, part) bytes (:150-154); a read that
+ *     completes a part's byte count resets the line state to the header and
+ *     seeks to the next part BEFORE its own bytes are parsed (:193-207), so
+ *     the state is not reset between the second and third parts, and a
+ *     sequence line cut at a part's end runs on into the next part's first
+ *     sequence line.  Reading stops once need bytes were read (:209-212).
+ *   gzip: 2 MiB reads from the start while fewer than need bytes were read;
+ *     the read that finds need reached is dropped (:180-190).
+ * Blocks are cut as in orc_fastq_blocks.  Returns 0 or -1 (capacity).
+ */
+ORC_API int orc_pre_read(const char* fq, uint64_t fq_len, uint64_t file_size, int is_gz, uint32_t k,
+                         uint64_t block_size, char* out, uint64_t out_cap, uint64_t* lens, uint64_t lens_cap,
+                         uint64_t* n_blocks, uint64_t* out_len)
+{
+    const uint64_t cap256 = 256ULL << 20;
+    const uint64_t need = file_size < cap256 ? file_size : cap256;
+    uint64_t part = need / 3;
+    if (part == 0) part = 1;
+    uint64_t chunk = 2ULL << 20;
+    if (part > chunk && !is_gz) chunk = part;
+    const uint64_t overlap = k > 1 ? (uint64_t)(k - 1) : 0;
+    orc_blocks_t b;
+    ob_init(&b, block_size, out, out_cap, lens, lens_cap);
+    int state = 0;
+    uint64_t pos = 0, have = 0;
+    if (is_gz) {
+        while (have < need && pos < fq_len) {
+            const uint64_t n = fq_len - pos < chunk ? fq_len - pos : chunk;
+            have += n;
+            if (ob_feed(&b, fq + pos, n, &state, overlap)) return -1;
+            pos += n;
+        }
+        return ob_finish(&b, n_blocks, out_len);
+    }
+    uint64_t offs[3];
+    offs[0] = 0;
+    offs[1] = file_size / 2 >= part / 2 ? file_size / 2 - part / 2 : 0;
+    offs[2] = file_size >= part ? file_size - part : 0;
+    int seg = 0;
+    uint64_t seg_read = 0;
+    for (;;) {
+        if (pos >= fq_len) break;
+        const uint64_t n = fq_len - pos < chunk ? fq_len - pos : chunk;
+        const uint64_t at = pos;
+        pos += n;
+        seg_read += n;
+        have += n;
+        if (seg_read >= part && seg < 2) {
+            state = 0;
+            ++seg;
+            seg_read = 0;
+            pos = pre_record_start(fq, fq_len, offs[seg]);
+        }
+        if (ob_feed(&b, fq + at, n, &state, overlap)) return -1;
+        if (have >= need) break;
+    }
+    return ob_finish(&b, n_blocks, out_len);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -385,6 +492,8 @@ ORC_API uint64_t orc_count_sorted(uint64_t* keys, uint64_t n, uint32_t nw, uint3
 /* :6927, XXH3_len_17to128_128b with XXH128_mix32B, XXH3_avalanche :4616).   */
 /* Only the secret bytes these paths read (offsets 0..63 of XXH3_kSecret,    */
 /* :4398) are needed.                                                        */
+/* The secret bytes and the XXH64/XXH3 prime constants below are those of    */
+/* xxHash, Copyright (C) 2012-2023 Yann Collet, BSD 2-Clause License.         */
 /* ------------------------------------------------------------------------- */
 
 static const uint8_t kSecret0to16[16] = {

@@ -57,6 +57,7 @@ def oracle() -> ctypes.CDLL:
     L.orc_clamp_prefix_counts.argtypes = [vp]
     L.orc_lpt.argtypes = [vp, u32, vp, vp]
     L.orc_bloom_capacity.argtypes = [vp, u64, vp]
+    L.orc_pre_read.argtypes = [cp, u64, u64, ctypes.c_int, u32, u64, vp, u64, vp, u64, vp, vp]
     return L
 
 
@@ -100,6 +101,50 @@ def extract_blocks(blocks: list[bytes], k: int) -> np.ndarray:
     n = O.orc_extract_blocks(data, lens.ctypes.data, len(blocks), k, out.ctypes.data, cap)
     assert n != UMAX
     return out[: n * nw].reshape(n, nw)
+
+
+def pre_read_blocks(fq: bytes, k: int, file_size: int | None = None, gz: bool = False,
+                    block_size: int = 32 * 1024) -> list[bytes]:
+    """Oracle Stage-0 sample blocks of a FASTQ file (FastqPreReader::pre_read);
+    fq is the decompressed content, file_size the size on disk."""
+    O = oracle()
+    fs = len(fq) if file_size is None else file_size
+    cap = 2 * len(fq) + (1 << 20)
+    out = ctypes.create_string_buffer(cap)
+    lens = np.zeros(cap // 64 + 16, dtype=np.uint64)
+    nb, ol = u64(), u64()
+    rc = O.orc_pre_read(fq, len(fq), fs, 1 if gz else 0, k, block_size, out, cap, lens.ctypes.data, lens.size,
+                        ctypes.byref(nb), ctypes.byref(ol))
+    assert rc == 0
+    raw = out.raw[: ol.value]
+    blocks, at = [], 0
+    for n in lens[: nb.value]:
+        blocks.append(raw[at: at + int(n)])
+        at += int(n)
+    return blocks
+
+
+def stage0_plan(keys: np.ndarray, classifiers: int, est_file_size: int):
+    """Oracle Stage-0 plan of sampled keys: clamped prefix counts, their
+    average, sorted LPT loads and Bloom capacities (src/main.cpp:38-108,175-184)."""
+    O = oracle()
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    counts = np.zeros(256, dtype=np.uint64)
+    O.orc_prefix_counts(keys.ctypes.data, keys.shape[0], keys.shape[1] if keys.ndim == 2 else 1, counts.ctypes.data)
+    avg = int(O.orc_clamp_prefix_counts(counts.ctypes.data))
+    owners = np.zeros(256, dtype=np.uint8)
+    loads = np.zeros(classifiers, dtype=np.uint64)
+    O.orc_lpt(counts.ctypes.data, classifiers, owners.ctypes.data, loads.ctypes.data)
+    caps = np.zeros(256, dtype=np.uint64)
+    O.orc_bloom_capacity(counts.ctypes.data, est_file_size, caps.ctypes.data)
+    return counts, avg, np.sort(loads), caps
+
+
+def classifiers_for_threads(n_thread: int) -> int:
+    """classifier_num of the reference's thread split (src/main.cpp:114-117)."""
+    parsers = max(1, n_thread // 8)
+    budget = n_thread - 2 - parsers
+    return max(1, int(budget / 2.0))
 
 
 def count_table(keys: np.ndarray) -> tuple[np.ndarray, np.ndarray]:

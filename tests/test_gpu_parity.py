@@ -247,21 +247,84 @@ def test_reference_fixture(name, counter):
     assert got == fx["histogram_nonzero"]
 
 
+def check_filtered(firsts, recs, stats, k, exp_keys, exp_counts, minc, maxc):
+    """The exact record set of append_export_record (src/NewKmerTree.h:1159-1168):
+    a key of count c whose first occurrence went to low.bin has the repeat count
+    r = c - 1, any other key r = c; it has a root record, of count r, exactly
+    when max(1, min) <= r <= max.  The firsts are input keys, each at most once,
+    and the unfiltered record count is the number of keys with r >= 1."""
+    nw = C.nwords(k)
+    maxc = maxc or 0xFFFFFFFF
+    assert stats.total_kmers == int(exp_counts.sum(dtype=np.uint64))
+    erows = as_rows(exp_keys)
+    frows = as_rows(firsts.reshape(-1, nw).copy())
+    assert np.unique(frows).size == frows.size
+    pos = np.searchsorted(erows, frows)
+    assert (pos < erows.size).all() and (erows[np.minimum(pos, erows.size - 1)] == frows).all()
+    in_f = np.zeros(erows.size, dtype=bool)
+    in_f[pos] = True
+    r = exp_counts.astype(np.int64) - in_f
+    assert stats.firsts == frows.size
+    assert stats.records_unfiltered == int((r >= 1).sum())
+    keep = (r >= max(1, minc)) & (r <= maxc)
+    assert stats.records == len(recs) == int(keep.sum())
+    rk = as_rows(recs["key"].reshape(-1, nw)) if len(recs) else erows[:0]
+    order = np.argsort(rk, kind="stable")
+    np.testing.assert_array_equal(rk[order], erows[keep])
+    np.testing.assert_array_equal(recs["count"][order].astype(np.int64), r[keep])
+    if len(recs):
+        assert (recs["pad"] == 0).all()
+
+
+# paths of the counting pipeline the record filter must hold on
+FILTER_PATHS = {
+    "buckets": dict(),
+    "passes": dict(sub_bits=2, pass_keys=2000),
+    "heavy": dict(sub_bits=K.KC_SUB_BITS_NONE, heavy_keys=1000),
+    "rounds": dict(max_round_keys=1_500_000),
+}
+
+
 @COUNTERS
-def test_min_max_filter(counter):
+@pytest.mark.parametrize("path", sorted(FILTER_PATHS))
+@pytest.mark.parametrize("minc,maxc", [(3, 9), (1, 1), (2, 0)], ids=["3-9", "1-1", "2-max"])
+def test_min_max_filter_exact_set(counter, path, minc, maxc):
     seq = K.generate(1, scale=20, seed=4, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
-    with K.Engine(31, min_count=3, max_count=9, counter=counter) as e:
+    with K.Engine(31, min_count=minc, max_count=maxc, counter=counter, **FILTER_PATHS[path]) as e:
         e.push_block(seq)
         e.finish()
-        f = e.firsts()
-        r = e.records()
         s = e.stats()
-    # records obey the filter on the repeat count (src/NewKmerTree.h:1159-1168)
-    assert ((r["count"] >= 3) & (r["count"] <= 9)).all()
-    # unfiltered, each repeated key would have a record; the filter keeps a subset
-    assert s.records <= s.records_unfiltered
-    assert s.total_kmers == int(counts.sum())
+        if path == "heavy":
+            assert s.heavy_buckets > 0
+        if path == "rounds":
+            assert s.rounds >= 2
+        check_filtered(e.firsts(), e.records(), s, 31, keys, counts, minc, maxc)
+
+
+def test_min_max_filter_spill_path():
+    # Bloom counter, one pass per 190k-key bucket: keys that find no LDS slot
+    # live in the global spill table, whose records go through the same filter
+    seq = K.generate(1, scale=1, seed=1, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    with K.Engine(31, min_count=2, max_count=40, sub_bits=K.KC_SUB_BITS_NONE, pass_keys=1 << 30,
+                  counter=K.KC_COUNTER_BLOOM) as e:
+        e.push_block(seq)
+        e.finish()
+        s = e.stats()
+        assert s.spilled_keys > 0
+        check_filtered(e.firsts(), e.records(), s, 31, keys, counts, 2, 40)
+
+
+@COUNTERS
+def test_min_max_filter_long_keys(counter):
+    seq = K.generate(1, scale=40, seed=9, fmt="seq")
+    for k in (63, 100):
+        keys, counts = C.count_table(C.extract(seq, k))
+        with K.Engine(k, min_count=2, max_count=20, counter=counter) as e:
+            e.push_block(seq)
+            e.finish()
+            check_filtered(e.firsts(), e.records(), e.stats(), k, keys, counts, 2, 20)
 
 
 def test_root_files_hold_their_prefix():

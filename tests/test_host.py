@@ -113,6 +113,27 @@ def test_file_reader_plain_and_gzip(tmp_path):
         assert blocks == K.fastq_blocks(fq, 31)
 
 
+@pytest.mark.parametrize("case", ["small", "parts", "gzip", "k63"])
+def test_stage0_sampler_matches_oracle(tmp_path, case):
+    # the host Stage-0 sampler against the oracle's restatement of
+    # FastqPreReader::pre_read: overlapping 2 MiB reads below 6 MiB, three
+    # record-aligned parts above, a gzip stream read up to its compressed size
+    scale, k = {"small": (40.0, 31), "parts": (10.0, 31), "gzip": (10.0, 31), "k63": (7.0, 63)}[case]
+    fq = K.generate(1, scale=scale, seed=31)
+    path = tmp_path / "in.fq"
+    if case == "gzip":
+        data = gzip.compress(fq, compresslevel=6, mtime=0)
+        path = tmp_path / "in.fq.gz"
+        path.write_bytes(data)
+    else:
+        data = fq
+        path.write_bytes(fq)
+    blocks, est = K.stage0_sample_file(path, k)
+    want = C.pre_read_blocks(fq, k, len(data), case == "gzip")
+    assert blocks == want
+    assert est == len(data) * (4 if case == "gzip" else 1)
+
+
 @pytest.mark.parametrize("k", [17, 31, 32, 33, 63, 64, 65, 96, 100, 128])
 def test_pack_low_matches_oracle(k):
     rng = np.random.default_rng(k)

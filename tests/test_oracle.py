@@ -7,6 +7,7 @@ import ctypes
 import json
 import subprocess
 from collections import Counter
+from functools import lru_cache
 
 import numpy as np
 import pytest
@@ -334,6 +335,66 @@ def test_lpt_and_bloom_capacity():
     assert caps[0] == 1 << 22
     assert caps.min() >= 1 << 15
     assert all(int(x) & (int(x) - 1) == 0 for x in caps)
+
+
+@lru_cache(maxsize=1)
+def stage0_runs() -> dict:
+    return json.loads((C.GOLDEN / "stage0_runs.json").read_text())
+
+
+@pytest.mark.parametrize("name", sorted(stage0_runs()))
+def test_stage0_chain_matches_the_reference_tree(name):
+    # FastqPreReader sample -> prefix histogram -> clamp -> LPT -> Bloom sizing,
+    # against the lines the reference Tree printed for the same file
+    # (src/main.cpp:77-82, :175-186, :106)
+    import gzip
+
+    import kmertreecounter_amd as K
+    fx = stage0_runs()[name]
+    fq = K.generate(fx["workload"], scale=fx["scale"], seed=fx["seed"])
+    assert len(fq) == fx["fastq_bytes"]
+    if fx["gzip"]:
+        assert len(gzip.compress(fq, compresslevel=6, mtime=0)) == fx["file_size"]
+    blocks = C.pre_read_blocks(fq, fx["k"], fx["file_size"], fx["gzip"])
+    keys = C.extract_blocks(blocks, fx["k"])
+    ncls = C.classifiers_for_threads(fx["n_thread"])
+    assert ncls == fx["classifiers"]
+    est = fx["file_size"] * (4 if fx["gzip"] else 1)
+    counts, avg, loads, caps = C.stage0_plan(keys, ncls, est)
+    assert avg == fx["average"]
+    assert loads.tolist() == fx["loads"]
+    assert int(caps.max()) == fx["max_bloom_capacity"]
+
+
+def test_prefix_counts_against_bincount():
+    # orc_prefix_counts is the histogram of word 0's top byte: numpy bincount on
+    # brute-force-checked keys, and it sums to K
+    import kmertreecounter_amd as K
+    seq = K.generate(3, scale=400, seed=5, fmt="seq")
+    for k in (31, 63, 100):
+        keys = C.extract(seq, k)
+        counts = np.zeros(256, dtype=np.uint64)
+        O.orc_prefix_counts(keys.ctypes.data, keys.shape[0], keys.shape[1], counts.ctypes.data)
+        want = np.bincount((keys[:, 0] >> np.uint64(56)).astype(np.int64), minlength=256)
+        np.testing.assert_array_equal(counts, want.astype(np.uint64))
+        assert int(counts.sum()) == keys.shape[0] == int(O.orc_count_kmers(seq, len(seq), k))
+
+
+def test_pre_read_parts():
+    # the sampler's parts: a small file reads 2 MiB at a time from three
+    # overlapping offsets; a larger one reads three record-aligned parts, the
+    # third parsed on from the second part's line state
+    import kmertreecounter_amd as K
+    fq = K.generate(1, scale=10, seed=11)
+    whole = b"".join(C.pre_read_blocks(fq, 31, gz=False, block_size=1 << 30))
+    part = len(fq) // 3
+    # the first part's sequence lines are a prefix of the sample
+    head = b"".join(K.fastq_blocks(fq[:part], 31, 1 << 30))
+    assert whole.startswith(head[: head.rfind(b"\n")])
+    # gzip: one stream from the start, up to the compressed size in 2 MiB reads
+    gz = C.pre_read_blocks(fq, 31, file_size=3 << 20, gz=True, block_size=1 << 30)
+    want = b"".join(K.fastq_blocks(fq[: 4 << 20], 31, 1 << 30))
+    assert b"".join(gz) == want
 
 
 # ------------------------------------------------------------------ fixtures
