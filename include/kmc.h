@@ -222,6 +222,43 @@ typedef struct kc_tune_result {
 } kc_tune_result;
 KC_API int kc_autotune(kc_engine* h, kc_tune_result* out);
 
+/* kc_count_noalloc: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): one pass of the counting pipeline of
+ * kc_finish / kc_count_resident over a sequence already in device memory
+ * (`d_seq`, `seq_len` bytes, the same '\n'-separated sequence lines
+ * kc_push_block takes), written to the caller's device buffers, with no
+ * allocation at all: every work buffer is carved out of one static 64 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends.  Limits: seq_len <= kc_noalloc_max_bytes(k) (2 MiB at k <= 32, 1 MiB
+ * at k <= 64, 512 KiB above; larger inputs return KC_ERR_CAPACITY), the prefix
+ * partition over the 256 root buckets (partition PREFIX, sub_bits 0 or
+ * NONE), and cfg's k, device, min_count, max_count, counter, pass_keys and
+ * table_slots; the other fields are ignored.  Outputs: `d_firsts` receives
+ * res->firsts keys (N words each, the low.bin key set), `d_records` res->records
+ * ExportRecord<N> structs grouped by root: the records of root r are
+ * [root_off[0][r], root_off[0][r+1]) followed by [root_off[1][r],
+ * root_off[1][r+1]).  firsts_cap / records_cap are the keys / records those
+ * buffers hold (every position yields at most one key); too small returns
+ * KC_ERR_CAPACITY with the needed counts in *res.  `stream` is a hipStream_t
+ * (NULL: the default stream); the call returns after the pass has finished.
+ * Calls on one GPU share its array and keep the last call's contents; they
+ * are serialized by a per-device lock inside one process.  Errors: the codes
+ * of kc_status, with the text in kc_last_error(NULL) on the calling thread. */
+typedef struct kc_noalloc_result {
+    uint64_t total_kmers;        /* K of the input */
+    uint64_t firsts;             /* keys written to d_firsts */
+    uint64_t records;            /* records written to d_records (after the min/max filter) */
+    uint64_t records_unfiltered; /* distinct keys seen again (before the filter) */
+    uint64_t spilled_keys;       /* distinct keys whose count lived in the spill table */
+    uint64_t root_off[2][257];   /* root ranges of the records, see above */
+} kc_noalloc_result;
+KC_API int kc_count_noalloc(const kc_config* cfg, const char* d_seq, uint64_t seq_len, uint64_t* d_firsts,
+                            uint64_t firsts_cap, void* d_records, uint64_t records_cap, kc_noalloc_result* res,
+                            void* stream);
+/* Largest seq_len kc_count_noalloc takes for k (0 if k is out of range). */
+KC_API uint64_t kc_noalloc_max_bytes(uint32_t k);
+
 /* Free and total device memory of a device in bytes (hipMemGetInfo); for
  * choosing max_round_keys, and for tests that check nothing stays allocated.
  * No reference counterpart. */
@@ -230,7 +267,8 @@ KC_API int kc_device_memory(uint32_t device, uint64_t* free_bytes, uint64_t* tot
 /* Number of u64 words per key for k (1, 2 or 4), 0 if k is 0 or above 128. */
 KC_API uint32_t kc_words_for_k(uint32_t k);
 
-/* Human-readable text of the last error on this handle (never NULL). */
+/* Human-readable text of the last error on this handle (never NULL); with
+ * h == NULL, of the calling thread's last kc_count_noalloc. */
 KC_API const char* kc_last_error(kc_engine* h);
 
 /* Test hook, no reference counterpart: copies up to `cap` bytes of the

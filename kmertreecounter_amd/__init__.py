@@ -51,6 +51,12 @@ class Config(ctypes.Structure):
     ]
 
 
+class NoallocResult(ctypes.Structure):
+    _fields_ = [("total_kmers", ctypes.c_uint64), ("firsts", ctypes.c_uint64), ("records", ctypes.c_uint64),
+                ("records_unfiltered", ctypes.c_uint64), ("spilled_keys", ctypes.c_uint64),
+                ("root_off", (ctypes.c_uint64 * 257) * 2)]
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("sub_bits", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("heavy_keys", ctypes.c_uint32),
                 ("candidates", ctypes.c_uint32), ("best_ms", ctypes.c_double)]
@@ -117,7 +123,7 @@ EXPORTED = [
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
     "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident", "kmc_stage0_sample_file",
-    "kc_debug_fault", "kc_device_memory",
+    "kc_debug_fault", "kc_device_memory", "kc_count_noalloc", "kc_noalloc_max_bytes",
 ]
 
 _lib = None
@@ -157,6 +163,9 @@ def lib() -> ctypes.CDLL:
     L.kc_debug_fault.argtypes = [u32, ctypes.c_int64]
     L.kc_debug_fault.restype = ctypes.c_int64
     L.kc_device_memory.argtypes = [u32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.kc_count_noalloc.argtypes = [ctypes.POINTER(Config), vp, u64, vp, u64, vp, u64, ctypes.POINTER(NoallocResult), vp]
+    L.kc_noalloc_max_bytes.argtypes = [u32]
+    L.kc_noalloc_max_bytes.restype = u64
     L.kc_histogram.argtypes = [vp, u32, u32, u32, vp]
     L.kc_autotune.argtypes = [vp, vp]
     L.kmc_write_histogram.argtypes = [cp, vp, u32, u32]
@@ -381,6 +390,31 @@ def count_sequence(seq: bytes, k: int, **kw) -> tuple[np.ndarray, np.ndarray, St
         e.push_block(seq)
         e.finish()
         return e.firsts(), e.records(), e.stats()
+
+
+def noalloc_max_bytes(k: int) -> int:
+    """Largest input kc_count_noalloc takes at k."""
+    return int(lib().kc_noalloc_max_bytes(k))
+
+
+def count_noalloc(k: int, d_seq: int, seq_len: int, d_firsts: int, firsts_cap: int, d_records: int,
+                  records_cap: int, stream: int | None = None, min_count: int = 1, max_count: int = 0,
+                  counter: int = KC_COUNTER_BLOOM, pass_keys: int = 0, table_slots: int = 0,
+                  device: int = 0) -> NoallocResult:
+    """kc_count_noalloc on device buffers given as integer addresses; returns the
+    result struct (raises KmcError with the status on failure)."""
+    cfg = Config(k, KC_PARTITION_PREFIX, min_count, max_count, 0, 0, device, 1, pass_keys, counter, table_slots,
+                 0, 0, 0, 0, 0, 0)
+    res = NoallocResult()
+    L = lib()
+    rc = L.kc_count_noalloc(ctypes.byref(cfg), ctypes.c_void_p(d_seq), seq_len, ctypes.c_void_p(d_firsts), firsts_cap,
+                            ctypes.c_void_p(d_records), records_cap, ctypes.byref(res), ctypes.c_void_p(stream))
+    if rc != 0:
+        msg = L.kc_last_error(None)
+        err = KmcError(f"kc_count_noalloc failed with status {rc}: {msg.decode() if msg else ''}", rc)
+        err.result = res
+        raise err
+    return res
 
 
 def device_memory(device: int = 0) -> tuple[int, int]:

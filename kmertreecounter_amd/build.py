@@ -27,7 +27,8 @@ TREE = PKG / "kmc_tree"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kmc_kernels.hip", "kmc_fastq.hip", "kmc_engine.hip", "kmc_plan.hip", "kmc_learned.hip"]
+HIP_SOURCES = ["kmc_kernels.hip", "kmc_fastq.hip", "kmc_engine.hip", "kmc_plan.hip", "kmc_learned.hip",
+               "kmc_noalloc.hip"]
 CXX_SOURCES = ["kmc_host.cpp", "synth.cpp"]
 
 HIP_FLAGS = [

@@ -20,6 +20,7 @@
 #include "kmc_fault.h"
 #include "kmc_kernels.h"
 #include "kmc_learned.h"
+#include "kmc_pipeline.h"
 #include "kmc_plan.h"
 
 using namespace kmc;
@@ -39,6 +40,121 @@ bool fault_hit(uint32_t kind)
     const int64_t at = g_fault_at[kind].load(std::memory_order_relaxed);
     return at > 0 && c == at;
 }
+
+template <int NW>
+hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st, bool graph, hipEvent_t* ev)
+{
+#define PIPE_CHECK(expr)                     \
+    do {                                     \
+        const hipError_t e_ = (expr);        \
+        if (e_ != hipSuccess) return e_;     \
+    } while (0)
+    const uint32_t sbits = o.sbits, nf = o.nf;
+    unsigned long long* counts = b.small + kSmCounts;
+    unsigned long long* cursor = b.small + kSmCursor;
+    unsigned long long* coarse_off = b.small + kSmCoarseOff;
+    unsigned long long* tile_start = b.small + kSmTileStart;
+    uint32_t* d_nfirst = b.bucket_u32;
+    uint32_t* d_nrec = d_nfirst + nf;
+    uint32_t* d_ndist = d_nrec + nf;
+    // keys grouped by fine bucket (P3 input) and the per-bucket first-occurrence slices
+    uint64_t* src = sbits ? b.B : b.A;
+    uint64_t* slices = sbits ? b.A : b.B;
+    auto fill = [&](void* p, uint64_t bytes, uint8_t v) -> hipError_t {
+        return graph ? launch_fill(p, bytes, v, st) : hipMemsetAsync(p, v, bytes, st);
+    };
+    auto mark = [&](int e) -> hipError_t { return ev ? hipEventRecord(ev[e], st) : hipSuccess; };
+    if (b.padded > b.seq_len) PIPE_CHECK(fill(b.seq + b.seq_len, b.padded - b.seq_len, '\n'));
+    PIPE_CHECK(fill(b.small, sizeof(unsigned long long) * kSmTotal, 0));
+    PIPE_CHECK(fill(b.ctr, sizeof(unsigned long long) * kCtrCount, 0));
+    // a graph cannot skip the clear, so a captured run always clears the spill table
+    if (b.clear_spill || graph) {
+        PIPE_CHECK(fill(b.spill_keys, b.spill_cap * 8ULL * TableWords<NW>::value, 0xFF));
+        PIPE_CHECK(fill(b.spill_cnt, b.spill_cap * 4ULL, 0));
+    }
+    if (sbits) PIPE_CHECK(fill(b.fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
+
+    // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]
+    PIPE_CHECK(mark(0));
+    PIPE_CHECK(launch_extract<NW>(false, b.seq, b.padded, o.k, o.rt, counts, cursor, nullptr, st));
+    PIPE_CHECK(mark(1));
+    PIPE_CHECK(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW, sbits), st, o.rt.rlo, o.rt.rhi));
+    PIPE_CHECK(mark(2));
+    PIPE_CHECK(launch_extract<NW>(true, b.seq, b.padded, o.k, o.rt, counts, cursor, b.A, st));
+    PIPE_CHECK(mark(3));
+    const unsigned long long* fine_off = coarse_off;
+    if (sbits) {
+        const uint64_t max_tiles = b.padded / sub_tile(NW, sbits) + 256;
+        PIPE_CHECK(mark(4));
+        PIPE_CHECK(launch_sub<NW>(false, max_tiles, b.A, coarse_off, tile_start, o.rt, b.fcounts, nullptr, nullptr, st));
+        PIPE_CHECK(mark(5));
+        PIPE_CHECK(launch_scan_fine(b.fcounts, nf, b.fine_off, b.fcursor, b.small + kSmScanPart, st));
+        PIPE_CHECK(mark(6));
+        PIPE_CHECK(launch_sub<NW>(true, max_tiles, b.A, coarse_off, tile_start, o.rt, b.fcounts, b.fcursor, b.B, st));
+        PIPE_CHECK(mark(7));
+        fine_off = b.fine_off;
+    } else {
+        for (int e = 4; e < 8; ++e) PIPE_CHECK(mark(e));
+    }
+    CountArgs a{};
+    a.firsts = slices;
+    a.rec_tmp = b.R;
+    a.nfirst = d_nfirst;
+    a.nrec = d_nrec;
+    a.ndist = d_ndist;
+    a.ovf = b.ovf;
+    a.ovf_count = b.ctr + kCtrOvf;
+    a.ovf_cap = b.ovf_cap;
+    a.spill_keys = b.spill_keys;
+    a.spill_cnt = b.spill_cnt;
+    a.spill_cap = b.spill_cap;
+    a.spilled = b.ctr + kCtrSpilled;
+    a.spill_distinct = b.ctr + kCtrSpillDistinct;
+    a.error_flags = b.ctr + kCtrErr;
+    a.min_count = o.min_count;
+    a.max_count = o.max_count;
+    // the exact counter reads 0 as its own default (its table size)
+    a.pass_keys = o.pass_keys ? o.pass_keys : (o.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
+    a.extra_passes = b.ctr + kCtrExtraPasses;
+    a.table_slots = o.table_slots;
+    a.heavy_keys = o.heavy_keys;
+    a.heavy_chunk = o.heavy_chunk;
+    a.heavy_list = b.heavy;
+    a.heavy_cap = b.heavy_cap;
+    a.heavy_count = b.ctr + kCtrHeavyChunks;
+    a.heavy_next = b.ctr + kCtrHeavyNext;
+    a.heavy_buckets = b.ctr + kCtrHeavyBuckets;
+    PIPE_CHECK(mark(8));
+    PIPE_CHECK(launch_count<NW>(nf, o.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
+    PIPE_CHECK(launch_heavy<NW>(o.heavy_workgroups, src, a, st));
+    PIPE_CHECK(mark(9));
+    PIPE_CHECK(launch_spill_emit<NW>(a, st));
+    SegArgs sa{};
+    sa.nf = nf;
+    sa.sbits = sbits;
+    sa.fine_off = fine_off;
+    sa.nfirst = d_nfirst;
+    sa.nrec = d_nrec;
+    sa.ndist = d_ndist;
+    sa.foff = b.foff;
+    sa.roff = b.roff;
+    sa.root_ovf = b.small + kSmRootOvf;
+    sa.root_main = b.small + kSmRootMain;
+    sa.root_off = b.small + kSmRootOff;
+    sa.ovf_off = b.small + kSmOvfOff;
+    sa.ovf_cursor = b.small + kSmOvfCursor;
+    sa.ctr = b.ctr;
+    sa.rec_out_cap = b.rec_out_cap;
+    sa.firsts_out_cap = b.firsts_out_cap;
+    // final firsts into the P3 key input (free now), final records into the slices buffer
+    PIPE_CHECK(launch_finalize<NW>(sa, a, slices, pipe_firsts<NW>(b, sbits), b.R, pipe_records<NW>(b, sbits), st));
+    return hipSuccess;
+#undef PIPE_CHECK
+}
+
+template hipError_t enqueue_pipeline<1>(const PipeBufs&, const PipeOpts&, hipStream_t, bool, hipEvent_t*);
+template hipError_t enqueue_pipeline<2>(const PipeBufs&, const PipeOpts&, hipStream_t, bool, hipEvent_t*);
+template hipError_t enqueue_pipeline<4>(const PipeBufs&, const PipeOpts&, hipStream_t, bool, hipEvent_t*);
 
 }  // namespace kmc
 
@@ -209,18 +325,6 @@ struct kc_engine {
 
 namespace {
 
-uint64_t ext_tile(uint32_t nw)
-{
-    return (nw == 1 ? ExtractCfg<1>::kThreads : nw == 2 ? ExtractCfg<2>::kThreads : ExtractCfg<4>::kThreads) * 16ULL;
-}
-uint64_t sub_tile(uint32_t nw, uint32_t sbits)
-{
-    if (sbits > 8) return nw == 1 ? SubWideCfg<1>::kTile : nw == 2 ? SubWideCfg<2>::kTile : SubWideCfg<4>::kTile;
-    return nw == 1 ? SubCfg<1>::kTile : nw == 2 ? SubCfg<2>::kTile : SubCfg<4>::kTile;
-}
-uint64_t rec_bytes(uint32_t nw) { return 8ULL * nw + 8; }
-// words of a key in the open-addressing tables (k <= 128 keys take five 63-bit words)
-uint64_t table_words(uint32_t nw) { return nw == 4 ? (uint64_t)TableWords<4>::value : nw; }
 
 uint64_t pow2_at_least(uint64_t v)
 {
@@ -382,11 +486,6 @@ void free_spill(kc_engine* h)
     h->spill_dirty = true;
 }
 
-// small arrays in d_small (unsigned long long units)
-constexpr uint64_t kSmCounts = 0, kSmCursor = 8 * 256, kSmCoarseOff = 16 * 256, kSmTileStart = kSmCoarseOff + 257,
-                   kSmRootOvf = kSmTileStart + 257, kSmRootMain = kSmRootOvf + 256, kSmRootOff = kSmRootMain + 256,
-                   kSmOvfOff = kSmRootOff + 257, kSmOvfCursor = kSmOvfOff + 257, kSmScanPart = kSmOvfCursor + 256,
-                   kSmTotal = kSmScanPart + kScanGroups;
 
 // Buckets above heavy_keys keys are counted in chunks by the whole GPU.
 uint64_t heavy_keys(const kc_engine* h) { return h->cfg.heavy_keys ? h->cfg.heavy_keys : (1ULL << 20); }
@@ -689,110 +788,50 @@ template <int NW>
 int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded)
 {
     hipStream_t st = h->st;
-    unsigned long long* counts = h->d_small + kSmCounts;
-    unsigned long long* cursor = h->d_small + kSmCursor;
     unsigned long long* coarse_off = h->d_small + kSmCoarseOff;
-    unsigned long long* tile_start = h->d_small + kSmTileStart;
-    uint32_t* d_nfirst = h->d_bucket_u32;
-    uint32_t* d_nrec = d_nfirst + nf;
-    uint32_t* d_ndist = d_nrec + nf;
-
-    // keys grouped by fine bucket (P3 input) and the per-bucket first-occurrence slices
-    uint64_t* src = sbits ? h->d_B : h->d_A;
-    uint64_t* slices = sbits ? h->d_A : h->d_B;
     for (int attempt = 0; attempt < 6; ++attempt) {
-        // One pass of the pipeline on the stream.  With `graph` it is being
-        // captured: clears are fill kernels (no memset nodes) and no events.
+        PipeBufs pb{};
+        pb.seq = h->d_seq;
+        pb.seq_len = h->seq_len;
+        pb.padded = padded;
+        pb.A = h->d_A;
+        pb.B = h->d_B;
+        pb.R = h->d_R;
+        pb.key_cap = h->key_cap;
+        pb.small = h->d_small;
+        pb.ctr = h->d_ctr;
+        pb.fcounts = h->d_fcounts;
+        pb.fcursor = h->d_fcursor;
+        pb.fine_off = h->d_fine_off;
+        pb.foff = h->d_foff;
+        pb.roff = h->d_roff;
+        pb.bucket_u32 = h->d_bucket_u32;
+        pb.ovf = h->d_ovf;
+        pb.ovf_cap = h->ovf_cap;
+        pb.spill_keys = h->d_spill_keys;
+        pb.spill_cnt = h->d_spill_cnt;
+        pb.spill_cap = h->spill_cap;
+        pb.clear_spill = h->spill_dirty;
+        pb.heavy = h->d_heavy;
+        pb.heavy_cap = h->heavy_cap;
+        pb.rec_out_cap = (h->key_cap * 8ULL * NW) / rec_bytes(NW);
+        pb.firsts_out_cap = h->key_cap;
+        PipeOpts po{};
+        po.k = h->k;
+        po.sbits = sbits;
+        po.nf = nf;
+        po.rt = rt;
+        po.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
+        po.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
+        po.pass_keys = h->cfg.pass_keys;
+        po.counter = h->cfg.counter;
+        po.table_slots = h->cfg.table_slots;
+        po.heavy_keys = heavy_keys(h);
+        po.heavy_chunk = heavy_chunk_keys(h);
+        po.heavy_workgroups = h->cu_count;
         auto enqueue = [&](bool graph) -> int {
-            auto fill = [&](void* p, uint64_t bytes, uint8_t v) -> hipError_t {
-                return graph ? launch_fill(p, bytes, v, st) : hipMemsetAsync(p, v, bytes, st);
-            };
-            auto mark = [&](int e) -> hipError_t { return graph ? hipSuccess : hipEventRecord(h->ev[e], st); };
-            if (padded > h->seq_len) HIPX(fill(h->d_seq + h->seq_len, padded - h->seq_len, '\n'));
-            HIPX(fill(h->d_small, sizeof(unsigned long long) * kSmTotal, 0));
-            HIPX(fill(h->d_ctr, sizeof(unsigned long long) * kCtrCount, 0));
-            // a graph cannot skip the clear, so a captured run always clears the spill table
-            if (h->spill_dirty || graph) {
-                HIPX(fill(h->d_spill_keys, h->spill_cap * 8ULL * TableWords<NW>::value, 0xFF));
-                HIPX(fill(h->d_spill_cnt, h->spill_cap * 4ULL, 0));
-            }
-            if (sbits) HIPX(fill(h->d_fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
-
-            // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]
-            HIPX(mark(0));
-            HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, cursor, nullptr, st));
-            HIPX(mark(1));
-            HIPX(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW, sbits), st, rt.rlo, rt.rhi));
-            HIPX(mark(2));
-            HIPX(launch_extract<NW>(true, h->d_seq, padded, h->k, rt, counts, cursor, h->d_A, st));
-            HIPX(mark(3));
-            const unsigned long long* fine_off = coarse_off;
-            if (sbits) {
-                const uint64_t max_tiles = padded / sub_tile(NW, sbits) + 256;
-                HIPX(mark(4));
-                HIPX(launch_sub<NW>(false, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, nullptr, nullptr, st));
-                HIPX(mark(5));
-                HIPX(launch_scan_fine(h->d_fcounts, nf, h->d_fine_off, h->d_fcursor, h->d_small + kSmScanPart, st));
-                HIPX(mark(6));
-                HIPX(launch_sub<NW>(true, max_tiles, h->d_A, coarse_off, tile_start, rt, h->d_fcounts, h->d_fcursor, h->d_B, st));
-                HIPX(mark(7));
-                fine_off = h->d_fine_off;
-            } else {
-                for (int e = 4; e < 8; ++e) HIPX(mark(e));
-            }
-            CountArgs a{};
-            a.firsts = slices;
-            a.rec_tmp = h->d_R;
-            a.nfirst = d_nfirst;
-            a.nrec = d_nrec;
-            a.ndist = d_ndist;
-            a.ovf = h->d_ovf;
-            a.ovf_count = h->d_ctr + kCtrOvf;
-            a.ovf_cap = h->ovf_cap;
-            a.spill_keys = h->d_spill_keys;
-            a.spill_cnt = h->d_spill_cnt;
-            a.spill_cap = h->spill_cap;
-            a.spilled = h->d_ctr + kCtrSpilled;
-            a.spill_distinct = h->d_ctr + kCtrSpillDistinct;
-            a.error_flags = h->d_ctr + kCtrErr;
-            a.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
-            a.max_count = h->cfg.max_count ? h->cfg.max_count : 0xFFFFFFFFu;
-            // the exact counter reads 0 as its own default (its table size)
-            a.pass_keys = h->cfg.pass_keys ? h->cfg.pass_keys
-                                           : (h->cfg.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
-            a.extra_passes = h->d_ctr + kCtrExtraPasses;
-            a.table_slots = h->cfg.table_slots;
-            a.heavy_keys = heavy_keys(h);
-            a.heavy_chunk = heavy_chunk_keys(h);
-            a.heavy_list = h->d_heavy;
-            a.heavy_cap = h->heavy_cap;
-            a.heavy_count = h->d_ctr + kCtrHeavyChunks;
-            a.heavy_next = h->d_ctr + kCtrHeavyNext;
-            a.heavy_buckets = h->d_ctr + kCtrHeavyBuckets;
-            HIPX(mark(8));
-            HIPX(launch_count<NW>(nf, h->cfg.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
-            HIPX(launch_heavy<NW>(h->cu_count, src, a, st));
-            HIPX(mark(9));
-            HIPX(launch_spill_emit<NW>(a, st));
-            SegArgs sa{};
-            sa.nf = nf;
-            sa.sbits = sbits;
-            sa.fine_off = fine_off;
-            sa.nfirst = d_nfirst;
-            sa.nrec = d_nrec;
-            sa.ndist = d_ndist;
-            sa.foff = h->d_foff;
-            sa.roff = h->d_roff;
-            sa.root_ovf = h->d_small + kSmRootOvf;
-            sa.root_main = h->d_small + kSmRootMain;
-            sa.root_off = h->d_small + kSmRootOff;
-            sa.ovf_off = h->d_small + kSmOvfOff;
-            sa.ovf_cursor = h->d_small + kSmOvfCursor;
-            sa.ctr = h->d_ctr;
-            sa.rec_out_cap = (h->key_cap * 8ULL * NW) / rec_bytes(NW);
-            // final firsts into the P3 key input (free now), final records into the slices buffer
-            HIPX(launch_finalize<NW>(sa, a, slices, src, h->d_R, reinterpret_cast<uint8_t*>(slices), st));
-            return KC_OK;
+            const hipError_t e = enqueue_pipeline<NW>(pb, po, st, graph, graph ? nullptr : h->ev);
+            return e == hipSuccess ? KC_OK : h->hip_fail(e, "enqueue of the counting pipeline");
         };
         const bool use_graph = h->cfg.graph && rt.rlo == 0 && rt.rhi == 256;
         if (use_graph) {
@@ -852,6 +891,8 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
             h->ovf_cap = ncap;
             continue;
         }
+        if (ctr[kCtrErr] & kErrFirstOut)
+            return h->fail(KC_ERR_CAPACITY, "first occurrences exceed the key buffer");
         if (ctr[kCtrErr] & kErrRecOut)
             return h->fail(KC_ERR_CAPACITY, "root records exceed the record buffer (too many overflow records)");
         h->K = coff256;
@@ -863,8 +904,8 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         h->extra_passes = ctr[kCtrExtraPasses];
         h->heavy_buckets = ctr[kCtrHeavyBuckets];
         h->nf_last = nf;
-        h->res_firsts = src;
-        h->res_recs = reinterpret_cast<const uint8_t*>(slices);
+        h->res_firsts = pipe_firsts<NW>(pb, sbits);
+        h->res_recs = pipe_records<NW>(pb, sbits);
         h->root_off.assign(2 * 257, 0);
         HIPX(dcopy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
         // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total
@@ -939,7 +980,8 @@ uint32_t kc_words_for_k(uint32_t k)
 
 const char* kc_last_error(kc_engine* h)
 {
-    return (h && !h->err.empty()) ? h->err.c_str() : "";
+    if (!h) return noalloc_last_error();
+    return h->err.c_str();
 }
 
 int kc_device_memory(uint32_t device, uint64_t* free_bytes, uint64_t* total_bytes)

@@ -14,6 +14,7 @@ constexpr unsigned long long kErrSpillFull = 1ULL;
 constexpr unsigned long long kErrRecFull = 2ULL;  // overflow record list full
 constexpr unsigned long long kErrRecOut = 4ULL;   // records exceed the final record buffer
 constexpr unsigned long long kErrHistFull = 8ULL; // histogram record table: a probe ran out
+constexpr unsigned long long kErrFirstOut = 16ULL; // first occurrences exceed the final firsts buffer
 
 // P0/P1: one 16-byte word of sequence per thread, kThreads*16 positions per tile.
 template <int NW> struct ExtractCfg;
@@ -128,6 +129,7 @@ struct SegArgs {
     unsigned long long* ovf_cursor;     // [256]
     unsigned long long* ctr;
     uint64_t rec_out_cap;               // records the final record buffer holds
+    uint64_t firsts_out_cap;            // keys the final firsts buffer holds
 };
 
 template <int NW>

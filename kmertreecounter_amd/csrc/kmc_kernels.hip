@@ -1503,6 +1503,7 @@ k_seg_scan(SegArgs a)
         a.ctr[kCtrFirst] = tf;
         a.ctr[kCtrRec] = tr + all;
         if (tr + all > a.rec_out_cap) atomicOr(&a.ctr[kCtrErr], kErrRecOut);
+        if (tf > a.firsts_out_cap) atomicOr(&a.ctr[kCtrErr], kErrFirstOut);
     }
 }
 
@@ -1514,6 +1515,7 @@ __global__ void __launch_bounds__(256)
 k_compact_firsts(SegArgs a, const uint64_t* __restrict__ firsts_in, uint64_t* __restrict__ firsts_out)
 {
     const uint32_t f = blockIdx.x;
+    if (a.ctr[kCtrErr] & kErrFirstOut) return;
     const unsigned long long lo = a.fine_off[f];
     const uint32_t nfst = a.nfirst[f];
     const unsigned long long fdst = a.foff[f];
