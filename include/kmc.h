@@ -156,6 +156,12 @@ typedef struct kc_stats {
     uint64_t heavy_buckets;             /* buckets counted in chunks by the whole GPU */
     uint64_t rounds;                    /* bucket-range rounds of the last run (1 = one pass) */
     uint64_t learned_split_buckets;     /* learned buckets the network left empty, refilled from a merged neighbour */
+    double bucket4096_max_over_mean;    /* keys of the largest of 4096 bucket groups over the mean, on the
+                                           counted input: the 4096 learned buckets, or the 4096 12-bit key
+                                           prefixes with the prefix partition (0 when the fan-out is below
+                                           4 bits) */
+    uint64_t hist_tail_keys;            /* last kc_histogram: keys whose count is above max_freq (not in it) */
+    uint64_t hist_tail_kmers;           /*   and the k-mers they hold (the sum of their counts) */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;

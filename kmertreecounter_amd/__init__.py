@@ -83,7 +83,8 @@ class Stats(ctypes.Structure):
         ("learned_sample_distinct_max_over_mean", ctypes.c_double),
         ("prefix_sample_distinct_max_over_mean", ctypes.c_double), ("extra_passes", ctypes.c_uint64),
         ("heavy_buckets", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
-        ("learned_split_buckets", ctypes.c_uint64),
+        ("learned_split_buckets", ctypes.c_uint64), ("bucket4096_max_over_mean", ctypes.c_double),
+        ("hist_tail_keys", ctypes.c_uint64), ("hist_tail_kmers", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

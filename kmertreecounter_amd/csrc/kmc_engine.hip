@@ -235,6 +235,8 @@ struct kc_engine {
     bool have_results = false;
     uint64_t K = 0, n_first = 0, n_rec = 0, n_distinct_rep = 0, n_spilled = 0, max_bucket = 0, extra_passes = 0;
     uint64_t heavy_buckets = 0;
+    uint64_t hist_tail_keys = 0, hist_tail_kmers = 0;  // last kc_histogram: keys above max_freq, their k-mers
+    std::vector<uint64_t> occ4096;                     // keys per group of fine buckets (4096 groups), last run
     // bucket-range rounds: each round's outputs in host memory (empty when the
     // results of a single round stay on the device)
     // Host memory kept across runs: pinned (hipHostMalloc), so device results
@@ -674,6 +676,7 @@ int run_pipeline_t(kc_engine* h)
     if (want < padded && h->key_cap < want && h->key_cap * 10 >= want * 9) want = h->key_cap;
     int rc = ensure_work(h, want, nf);
     if (rc) return rc;
+    h->occ4096.clear();
     h->rounds_used = 0;
     h->n_rounds = 1;
 
@@ -904,6 +907,12 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         h->extra_passes = ctr[kCtrExtraPasses];
         h->heavy_buckets = ctr[kCtrHeavyBuckets];
         h->nf_last = nf;
+        if (sbits >= 4) {  // keys per group of 2^(sbits-4) fine buckets: learned buckets or 12-bit prefixes
+            std::vector<unsigned long long> fo(nf + 1);
+            HIPX(dcopy(fo.data(), h->d_fine_off, sizeof(unsigned long long) * (nf + 1), hipMemcpyDeviceToHost));
+            if (h->occ4096.size() != 4096) h->occ4096.assign(4096, 0);
+            for (uint32_t f = 0; f < nf; ++f) h->occ4096[f >> (sbits - 4)] += fo[f + 1] - fo[f];
+        }
         h->res_firsts = pipe_firsts<NW>(pb, sbits);
         h->res_recs = pipe_records<NW>(pb, sbits);
         h->root_off.assign(2 * 257, 0);
@@ -1339,7 +1348,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
         hipFree(d_hist); hipFree(d_err); free_table();
         d_hist = nullptr; d_err = nullptr;
     };
-    if (dmalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || dmalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) {
+    if (dmalloc(&d_hist, sizeof(long long) * nb) != hipSuccess || dmalloc(&d_err, sizeof(unsigned long long) * 3) != hipSuccess) {
         release();
         return h->fail(KC_ERR_NOMEM, "histogram allocation failed");
     }
@@ -1362,7 +1371,7 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
             }
         }
         hipError_t e = hipMemsetAsync(d_hist, 0, sizeof(long long) * nb, h->st);
-        if (e == hipSuccess) e = hipMemsetAsync(d_err, 0, sizeof(unsigned long long), h->st);
+        if (e == hipSuccess) e = hipMemsetAsync(d_err, 0, sizeof(unsigned long long) * 3, h->st);
         if (e == hipSuccess && precise) e = hipMemsetAsync(d_keys, 0xFF, cap * 8ULL * table_words(h->nw), h->st);
         HistArgs a{};
         a.hist = d_hist;
@@ -1373,17 +1382,19 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
         a.vals = d_vals;
         a.cap = cap;
         a.error_flags = d_err;
+        a.tail = d_err + 1;
         if (e == hipSuccess)
             e = h->nw == 1   ? launch_histogram<1>(recs, nrec, firsts, nfirst, a, h->st)
                 : h->nw == 2 ? launch_histogram<2>(recs, nrec, firsts, nfirst, a, h->st)
                              : launch_histogram<4>(recs, nrec, firsts, nfirst, a, h->st);
-        unsigned long long err = 0;
-        if (e == hipSuccess) e = dcopy_async(&err, d_err, sizeof(err), hipMemcpyDeviceToHost, h->st);
+        unsigned long long errt[3] = {0, 0, 0};
+        if (e == hipSuccess) e = dcopy_async(errt, d_err, sizeof(errt), hipMemcpyDeviceToHost, h->st);
         if (e == hipSuccess) e = hipStreamSynchronize(h->st);
         if (e != hipSuccess) {
             release();
             return h->fail(KC_ERR_HIP, std::string("histogram: ") + hipGetErrorString(e));
         }
+        const unsigned long long err = errt[0];
         if (err & kErrHistFull) {
             cap *= 2;
             continue;
@@ -1392,6 +1403,8 @@ int histogram_pass(kc_engine* h, bool precise, uint32_t min_freq, uint32_t max_f
         release();
         if (e != hipSuccess) return h->fail(KC_ERR_HIP, std::string("histogram: ") + hipGetErrorString(e));
         for (uint64_t b = 0; b < nb; ++b) hist[b] += part[b];
+        h->hist_tail_keys += errt[1];
+        h->hist_tail_kmers += errt[2];
         return KC_OK;
     }
     release();
@@ -1407,6 +1420,8 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
     HIPX(hipSetDevice(h->device));
     const bool precise = mode == KC_HISTOGRAM_PRECISE;
     std::fill(hist, hist + ((uint64_t)max_freq - min_freq + 1), (int64_t)0);
+    h->hist_tail_keys = 0;
+    h->hist_tail_kmers = 0;
     const uint64_t buf_bytes = h->key_cap * 8ULL * h->nw;
     if (!h->rounds_used)  // results in A and B; the record slices (R) are free
         return histogram_pass(h, precise, min_freq, max_freq, h->res_recs, h->n_rec, h->res_firsts, h->n_first, h->d_R,
@@ -1441,6 +1456,12 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->extra_passes = h->extra_passes;
     s->heavy_buckets = h->heavy_buckets;
     s->rounds = h->n_rounds;
+    s->hist_tail_keys = h->hist_tail_keys;
+    s->hist_tail_kmers = h->hist_tail_kmers;
+    if (h->occ4096.size() == 4096 && h->K) {
+        const uint64_t mx = *std::max_element(h->occ4096.begin(), h->occ4096.end());
+        s->bucket4096_max_over_mean = (double)mx / ((double)h->K / 4096.0);
+    }
     s->mean_bucket_keys = h->nf_last ? (double)h->K / h->nf_last : 0.0;
     s->ms_extract_count = h->phase_ms[0];
     s->ms_extract_scatter = h->phase_ms[1];

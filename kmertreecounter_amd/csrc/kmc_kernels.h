@@ -176,6 +176,7 @@ struct HistArgs {
     uint32_t* vals;                     // [cap] record counts
     uint64_t cap;                       // power of two, at least twice the records
     unsigned long long* error_flags;    // kErrHistFull (8) when a probe runs out
+    unsigned long long* tail;           // [2]: keys whose merged count is above maxf, and their k-mers
 };
 template <int NW>
 hipError_t launch_histogram(const uint8_t* recs, uint64_t nrec, const uint64_t* firsts, uint64_t nfirst, const HistArgs& a,

@@ -1790,6 +1790,10 @@ k_hist_records(const uint8_t* __restrict__ recs, uint64_t nrec, HistArgs a)
         const uint32_t c = (uint32_t)r[NW];
         if (a.precise && (uint64_t)c + 1 < a.minf) continue;
         hist_add(s_h, a.hist, a.minf, a.maxf, c, 1);
+        if (c > a.maxf) {  // beyond the histogram: kept in the tail totals
+            atomicAdd(&a.tail[0], 1ULL);
+            atomicAdd(&a.tail[1], (unsigned long long)c);
+        }
         if (!a.precise) continue;
         Key<NW> key;
 #pragma unroll
@@ -1834,6 +1838,14 @@ k_hist_firsts(const uint64_t* __restrict__ firsts, uint64_t nfirst, HistArgs a)
             const uint32_t c = a.vals[s];
             hist_add(s_h, a.hist, a.minf, a.maxf, c, -1);
             hist_add(s_h, a.hist, a.minf, a.maxf, (uint64_t)c + 1, 1);
+            // a key of merged count c + 1 above maxf is in the tail (moved there
+            // from bin maxf, or already there at c)
+            if (c > a.maxf) {
+                atomicAdd(&a.tail[1], 1ULL);
+            } else if (c == a.maxf) {
+                atomicAdd(&a.tail[0], 1ULL);
+                atomicAdd(&a.tail[1], (unsigned long long)c + 1);
+            }
         } else {
             hist_add(s_h, a.hist, a.minf, a.maxf, 1, 1);
         }

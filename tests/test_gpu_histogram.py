@@ -88,3 +88,35 @@ def test_tree_run_writes_the_histogram_file(tmp_path):
     got = np.array([int(x.split("\t")[1]) for x in lines])
     np.testing.assert_array_equal(got, C.histogram(counts, 1, 300))
     assert not (tmp_path / "tmp" / "low.bin").exists()  # histogram only: no files
+
+
+@pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT], ids=["bloom", "exact"])
+@pytest.mark.parametrize("rounds", [False, True], ids=["one_round", "rounds"])
+def test_histogram_tail_totals(counter, rounds):
+    # keys whose merged count is above max_freq are left out of the histogram
+    # (as histogram_tool drops them) and reported as tail totals, so that
+    # sum f * hist[f] + tail k-mers == K and sum hist[f] + tail keys == D
+    seq = microsatellite_reads(100_000, 9)
+    keys, counts = C.count_table(C.extract(seq, 31))
+    kw = dict(max_round_keys=5_000_000) if rounds else {}
+    with K.Engine(31, counter=counter, **kw) as e:
+        e.push_block(seq)
+        e.finish()
+        if rounds:
+            assert e.stats().rounds >= 2
+        for maxf in (1, 30, 1000):
+            h = e.histogram(1, maxf)
+            s = e.stats()
+            over = counts > maxf
+            assert s.hist_tail_keys == int(over.sum())
+            assert s.hist_tail_kmers == int(counts[over].sum(dtype=np.uint64))
+            f = np.arange(1, maxf + 1, dtype=np.int64)
+            assert int((f * h).sum()) + s.hist_tail_kmers == int(counts.sum(dtype=np.uint64)) == s.total_kmers
+            assert int(h.sum()) + s.hist_tail_keys == keys.shape[0]
+        # approximate mode: the records only, their counts above max_freq in the tail
+        recs = e.records()
+        h = e.histogram(1, 30, precise=False)
+        s = e.stats()
+        over = recs["count"] > 30
+        assert s.hist_tail_keys == int(over.sum())
+        assert s.hist_tail_kmers == int(recs["count"][over].sum(dtype=np.uint64))

@@ -176,3 +176,15 @@ def test_tree_cli_rejects_bad_arguments(tmp_path):
     for args in bad:
         r = subprocess.run([str(exe), *args], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode != 0, args
+
+
+def test_bench_closed_form_kmers_matches_oracle():
+    # bench.py's full-size check counts K with numpy in chunks; it must give the
+    # oracle's K, also when runs cross chunk boundaries
+    import bench
+    for w, sc in ((2, 400), (3, 800), (4, 2000)):
+        seq = K.generate(w, scale=sc, seed=3, fmt="seq")
+        for k in (1, 31, 63):
+            want = int(C.oracle().orc_count_kmers(seq, len(seq), k))
+            assert bench.closed_form_kmers(seq, k, chunk=1 << 15) == want
+            assert bench.closed_form_kmers(seq, k) == want
