@@ -104,7 +104,12 @@ typedef struct kc_config {
     uint32_t heavy_keys;      /* a bucket of more keys than this (0 = 1Mi) is counted in chunks by the
                                  whole GPU instead of one workgroup; its keys are written as root
                                  records with their whole count (no low.bin entry) */
-    uint32_t reserved;
+    uint32_t learned_min_skew_pct; /* KC_PARTITION_LEARNED: the network is trained only when the largest
+                                 of the Stage-0 sample's 4096 12-bit key prefixes holds more than this
+                                 percentage of their mean distinct keys (0 = 250).  Canonical keys of a
+                                 uniform genome have density 2(1 - x) over the key range, so their
+                                 prefixes reach about 200% with no skew of the input; at or below the
+                                 threshold the run keeps the prefix partition (kc_stats.learned_fallback) */
     uint64_t max_round_keys;  /* keys one pass over the device holds (0 = as many as the free device
                                  memory allows).  An input with more positions is counted in
                                  bucket-range rounds: contiguous root-bucket ranges, one pass each,
@@ -162,6 +167,9 @@ typedef struct kc_stats {
                                            4 bits) */
     uint64_t hist_tail_keys;            /* last kc_histogram: keys whose count is above max_freq (not in it) */
     uint64_t hist_tail_kmers;           /*   and the k-mers they hold (the sum of their counts) */
+    uint64_t learned_fallback;          /* kc_plan with KC_PARTITION_LEARNED kept the prefix partition: 1 the
+                                           sample's prefixes were balanced (learned_min_skew_pct), 2 the sample
+                                           had fewer distinct keys than buckets */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;

@@ -46,7 +46,7 @@ class Config(ctypes.Structure):
         ("expected_bytes", ctypes.c_uint64), ("sub_bits", ctypes.c_uint32),
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
-        ("heavy_keys", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
+        ("heavy_keys", ctypes.c_uint32), ("learned_min_skew_pct", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
         ("graph", ctypes.c_uint32), ("reserved2", ctypes.c_uint32), ("est_file_size", ctypes.c_uint64),
     ]
 
@@ -85,6 +85,7 @@ class Stats(ctypes.Structure):
         ("heavy_buckets", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
         ("learned_split_buckets", ctypes.c_uint64), ("bucket4096_max_over_mean", ctypes.c_double),
         ("hist_tail_keys", ctypes.c_uint64), ("hist_tail_kmers", ctypes.c_uint64),
+        ("learned_fallback", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -271,12 +272,14 @@ class Engine:
     def __init__(self, k: int, partition: int = KC_PARTITION_PREFIX, min_count: int = 1, max_count: int = 0,
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
                  pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0,
-                 max_round_keys: int = 0, graph: bool = False, est_file_size: int = 0):
+                 max_round_keys: int = 0, graph: bool = False, est_file_size: int = 0,
+                 learned_min_skew_pct: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
-                     counter, table_slots, heavy_keys, 0, max_round_keys, 1 if graph else 0, 0, est_file_size)
+                     counter, table_slots, heavy_keys, learned_min_skew_pct, max_round_keys, 1 if graph else 0, 0,
+                     est_file_size)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

@@ -1119,7 +1119,8 @@ int kc_plan(kc_engine* h)
     int rc = stage0_plan(h->k, h->cfg, h->sample, h->st, h->plan, h->err);
     if (rc) return rc;
     if (h->cfg.partition == KC_PARTITION_LEARNED) {
-        rc = learned_plan(h->k, h->sample, h->st, h->plan, h->err);
+        const double min_skew = (h->cfg.learned_min_skew_pct ? h->cfg.learned_min_skew_pct : 250u) / 100.0;
+        rc = learned_plan(h->k, h->sample, min_skew, h->st, h->plan, h->err);
         if (rc) return rc;
         h->bounds_sbits = ~0u;  // routing bounds are rebuilt for the run's fan-out
     }
@@ -1473,6 +1474,7 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->ms_h2d = h->h2d_ms;
     if (h->planned) {
         s->learned_train_keys = h->plan.train_keys;
+        s->learned_fallback = h->plan.fallback;
         s->learned_split_buckets = h->plan.split_buckets;
         s->learned_sample_distinct_max_over_mean = h->plan.sample_distinct_learned;
         s->prefix_sample_distinct_max_over_mean = h->plan.sample_distinct_prefix;

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -80,8 +81,9 @@ k_sum_slabs(const float* in, uint32_t nslab, size_t stride, uint32_t n, float* o
 
 __device__ __forceinline__ float xin(uint64_t key, int j) { return ((key >> (63 - j)) & 1ULL) ? 1.0f : -1.0f; }
 
-// Samples per workgroup in the hidden-layer kernels (four per wave group).
-constexpr uint32_t kSpb = 16;
+// Samples per workgroup in the hidden-layer kernels (two per wave group: 256
+// workgroups for a batch of 2048).
+constexpr uint32_t kSpb = 8;
 
 // Hidden layers for kSpb samples per workgroup: thread (unit i = tid % 64,
 // sample group tid / 64) keeps row i of W1 and then of W2 in registers; a1
@@ -136,14 +138,36 @@ k_mlp_hidden(const uint64_t* __restrict__ keys, uint32_t nb, const float* __rest
     }
 }
 
+// Reductions over a 256-thread workgroup: within each wave by shuffles, then
+// the four wave results through LDS (s4 holds four floats; every thread gets
+// the result).
+__device__ __forceinline__ float block256_max(float x, float* s4)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+    __syncthreads();  // s4 may still be read from the previous reduction
+    if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return fmaxf(fmaxf(s4[0], s4[1]), fmaxf(s4[2], s4[3]));
+}
+__device__ __forceinline__ float block256_sum(float x, float* s4)
+{
+#pragma unroll
+    for (int o = 32; o; o >>= 1) x += __shfl_xor(x, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return (s4[0] + s4[1]) + (s4[2] + s4[3]);
+}
+
 // Row softmax + cross entropy: one workgroup per sample, the row held in
 // registers.  dz = (p - onehot)/nb in bf16 (row layout); loss summed into *loss.
 __global__ void __launch_bounds__(256)
 k_softmax_ce(const float* __restrict__ logits, const float* __restrict__ b3, const uint32_t* __restrict__ labels,
-             uint32_t nb, __bf16* __restrict__ dz, float* __restrict__ loss)
+             uint32_t nb, __bf16* __restrict__ dz, float* __restrict__ row_loss)
 {
     constexpr int kPer = kMlpC / 256;
-    __shared__ float s_red[256];
+    __shared__ float s4[4];
     const uint32_t row = blockIdx.x;
     const float* z = logits + (size_t)row * kMlpC;
     float v[kPer];
@@ -154,27 +178,14 @@ k_softmax_ce(const float* __restrict__ logits, const float* __restrict__ b3, con
         v[r] = z[c] + b3[c];
         mx = fmaxf(mx, v[r]);
     }
-    s_red[threadIdx.x] = mx;
-    __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s_red[threadIdx.x] = fmaxf(s_red[threadIdx.x], s_red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    mx = s_red[0];
-    __syncthreads();
+    mx = block256_max(mx, s4);
     float se = 0.f;
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         v[r] = __expf(v[r] - mx);
         se += v[r];
     }
-    s_red[threadIdx.x] = se;
-    __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
-        __syncthreads();
-    }
-    se = s_red[0];
+    se = block256_sum(se, s4);
     const uint32_t y = labels[row];
     const float inv = 1.0f / se, invb = 1.0f / (float)nb;
 #pragma unroll
@@ -182,7 +193,9 @@ k_softmax_ce(const float* __restrict__ logits, const float* __restrict__ b3, con
         const uint32_t c = threadIdx.x + 256 * r;
         dz[(size_t)row * kMlpC + c] = to_bf16((v[r] * inv - (c == y ? 1.0f : 0.0f)) * invb);
     }
-    if (threadIdx.x == 0) atomicAdd(loss, (__logf(se) + mx - (z[y] + b3[y])) * invb);
+    // one loss term per row (same-address atomics from every row serialised
+    // in the L2 and took most of this kernel's time); k_step_end sums them
+    if (threadIdx.x == 0) row_loss[row] = (__logf(se) + mx - (z[y] + b3[y])) * invb;
 }
 
 // 64x64 tile transpose of a bf16 matrix: out[c][r] = in[r][c]
@@ -206,17 +219,12 @@ k_transpose_bf16(const __bf16* __restrict__ in, __bf16* __restrict__ out, uint32
 __global__ void __launch_bounds__(256)
 k_row_sum_bf16(const __bf16* __restrict__ m, uint32_t cols, float* __restrict__ out)
 {
-    __shared__ float s_red[256];
+    __shared__ float s4[4];
     const __bf16* row = m + (size_t)blockIdx.x * cols;
     float acc = 0.f;
     for (uint32_t i = threadIdx.x; i < cols; i += 256) acc += from_bf16(row[i]);
-    s_red[threadIdx.x] = acc;
-    __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[blockIdx.x] = s_red[0];
+    acc = block256_sum(acc, s4);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc;
 }
 
 // Backward through the hidden layers, kSpb samples per workgroup (nb a
@@ -256,7 +264,7 @@ k_mlp_back_hidden(const float* __restrict__ dh2, const __bf16* __restrict__ h2, 
 // its partial sums to part[blockIdx.x][kSmallGrads] (summed in order afterwards).
 // Output o: [0, 4096) dW2[i][j]; [4096, 8192) dW1[i][j]; then db2, db1.
 constexpr uint32_t kSmallGrads = 2 * kMlpH * kMlpH + 2 * kMlpH;
-constexpr uint32_t kSgChunk = 32;
+constexpr uint32_t kSgChunk = 8;  // 256 workgroups for a batch of 2048
 __global__ void __launch_bounds__(256)
 k_mlp_small_grads(const float* __restrict__ da2, const float* __restrict__ h1, const float* __restrict__ da1,
                   const uint64_t* __restrict__ keys, float* __restrict__ part)
@@ -295,14 +303,30 @@ k_mlp_small_grads(const float* __restrict__ da2, const float* __restrict__ h1, c
     }
 }
 
-// grad[W1, b1, W2, b2] from the per-workgroup partials of k_mlp_small_grads
+// grad[W1, b1, W2, b2] from the per-workgroup partials of k_mlp_small_grads:
+// 64 outputs per workgroup (consecutive, so each partial row is read
+// coalesced), the partials split over the four waves, eight loads in flight
+// per thread, the quarters combined through LDS; all in a fixed order
 __global__ void __launch_bounds__(256)
 k_small_grads_reduce(const float* __restrict__ part, uint32_t nblk, float* __restrict__ grad)
 {
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= kSmallGrads) return;
-    float acc = 0.f;
-    for (uint32_t b = 0; b < nblk; ++b) acc += part[(size_t)b * kSmallGrads + o];
+    __shared__ float s_q[4][64];
+    const uint32_t lane = threadIdx.x % 64, q = threadIdx.x / 64;
+    const uint32_t o = blockIdx.x * 64 + lane;
+    const uint32_t per = (nblk + 3) / 4, b0 = q * per, b1 = min(nblk, b0 + per);
+    float a8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (o < kSmallGrads) {
+        uint32_t b = b0;
+        for (; b + 8 <= b1; b += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a8[u] += part[(size_t)(b + u) * kSmallGrads + o];
+        }
+        for (; b < b1; ++b) a8[(b - b0) & 7] += part[(size_t)b * kSmallGrads + o];
+    }
+    s_q[q][lane] = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
+    __syncthreads();
+    if (q != 0 || o >= kSmallGrads) return;
+    const float acc = (s_q[0][lane] + s_q[1][lane]) + (s_q[2][lane] + s_q[3][lane]);
     uint32_t dst;
     if (o < kMlpH * kMlpH) dst = kMlpOffW2 + o;
     else if (o < 2 * kMlpH * kMlpH) dst = o - kMlpH * kMlpH;
@@ -315,11 +339,15 @@ k_small_grads_reduce(const float* __restrict__ part, uint32_t nblk, float* __res
 // W1, W2 (row major), W3 [C][H] and W3^T [H][C].
 __global__ void __launch_bounds__(256)
 k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, uint32_t n,
-       float lr, float b1, float b2, float eps, float c1, float c2, __bf16* __restrict__ w1b, __bf16* __restrict__ w2b,
-       __bf16* __restrict__ w3b, __bf16* __restrict__ w3tb)
+       const uint32_t* __restrict__ step, uint32_t steps, float lr0, float b1, float b2, float eps,
+       __bf16* __restrict__ w1b, __bf16* __restrict__ w2b, __bf16* __restrict__ w3b, __bf16* __restrict__ w3tb)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // step size with cosine decay to zero, and Adam's bias corrections, from the step counter
+    const uint32_t t = *step;
+    const float lr = lr0 * 0.5f * (1.0f + cosf(3.14159265f * (float)t / (float)steps));
+    const float c1 = 1.f - powf(b1, (float)(t + 1)), c2 = 1.f - powf(b2, (float)(t + 1));
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -328,6 +356,29 @@ k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
     write_bf16_copy(i, pi, w1b, w2b, w3b, w3tb);
+}
+
+// End of a step: the batch loss (the rows' terms summed in a fixed order) into
+// loss[*step], then the step counter moves on (the next step reads it).  One
+// workgroup of 256 threads.
+__global__ void __launch_bounds__(256)
+k_step_end(const float* __restrict__ row_loss, uint32_t nb, float* __restrict__ loss, uint32_t* __restrict__ step)
+{
+    __shared__ float s4[4];
+    float acc = 0.f;
+    for (uint32_t i = threadIdx.x; i < nb; i += 256) acc += row_loss[i];
+    acc = block256_sum(acc, s4);
+    if (threadIdx.x == 0) {
+        const uint32_t t = step ? *step : 0u;
+        loss[t] = acc;
+        if (step) *step = t + 1;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_zero_f32(float* __restrict__ p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
 }
 
 __global__ void __launch_bounds__(256)
@@ -340,12 +391,12 @@ k_to_bf16_copies(const float* __restrict__ p, uint32_t n, __bf16* __restrict__ w
 
 // Batch of sorted-sample indices from a counter-based hash; label = quantile bucket
 __global__ void __launch_bounds__(256)
-k_gather_batch(const uint64_t* __restrict__ sorted, uint64_t n, uint64_t seed, uint32_t nb, uint64_t* __restrict__ keys,
-               uint32_t* __restrict__ labels)
+k_gather_batch(const uint64_t* __restrict__ sorted, uint64_t n, uint64_t seed, const uint32_t* __restrict__ step,
+               uint32_t nb, uint64_t* __restrict__ keys, uint32_t* __restrict__ labels)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
-    uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (i + 1);
+    uint64_t z = seed * 0x100000001B3ULL + *step + 0x9E3779B97F4A7C15ULL * (i + 1);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     z ^= z >> 31;
@@ -451,14 +502,16 @@ struct DevBuf {
 struct MlpWork {
     uint32_t cap = 0;  // batch capacity (multiple of 64)
     DevBuf params, grad, m, v, w1b, w2b, w3b, w3tb;
-    DevBuf keys, labels, h1, h2, h2t, logits, dz, dzt, dh2, da2, da1, loss, part;
+    DevBuf keys, labels, h1, h2, h2t, logits, dz, dzt, dh2, da2, da1, loss, row_loss, part, step;
+    uint32_t loss_slots = 1;  // training: one loss per step
 
     // split-K partial slabs and per-workgroup partial gradients
     static constexpr uint32_t kSplitW3 = 4, kSplitDh2 = 16;
 
-    int init(uint32_t batch, std::string& err, bool train = true)
+    int init(uint32_t batch, std::string& err, bool train = true, uint32_t steps = 1)
     {
         cap = batch;
+        loss_slots = steps ? steps : 1;
         LCHK(params.alloc(sizeof(float) * kMlpNParam));
         LCHK(grad.alloc(sizeof(float) * kMlpNParam));
         LCHK(m.alloc(sizeof(float) * kMlpNParam));
@@ -475,7 +528,9 @@ struct MlpWork {
         LCHK(logits.alloc(4ULL * batch * kMlpC));
         LCHK(dz.alloc(2ULL * batch * kMlpC));
         LCHK(dzt.alloc(2ULL * batch * kMlpC));
-        LCHK(loss.alloc(4));
+        LCHK(loss.alloc(4ULL * loss_slots));
+        LCHK(row_loss.alloc(4ULL * batch));
+        LCHK(step.alloc(4));
         if (!train) return 0;
         LCHK(dh2.alloc(4ULL * kSplitDh2 * batch * kMlpH));
         LCHK(da2.alloc(4ULL * batch * kMlpH));
@@ -497,10 +552,10 @@ static hipError_t gemm_nt(const __bf16* A, const __bf16* Bt, float* C, uint32_t 
     return hipGetLastError();
 }
 
-// loss and gradient of the batch already in w.keys / w.labels (nb <= cap, multiple of 64)
+// loss and gradient of the batch already in w.keys / w.labels (nb <= cap,
+// multiple of 64); each row's loss term goes to row_loss (k_step_end sums them)
 static int mlp_loss_grad(MlpWork& w, uint32_t nb, hipStream_t st, std::string& err)
 {
-    LCHK(hipMemsetAsync(w.loss.p, 0, 4, st));
     hipLaunchKernelGGL(k_mlp_hidden, dim3(nb / kSpb), dim3(256), 0, st, w.keys.as<uint64_t>(), nb,
                        w.params.as<float>(), w.w1b.as<__bf16>(), w.w2b.as<__bf16>(), w.h1.as<float>(),
                        w.h2.as<__bf16>(), w.h2t.as<__bf16>(), nb);
@@ -509,7 +564,7 @@ static int mlp_loss_grad(MlpWork& w, uint32_t nb, hipStream_t st, std::string& e
     LCHK(gemm_nt(w.h2.as<__bf16>(), w.w3b.as<__bf16>(), w.logits.as<float>(), nb, kMlpC, kMlpH, kMlpH, kMlpH, kMlpC, st));
     hipLaunchKernelGGL(k_softmax_ce, dim3(nb), dim3(256), 0, st, w.logits.as<float>(),
                        w.params.as<float>() + kMlpOffB3, w.labels.as<uint32_t>(), nb, w.dz.as<__bf16>(),
-                       w.loss.as<float>());
+                       w.row_loss.as<float>());
     LCHK(hipGetLastError());
     hipLaunchKernelGGL(k_transpose_bf16, dim3(kMlpC / 64, nb / 64), dim3(256), 0, st, w.dz.as<__bf16>(),
                        w.dzt.as<__bf16>(), nb, kMlpC);
@@ -537,7 +592,7 @@ static int mlp_loss_grad(MlpWork& w, uint32_t nb, hipStream_t st, std::string& e
     hipLaunchKernelGGL(k_mlp_small_grads, dim3(nb / kSgChunk), dim3(256), 0, st, w.da2.as<float>(), w.h1.as<float>(),
                        w.da1.as<float>(), w.keys.as<uint64_t>(), w.part.as<float>());
     LCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_small_grads_reduce, dim3((kSmallGrads + 255) / 256), dim3(256), 0, st, w.part.as<float>(),
+    hipLaunchKernelGGL(k_small_grads_reduce, dim3((kSmallGrads + 63) / 64), dim3(256), 0, st, w.part.as<float>(),
                        nb / kSgChunk, g);
     LCHK(hipGetLastError());
     return 0;
@@ -582,7 +637,11 @@ int mlp_debug_loss_grad(const float* params, const uint64_t* keys, const uint32_
     LCHK(dcopy(w.keys.p, keys, 8ULL * nb, hipMemcpyHostToDevice));
     LCHK(dcopy(w.labels.p, labels, 4ULL * nb, hipMemcpyHostToDevice));
     LCHK(hipMemset(w.grad.p, 0, sizeof(float) * kMlpNParam));
+    LCHK(hipMemset(w.loss.p, 0, 4));
     if (mlp_loss_grad(w, nb, st, err)) return -2;
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, st, w.row_loss.as<float>(), nb, w.loss.as<float>(),
+                       (uint32_t*)nullptr);
+    LCHK(hipGetLastError());
     LCHK(hipDeviceSynchronize());
     LCHK(dcopy(grad, w.grad.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost));
     LCHK(dcopy(loss, w.loss.p, 4, hipMemcpyDeviceToHost));
@@ -604,40 +663,92 @@ int mlp_debug_gemm(const uint16_t* A, const uint16_t* Bt, float* C, uint32_t M, 
     return 0;
 }
 
+// One training step on `st`: batch, loss and gradient, Adam, step counter.
+// Everything that changes from step to step is read from the device (the
+// step counter), so a run of steps can be captured once and replayed.
+static int train_step(MlpWork& w, const uint64_t* d_sorted, uint64_t n, const MlpTrainOpts& o, hipStream_t st,
+                      std::string& err)
+{
+    const uint32_t* step = w.step.as<uint32_t>();
+    hipLaunchKernelGGL(k_gather_batch, dim3((o.batch + 255) / 256), dim3(256), 0, st, d_sorted, n, o.seed, step,
+                       o.batch, w.keys.as<uint64_t>(), w.labels.as<uint32_t>());
+    LCHK(hipGetLastError());
+    if (mlp_loss_grad(w, o.batch, st, err)) return -2;
+    hipLaunchKernelGGL(k_adam, dim3((kMlpNParam + 255) / 256), dim3(256), 0, st, w.params.as<float>(),
+                       w.grad.as<float>(), w.m.as<float>(), w.v.as<float>(), (uint32_t)kMlpNParam, step, o.steps, o.lr,
+                       0.9f, 0.999f, 1e-8f, w.w1b.as<__bf16>(), w.w2b.as<__bf16>(), w.w3b.as<__bf16>(),
+                       w.w3tb.as<__bf16>());
+    LCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, st, w.row_loss.as<float>(), o.batch, w.loss.as<float>(),
+                       w.step.as<uint32_t>());
+    LCHK(hipGetLastError());
+    return 0;
+}
+
 int mlp_train(const uint64_t* d_sorted, uint64_t n, const MlpTrainOpts& o, hipStream_t st, std::vector<float>& params,
               std::vector<float>& loss_curve, std::string& err)
 {
     MlpWork w;
-    if (w.init(o.batch, err)) return -3;
+    if (w.init(o.batch, err, true, o.steps)) return -3;
     mlp_init_params(o.seed, params);
     if (upload_params(w, params.data(), st, err)) return -2;
-    LCHK(hipMemsetAsync(w.m.p, 0, sizeof(float) * kMlpNParam, st));
-    LCHK(hipMemsetAsync(w.v.p, 0, sizeof(float) * kMlpNParam, st));
-    loss_curve.clear();
-    float b1p = 1.f, b2p = 1.f;
-    for (uint32_t step = 0; step < o.steps; ++step) {
-        hipLaunchKernelGGL(k_gather_batch, dim3((o.batch + 255) / 256), dim3(256), 0, st, d_sorted, n,
-                           o.seed * 0x100000001B3ULL + step, o.batch, w.keys.as<uint64_t>(), w.labels.as<uint32_t>());
-        LCHK(hipGetLastError());
-        if (mlp_loss_grad(w, o.batch, st, err)) return -2;
-        b1p *= 0.9f;
-        b2p *= 0.999f;
-        // cosine decay of the step size
-        const float lr = o.lr * 0.5f * (1.0f + std::cos(3.14159265f * (float)step / (float)o.steps));
-        hipLaunchKernelGGL(k_adam, dim3((kMlpNParam + 255) / 256), dim3(256), 0, st, w.params.as<float>(),
-                           w.grad.as<float>(), w.m.as<float>(), w.v.as<float>(), (uint32_t)kMlpNParam, lr, 0.9f,
-                           0.999f, 1e-8f, 1.f - b1p, 1.f - b2p, w.w1b.as<__bf16>(), w.w2b.as<__bf16>(),
-                           w.w3b.as<__bf16>(), w.w3tb.as<__bf16>());
-        LCHK(hipGetLastError());
-        if (step % 50 == 0 || step + 1 == o.steps) {
-            float l = 0;
-            LCHK(dcopy_async(&l, w.loss.p, 4, hipMemcpyDeviceToHost, st));
-            LCHK(hipStreamSynchronize(st));
-            loss_curve.push_back(l);
+    hipLaunchKernelGGL(k_zero_f32, dim3((kMlpNParam + 255) / 256), dim3(256), 0, st, w.m.as<float>(), (uint32_t)kMlpNParam);
+    hipLaunchKernelGGL(k_zero_f32, dim3((kMlpNParam + 255) / 256), dim3(256), 0, st, w.v.as<float>(), (uint32_t)kMlpNParam);
+    hipLaunchKernelGGL(k_zero_f32, dim3((o.steps + 255) / 256), dim3(256), 0, st, w.loss.as<float>(), o.steps);
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(1), 0, st, w.step.as<float>(), 1u);  // step counter 0
+    LCHK(hipGetLastError());
+    // runs of kGraphSteps steps are captured once and replayed (each step is
+    // about 16 short kernels: launched one by one, the launches took about
+    // half of the training time); the steps left over run eagerly
+    // (KMC_TRAIN_GRAPH=0: every step launched eagerly, for profilers that do
+    // not follow graph capture)
+    constexpr uint32_t kGraphSteps = 50;
+    const char* tg = std::getenv("KMC_TRAIN_GRAPH");
+    const uint32_t runs = (tg && tg[0] == '0') ? 0u : o.steps / kGraphSteps;
+    uint32_t done = 0;
+    if (runs > 1) {
+        hipStream_t cs = nullptr;
+        LCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        // the capture stream starts after what `st` has queued, and `st` waits for the replays
+        hipEvent_t ev = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(ev, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cs, ev, 0);
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        int rc = 0;
+        if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (uint32_t i = 0; i < kGraphSteps && !rc; ++i) rc = train_step(w, d_sorted, n, o, cs, err);
+            const hipError_t ee = hipStreamEndCapture(cs, &g);
+            if (!rc && ee != hipSuccess) e = ee;
         }
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        for (uint32_t r = 0; r < runs && !rc && e == hipSuccess; ++r) e = hipGraphLaunch(ge, cs);
+        if (!rc && e == hipSuccess) e = hipEventRecord(ev, cs);
+        if (!rc && e == hipSuccess) e = hipStreamWaitEvent(st, ev, 0);
+        if (e == hipSuccess) e = hipStreamSynchronize(cs);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        if (ev) (void)hipEventDestroy(ev);
+        (void)hipStreamDestroy(cs);
+        if (rc) return rc;
+        if (e != hipSuccess) {
+            err = std::string("training graph: ") + hipGetErrorString(e);
+            return -2;
+        }
+        done = runs * kGraphSteps;
     }
+    for (uint32_t step = done; step < o.steps; ++step)
+        if (train_step(w, d_sorted, n, o, st, err)) return -2;
+    // the loss every 50 steps and at the last step
+    std::vector<float> all(o.steps);
+    LCHK(dcopy_async(all.data(), w.loss.p, sizeof(float) * o.steps, hipMemcpyDeviceToHost, st));
     LCHK(dcopy_async(params.data(), w.params.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
+    loss_curve.clear();
+    for (uint32_t step = 0; step < o.steps; ++step)
+        if (step % 50 == 0 || step + 1 == o.steps) loss_curve.push_back(all[step]);
     return 0;
 }
 

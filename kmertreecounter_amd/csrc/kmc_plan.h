@@ -24,6 +24,7 @@ struct PlanResult {
     std::vector<float> loss_curve;      // training loss every 50 steps
     uint64_t train_keys = 0;            // sorted sample keys the model saw
     uint32_t split_buckets = 0;         // empty learned buckets refilled by split_merged_buckets
+    uint32_t fallback = 0;              // learned plan kept the prefix partition: 1 balanced sample, 2 too few keys
     double train_ms = 0.0;
     double sample_max_over_mean_learned = 0.0;  // over the 4096 learned buckets, on the sample
     double sample_max_over_mean_prefix = 0.0;   // over the 4096 12-bit prefixes, on the sample
@@ -41,6 +42,9 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
 // Train the CDF model on the sample's keys and fill out.learned_b (and its
 // stats).  A sample with fewer distinct keys than buckets leaves learned_b
 // empty: the run then uses the prefix partition.
-int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, PlanResult& out, std::string& err);
+// The network is trained only when the sample's largest 12-bit prefix holds
+// more than min_skew times their mean distinct keys.
+int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, hipStream_t st, PlanResult& out,
+                 std::string& err);
 
 }  // namespace kmc

@@ -5,9 +5,13 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <numeric>
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 #include "kmc_fault.h"
 #include "kmc_kernels.h"
@@ -127,6 +131,33 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
 
 namespace {
 
+// LSD radix sort of 64-bit words, 16-bit digits; a digit that every word
+// shares is skipped (one counting pass for all four digits first).
+void radix_sort_u64(std::vector<uint64_t>& v)
+{
+    const size_t n = v.size();
+    if (n < 2) return;
+    std::vector<uint64_t> tmp(n);
+    std::vector<uint32_t> cnt(4 * 65536, 0);
+    for (uint64_t x : v)
+        for (int d = 0; d < 4; ++d) cnt[(size_t)d * 65536 + ((x >> (16 * d)) & 0xFFFF)]++;
+    uint64_t* src = v.data();
+    uint64_t* dst = tmp.data();
+    for (int d = 0; d < 4; ++d) {
+        uint32_t* c = cnt.data() + (size_t)d * 65536;
+        if (c[(src[0] >> (16 * d)) & 0xFFFF] == n) continue;  // one value of this digit: order kept
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < 65536; ++i) {
+            const uint32_t k = c[i];
+            c[i] = run;
+            run += k;
+        }
+        for (size_t i = 0; i < n; ++i) dst[c[(src[i] >> (16 * d)) & 0xFFFF]++] = src[i];
+        std::swap(src, dst);
+    }
+    if (src != v.data()) std::memcpy(v.data(), src, n * sizeof(uint64_t));
+}
+
 // max/mean of bucket sizes, buckets given by sorted lower bounds over sorted keys
 double imbalance(const std::vector<uint64_t>& sorted, const std::vector<uint64_t>& lower_bounds)
 {
@@ -143,7 +174,8 @@ double imbalance(const std::vector<uint64_t>& sorted, const std::vector<uint64_t
 
 }  // namespace
 
-int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, PlanResult& out, std::string& err)
+int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, hipStream_t st, PlanResult& out,
+                 std::string& err)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t nw = key_words(k);
@@ -176,10 +208,10 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     unsigned long long nkeys = 0;
     if (e == hipSuccess) e = dcopy_async(&nkeys, coff + 256, sizeof(nkeys), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    std::vector<uint64_t> all;
+    std::unique_ptr<uint64_t[]> all;  // not value-initialised: the copy fills it
     if (e == hipSuccess && nkeys) {
-        all.resize(nkeys * nw);
-        e = dcopy(all.data(), d_keys, nkeys * 8ULL * nw, hipMemcpyDeviceToHost);
+        all.reset(new (std::nothrow) uint64_t[nkeys * nw]);
+        e = all ? dcopy(all.get(), d_keys, nkeys * 8ULL * nw, hipMemcpyDeviceToHost) : hipErrorOutOfMemory;
     }
     if (e != hipSuccess) {
         cleanup();
@@ -191,22 +223,40 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     // wrote the keys in, and a kept key keeps all its occurrences.
     const uint64_t cap = 4ULL << 20;
     const uint64_t stride = nkeys > cap ? (nkeys + cap - 1) / cap : 1;
+    // kept: hash below 2^64 / stride (about one key in `stride`)
+    const uint64_t keep_below = stride == 1 ? ~0ULL : ~0ULL / stride;
     std::vector<uint64_t> sorted;
     sorted.reserve(nkeys / stride + nkeys / (4 * stride) + 1);
     for (uint64_t i = 0; i < nkeys; ++i) {
         const uint64_t w0 = all[i * nw];
-        if (stride == 1 || splitmix64(w0) % stride == 0) sorted.push_back(w0);
+        if (stride == 1 || splitmix64(w0) < keep_below) sorted.push_back(w0);
     }
-    std::vector<uint64_t>().swap(all);
-    std::sort(sorted.begin(), sorted.end());
+    all.reset();
+    auto lap = [&, last = t0](const char* what) mutable {
+        if (!std::getenv("KMC_TIMING")) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[kmc] learned plan: %s %.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    };
+    lap("sample keys");
+    radix_sort_u64(sorted);
+    lap("sort");
     MlpTrainOpts o;
     // the network learns the quantiles of the keys it is shown
     std::vector<uint64_t> uniq(sorted);
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
     const std::vector<uint64_t>& train = o.distinct ? uniq : sorted;
-    if (uniq.size() < kMlpC) {
-        // fewer distinct sample keys than buckets: nothing to learn, the run
+    std::vector<uint64_t> pb(kMlpC);
+    for (uint32_t j = 0; j < kMlpC; ++j) pb[j] = (uint64_t)j << 52;
+    out.sample_max_over_mean_prefix = imbalance(sorted, pb);
+    out.sample_distinct_prefix = imbalance(uniq, pb);
+    if (uniq.size() < kMlpC || out.sample_distinct_prefix <= min_skew) {
+        // fewer distinct sample keys than buckets (nothing to learn), or 12-bit
+        // prefixes that already hold about as many distinct keys each (nothing
+        // to gain, and learned routing costs more than prefix bits): the run
         // keeps the prefix partition (learned_b stays empty)
+        out.fallback = uniq.size() < kMlpC ? 2u : 1u;
         cleanup();
         out.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return KC_OK;
@@ -219,20 +269,19 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, hipStream_t st, Pl
     }
     std::vector<float> params;
     int rc = mlp_train(d_keys, train.size(), o, st, params, out.loss_curve, err);
+    lap("train");
     std::vector<float> pred;
     if (!rc) rc = mlp_predict(params.data(), d_keys, train.size(), o.expect, st, pred, err);
+    lap("predict");
     cleanup();
     if (rc) return KC_ERR_HIP;
     boundaries_from_predictions(train, pred, out.learned_b);
+    lap("boundaries");
     out.split_buckets = split_merged_buckets(uniq, out.learned_b);
     out.train_keys = train.size();
     std::vector<uint64_t> lb(out.learned_b.begin(), out.learned_b.begin() + kMlpC);
     out.sample_max_over_mean_learned = imbalance(sorted, lb);
-    std::vector<uint64_t> pb(kMlpC);
-    for (uint32_t j = 0; j < kMlpC; ++j) pb[j] = (uint64_t)j << 52;
-    out.sample_max_over_mean_prefix = imbalance(sorted, pb);
     out.sample_distinct_learned = imbalance(uniq, lb);
-    out.sample_distinct_prefix = imbalance(uniq, pb);
     out.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return KC_OK;
 }

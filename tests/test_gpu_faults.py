@@ -25,7 +25,7 @@ def device_free_bytes() -> int:
 MODES = {
     # name: engine options, learned plan, device FASTQ parsing
     "prefix": (dict(), False, False),
-    "learned": (dict(partition=K.KC_PARTITION_LEARNED), True, False),
+    "learned": (dict(partition=K.KC_PARTITION_LEARNED, learned_min_skew_pct=1), True, False),
     "fastq_rounds": (dict(max_round_keys=600_000), False, True),
 }
 

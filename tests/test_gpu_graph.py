@@ -29,7 +29,7 @@ def test_graph_replays_match_oracle_and_eager(case):
         kw = {}
     keys, counts = C.count_table(C.extract(seq, k))
     part = K.KC_PARTITION_LEARNED if case == "learned" else K.KC_PARTITION_PREFIX
-    with K.Engine(k, partition=part, graph=True, **kw) as g, K.Engine(k, **kw) as eager:
+    with K.Engine(k, partition=part, graph=True, learned_min_skew_pct=1, **kw) as g, K.Engine(k, **kw) as eager:
         if part == K.KC_PARTITION_LEARNED:
             g.sample_block(seq[: len(seq) // 20])
             g.plan()

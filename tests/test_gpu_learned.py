@@ -57,6 +57,7 @@ def test_mlp_gradient_matches_oracle():
 
 
 def learned_engine_run(seq: bytes, k: int, partition: int, sample_frac: float = 0.01, **kw):
+    kw.setdefault("learned_min_skew_pct", 1)
     e = K.Engine(k, partition=partition, expected_bytes=len(seq), **kw)
     e.sample_block(seq[: max(1, int(len(seq) * sample_frac))])
     e.plan()
@@ -109,7 +110,7 @@ def test_learned_plan_is_deterministic_and_balanced_on_uniform_input():
     seq = K.generate(2, scale=10, seed=7, fmt="seq")
     stats = []
     for _ in range(2):
-        with K.Engine(31, partition=K.KC_PARTITION_LEARNED) as e:
+        with K.Engine(31, partition=K.KC_PARTITION_LEARNED, learned_min_skew_pct=1) as e:
             e.sample_block(seq[: len(seq) // 10])
             e.plan()
             stats.append(e.stats())
@@ -132,7 +133,29 @@ def test_learned_plan_falls_back_to_prefix_on_a_degenerate_sample():
     e = learned_engine_run(seq, 31, K.KC_PARTITION_LEARNED, sample_frac=0.5)
     s = e.stats()
     assert s.learned_train_keys == 0
+    assert s.learned_fallback == 2
     mk, mc = C.merge_outputs(e.firsts(), e.records(), 1)
     np.testing.assert_array_equal(mk, keys)
     np.testing.assert_array_equal(mc, counts)
     e.close()
+
+
+def test_learned_plan_keeps_prefixes_when_the_sample_is_balanced():
+    # a uniform genome: its canonical keys (the smaller of two uniform values)
+    # have density 2(1 - x), so the sample's 12-bit prefixes reach about twice
+    # their mean and no more; below the 250% default the plan does not train
+    # (the learned routing would cost more than the prefix bits and gain
+    # little) and the run uses the prefix partition; with a 1% threshold the
+    # same sample trains
+    seq = K.generate(4, scale=100, seed=8, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    for pct, trains in ((0, False), (1, True)):
+        e = learned_engine_run(seq, 31, K.KC_PARTITION_LEARNED, sample_frac=0.05, learned_min_skew_pct=pct)
+        s = e.stats()
+        assert 1.8 < s.prefix_sample_distinct_max_over_mean < 2.5
+        assert (s.learned_fallback == 0) == trains
+        assert (s.learned_train_keys > 0) == trains
+        mk, mc = C.merge_outputs(e.firsts(), e.records(), 1)
+        np.testing.assert_array_equal(mk, keys)
+        np.testing.assert_array_equal(mc, counts)
+        e.close()

@@ -115,7 +115,7 @@ def test_long_k_partitions_and_rounds(partition):
     k = 100
     seq = K.generate(1, scale=20, seed=31, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, k))
-    with K.Engine(k, partition=partition, max_round_keys=400_000) as e:
+    with K.Engine(k, partition=partition, max_round_keys=400_000, learned_min_skew_pct=1) as e:
         if partition == K.KC_PARTITION_LEARNED:
             e.sample_block(seq[: len(seq) // 10])
             e.plan()

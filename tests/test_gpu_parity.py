@@ -122,7 +122,7 @@ def test_wide_fanout_with_learned_routing(k, sub_bits):
     seq = K.generate(2, scale=200, seed=5 + k, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, k))
     for part in (K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED):
-        with K.Engine(k, partition=part, sub_bits=sub_bits) as e:
+        with K.Engine(k, partition=part, sub_bits=sub_bits, learned_min_skew_pct=1) as e:
             if part == K.KC_PARTITION_LEARNED:
                 e.sample_block(seq[: len(seq) // 5])
                 e.plan()
@@ -347,7 +347,7 @@ def test_bucket_range_rounds(k, partition):
     seq = K.generate(1, scale=10, seed=14, fmt="seq")
     nw = C.nwords(k)
     keys, counts = C.count_table(C.extract(seq, k))
-    with K.Engine(k, partition=partition, max_round_keys=1_500_000) as e:
+    with K.Engine(k, partition=partition, max_round_keys=1_500_000, learned_min_skew_pct=1) as e:
         if partition == K.KC_PARTITION_LEARNED:
             e.sample_block(seq[: len(seq) // 20])
             e.plan()
