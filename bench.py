@@ -248,7 +248,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
                   autotune: bool = False, closed_k: int | None = None) -> dict:
     """Resident run of one partition method: plan (learned only), copy, warm up, time `steps` passes."""
     part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
-    cnt = K.KC_COUNTER_BLOOM if counter == "bloom" else K.KC_COUNTER_EXACT
+    cnt = {"bloom": K.KC_COUNTER_BLOOM, "exact": K.KC_COUNTER_EXACT, "sort": K.KC_COUNTER_SORT}[counter]
     eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt, graph=graph)
     plan_ms = 0.0
     if part == K.KC_PARTITION_LEARNED:
@@ -264,7 +264,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     if autotune:  # untimed: time the candidates once, keep the fastest
         t_t = time.perf_counter()
         r = eng.autotune()
-        tuned = {"sub_bits": r.sub_bits, "counter": "exact" if r.counter == K.KC_COUNTER_EXACT else "bloom",
+        tuned = {"sub_bits": r.sub_bits, "counter": {K.KC_COUNTER_EXACT: "exact", K.KC_COUNTER_SORT: "sort"}.get(r.counter, "bloom"),
                  "heavy_keys": r.heavy_keys, "candidates": r.candidates, "best_ms": r.best_ms,
                  "tune_ms": (time.perf_counter() - t_t) * 1e3}
         log(f"[{partition}] autotune: {tuned}")
@@ -352,7 +352,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
 
 def eng_counter(eng, cnt: int, tuned: dict | None) -> int:
     if tuned:
-        return K.KC_COUNTER_EXACT if tuned["counter"] == "exact" else K.KC_COUNTER_BLOOM
+        return {"exact": K.KC_COUNTER_EXACT, "sort": K.KC_COUNTER_SORT}.get(tuned["counter"], K.KC_COUNTER_BLOOM)
     return cnt
 
 
@@ -499,7 +499,7 @@ def main() -> int:
     ap.add_argument("--workload", type=int, choices=sorted(WORKLOADS), help="one workload only")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--partition", choices=["prefix", "learned", "both"], default="both")
-    ap.add_argument("--counter", choices=["bloom", "exact"], default="bloom")
+    ap.add_argument("--counter", choices=["bloom", "exact", "sort"], default="bloom")
     ap.add_argument("--autotune", action="store_true", help="autotune every workload (untimed, before the steps)")
     ap.add_argument("--graph", action="store_true", help="replay every workload as a captured HIP graph")
     ap.add_argument("--eager", action="store_true", help="no graph and no autotuner for any workload")

@@ -81,7 +81,10 @@ typedef enum kc_partition {
  * of a pass, not only the repeated ones, so it takes more passes. */
 typedef enum kc_counter {
     KC_COUNTER_BLOOM = 0,
-    KC_COUNTER_EXACT = 1
+    KC_COUNTER_EXACT = 1,
+    KC_COUNTER_SORT = 2      /* the exact counter's split from a sort: each pass of a bucket's keys is
+                                sorted in LDS (stable LSD radix over the bits that vary) and counted by
+                                a run-length reduce (sort_and_export_leaf, src/NewKmerTree.h:1170-1250) */
 } kc_counter;
 
 typedef struct kc_config {
@@ -99,8 +102,9 @@ typedef struct kc_config {
                                  a pass aims to hold (0 = the table size) */
     uint32_t counter;         /* kc_counter */
     uint32_t table_slots;     /* exact counter: count-table slots per bucket and pass (0 = the whole
-                                 LDS table, 10240 for k <= 32 and 6144 above; smaller values narrow
-                                 each pass and are meant for tuning and tests) */
+                                 LDS table, 10240 for k <= 32 and 6144 above); sort counter: keys a
+                                 pass stages (0 = 8192 / N); smaller values narrow each pass and are
+                                 meant for tuning and tests */
     uint32_t heavy_keys;      /* a bucket of more keys than this (0 = 1Mi) is counted in chunks by the
                                  whole GPU instead of one workgroup; its keys are written as root
                                  records with their whole count (no low.bin entry) */
@@ -224,8 +228,8 @@ KC_API int kc_count_resident(kc_engine* h);
 
 /* Autotuner (no counterpart in the reference; its CPU pipeline has no such
  * parameters): times one pass over the resident input for each candidate --
- * the automatic fine fan-out and one bit less (tile and bucket size), the
- * Bloom and the exact counter, two heavy-bucket thresholds -- keeps the
+ * the Bloom, exact and sort counters, each with its automatic fine fan-out and
+ * one bit less (tile and bucket size), two heavy-bucket thresholds -- keeps the
  * fastest in the handle's configuration and leaves its results in place. */
 typedef struct kc_tune_result {
     uint32_t sub_bits;    /* chosen fine fan-out bits */

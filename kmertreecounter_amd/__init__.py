@@ -29,6 +29,7 @@ KC_PARTITION_LEARNED = 1
 KC_SUB_BITS_NONE = 0xFFFFFFFF
 KC_COUNTER_BLOOM = 0
 KC_COUNTER_EXACT = 1
+KC_COUNTER_SORT = 2
 KC_HISTOGRAM_PRECISE = 0
 KC_HISTOGRAM_APPROXIMATE = 1
 

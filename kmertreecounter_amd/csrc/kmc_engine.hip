@@ -25,6 +25,9 @@
 
 using namespace kmc;
 
+static_assert(kCounterBloom == KC_COUNTER_BLOOM && kCounterExact == KC_COUNTER_EXACT && kCounterSort == KC_COUNTER_SORT,
+              "kernel counter ids are the kc_counter values");
+
 namespace kmc {
 
 namespace {
@@ -113,8 +116,8 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     a.error_flags = b.ctr + kCtrErr;
     a.min_count = o.min_count;
     a.max_count = o.max_count;
-    // the exact counter reads 0 as its own default (its table size)
-    a.pass_keys = o.pass_keys ? o.pass_keys : (o.counter == KC_COUNTER_EXACT ? 0u : CountCfg<NW>::kPassKeys);
+    // the exact and sort counters read 0 as their own default (their table or stage size)
+    a.pass_keys = o.pass_keys ? o.pass_keys : (o.counter != KC_COUNTER_BLOOM ? 0u : CountCfg<NW>::kPassKeys);
     a.extra_passes = b.ctr + kCtrExtraPasses;
     a.table_slots = o.table_slots;
     a.heavy_keys = o.heavy_keys;
@@ -125,7 +128,7 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     a.heavy_next = b.ctr + kCtrHeavyNext;
     a.heavy_buckets = b.ctr + kCtrHeavyBuckets;
     PIPE_CHECK(mark(8));
-    PIPE_CHECK(launch_count<NW>(nf, o.counter != KC_COUNTER_EXACT, src, fine_off, a, st));
+    PIPE_CHECK(launch_count<NW>(nf, o.counter, src, fine_off, a, st));
     PIPE_CHECK(launch_heavy<NW>(o.heavy_workgroups, src, a, st));
     PIPE_CHECK(mark(9));
     PIPE_CHECK(launch_spill_emit<NW>(a, st));
@@ -618,7 +621,9 @@ uint32_t choose_sbits(const kc_engine* h, uint64_t padded_len)
     // short runs and runs at about half speed, so it is only worth it where the
     // counting kernel would otherwise pass over most buckets several times
     // (measured: workload 4 -> 9 bits, workload 5 -> 10 bits, workload 2 stays at 8).
-    const double pass_keys = h->nw == 1 ? CountCfg<1>::kPassKeys : h->nw == 2 ? CountCfg<2>::kPassKeys : CountCfg<4>::kPassKeys;
+    double pass_keys = h->nw == 1 ? CountCfg<1>::kPassKeys : h->nw == 2 ? CountCfg<2>::kPassKeys : CountCfg<4>::kPassKeys;
+    // the sort counter stages at most 8192 / N keys per pass: its buckets aim at 3/4 of that
+    if (h->cfg.counter == KC_COUNTER_SORT) pass_keys = 0.75 * (8192.0 / h->nw);
     while (sb >= 8 && sb < kMaxSubBits && per_coarse / (double)(1u << sb) > pass_keys) sb++;
     return sb;
 }
@@ -1028,7 +1033,7 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     const uint32_t nw = kc_words_for_k(cfg->k);
     if (!nw) return KC_ERR_ARG;
     if (cfg->partition > 1 || (cfg->sub_bits > kMaxSubBits && cfg->sub_bits != KC_SUB_BITS_NONE)) return KC_ERR_ARG;
-    if (cfg->counter > KC_COUNTER_EXACT || (cfg->table_slots && cfg->table_slots < 16)) return KC_ERR_ARG;
+    if (cfg->counter > KC_COUNTER_SORT || (cfg->table_slots && cfg->table_slots < 16)) return KC_ERR_ARG;
     if (cfg->max_count && cfg->min_count > cfg->max_count) return KC_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_ERR_NODEV;
@@ -1227,25 +1232,27 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
     HIPX(hipSetDevice(h->device));
     if (h->stage_len || h->fq_unsettled) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
     if (!h->seq_len) return h->fail(KC_ERR_STATE, "no resident input: push blocks and call kc_finish first");
-    // candidates: fine fan-out around the automatic one, both counters, two
-    // heavy-bucket thresholds; each is timed on one pass over the resident input
+    // candidates: for each counter (Bloom, exact, sort) its automatic fine
+    // fan-out and one bit less, two heavy-bucket thresholds; each is timed on
+    // one pass over the resident input
     const kc_config saved = h->cfg;
     const uint64_t tile = ext_tile(h->nw);
     const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
-    kc_config probe = saved;
-    probe.sub_bits = 0;
-    h->cfg = probe;
-    const uint32_t sb_auto = choose_sbits(h, padded);
-    std::vector<uint32_t> sbs{sb_auto};
-    if (sb_auto > 1) sbs.push_back(sb_auto - 1);
-    if (saved.sub_bits && saved.sub_bits != KC_SUB_BITS_NONE && saved.sub_bits != sb_auto) sbs.push_back(saved.sub_bits);
-    const uint32_t counters[2] = {KC_COUNTER_BLOOM, KC_COUNTER_EXACT};
+    const uint32_t counters[3] = {KC_COUNTER_BLOOM, KC_COUNTER_EXACT, KC_COUNTER_SORT};
     const uint32_t heavies[2] = {0u, 1u << 18};
     double best = 1e300;
     kc_config best_cfg = saved;
     uint32_t tried = 0;
-    for (uint32_t sb : sbs)
-        for (uint32_t c : counters)
+    for (uint32_t c : counters) {
+        kc_config probe = saved;
+        probe.sub_bits = 0;
+        probe.counter = c;
+        h->cfg = probe;
+        const uint32_t sb_auto = choose_sbits(h, padded);
+        std::vector<uint32_t> sbs{sb_auto};
+        if (sb_auto > 1) sbs.push_back(sb_auto - 1);
+        if (saved.sub_bits && saved.sub_bits != KC_SUB_BITS_NONE && saved.sub_bits != sb_auto) sbs.push_back(saved.sub_bits);
+        for (uint32_t sb : sbs)
             for (uint32_t hv : heavies) {
                 kc_config t = saved;
                 t.sub_bits = sb ? sb : KC_SUB_BITS_NONE;
@@ -1263,6 +1270,7 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
                     best_cfg = t;
                 }
             }
+    }
     h->cfg = best_cfg;
     const int rc = run_pipeline(h);  // results of the chosen configuration
     if (rc) return rc;

@@ -71,6 +71,14 @@ template <> struct CountCfg<4> {
     static constexpr uint32_t kExactSlots = 2560;
 };
 
+// Sort counter: one 1024-thread workgroup per fine bucket; a pass stages kKeys
+// keys (two LDS buffers of kKeys keys for the radix sort: 128 KiB) and sorts
+// them by 6-bit digits.
+constexpr int kSortThreads = 1024;
+constexpr uint32_t kSortDigitBits = 6;
+constexpr uint32_t kSortDigits = 1u << kSortDigitBits;
+template <int NW> struct SortCfg { static constexpr uint32_t kKeys = 8192 / NW; };
+
 // probes into the global spill table before a key is declared unplaceable
 constexpr uint32_t kSpillProbe = 1024;
 
@@ -146,9 +154,11 @@ template <int NW>
 hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const unsigned long long* coarse_off,
                       const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,
                       unsigned long long* cursor, uint64_t* B, hipStream_t st);
+// counter: the kc_counter values
+enum : uint32_t { kCounterBloom = 0, kCounterExact = 1, kCounterSort = 2 };
 template <int NW>
-hipError_t launch_count(uint32_t nf, bool bloom, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
-                        hipStream_t st);
+hipError_t launch_count(uint32_t nf, uint32_t counter, const uint64_t* B, const unsigned long long* fine_off,
+                        const CountArgs& args, hipStream_t st);
 template <int NW>
 hipError_t launch_heavy(uint32_t workgroups, const uint64_t* B, const CountArgs& args, hipStream_t st);
 template <int NW>

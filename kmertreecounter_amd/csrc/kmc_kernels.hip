@@ -1322,6 +1322,284 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
     }
 }
 
+// ------------------------------------------------------------------ sort counter
+
+// Sort counter (KC_COUNTER_SORT): the keys of a hash-range pass of the bucket
+// are staged in LDS, sorted by a stable LSD radix sort over the bits that vary
+// among them (6-bit digits; the bucket's shared prefix bits are skipped, as
+// sort/ExportRecordRadixSort.h:19-24 skips its 20 known prefix bits), and
+// counted by a run-length reduce over the sorted keys, as sort_and_export_leaf
+// does for a leaf (src/NewKmerTree.h:1170-1250).  The split is the exact
+// counter's: each distinct key leaves one first occurrence and, if it was seen
+// c >= 2 times, the root record (key, c - 1) under the min/max filter.
+// Passes take the keys whose top 16 hash bits lie in [hstart, hend), sized to
+// fill about 3/4 of the stage; a pass that overflows is narrowed and read
+// again, and a single hash value with more occurrences than the stage holds
+// (one key repeated that often) is counted in the global spill table (its
+// record keeps the whole count, with no first occurrence).
+template <int NW>
+__global__ void __launch_bounds__(kSortThreads)
+k_count_sort(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ fine_off, CountArgs args)
+{
+    constexpr int T = kSortThreads;
+    constexpr uint32_t NWV = T / kWave;              // waves
+    constexpr uint32_t S = SortCfg<NW>::kKeys;       // keys a pass stages
+    constexpr uint32_t P = S / T;                    // keys per thread and rank step
+    constexpr uint32_t NC = kSortDigits * P * NWV;   // rank counters: (digit, step, wave)
+    static_assert(NC % T == 0, "counters split evenly over the threads");
+    __shared__ unsigned long long s_k0[NW * S];      // word w of entry i at [w * S + i]
+    __shared__ unsigned long long s_k1[NW * S];
+    __shared__ uint16_t s_cnt[NC];
+    __shared__ uint32_t s_scan[NWV + 1];
+    __shared__ unsigned long long s_or[NW], s_and[NW];
+    __shared__ uint32_t s_m, s_nfirst, s_nrec, s_dist, s_dpass, s_passes, s_hstart, s_width, s_abort;
+
+    const int t = threadIdx.x;
+    const uint32_t lane = t & (kWave - 1), wave = t / kWave;
+    const uint32_t f = blockIdx.x;
+    const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
+    const unsigned long long n = hi - lo;
+    const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
+    uint8_t* rec_region = args.rec_tmp + lo * 8ULL * NW;
+    if (hand_over_heavy(args, f, lo, hi)) return;
+    // keys a pass may stage (a smaller stage is for tests); passes aim at 3/4 of it
+    const uint32_t cap = args.table_slots ? min(args.table_slots, S) : S;
+    const uint32_t aim = cap - cap / 4;
+    if (t == 0) {
+        s_nfirst = 0;
+        s_nrec = 0;
+        s_dist = 0;
+        s_passes = 0;
+        s_hstart = 0;
+        s_width = n <= cap ? 65536u : (uint32_t)max(1ULL, (65536ULL * aim) / n);
+    }
+    const uint64_t below = (1ULL << lane) - 1ULL;
+
+    for (;;) {
+        if (t == 0) {
+            s_m = 0;
+            s_dpass = 0;
+            s_abort = (__atomic_load_n(args.error_flags, __ATOMIC_RELAXED) & kErrSpillFull) ? 1u : 0u;
+        }
+        if (t < NW) {
+            s_or[t] = 0ULL;
+            s_and[t] = ~0ULL;
+        }
+        __syncthreads();
+        if (s_abort) break;
+        const uint32_t hstart = s_hstart;
+        const uint32_t hend = min(65536u, hstart + s_width);
+        // stage the pass's keys (wave-compacted appends)
+        for (unsigned long long b0 = lo; b0 < hi; b0 += T) {
+            const unsigned long long i = b0 + t;
+            Key<NW> key;
+            bool in = false;
+            if (i < hi) {
+                key = load_key<NW>(B, i);
+                const uint32_t hb = (uint32_t)(count_mix<NW>(key) >> 48);
+                in = hb >= hstart && hb < hend;
+            }
+            const uint64_t bal = __ballot(in);
+            uint32_t wbase = 0;
+            if (lane == 0 && bal) wbase = atomicAdd(&s_m, (uint32_t)__popcll(bal));
+            wbase = __shfl(wbase, 0, kWave);
+            const uint32_t pos = wbase + (uint32_t)__popcll(bal & below);
+            if (in && pos < cap) {
+#pragma unroll
+                for (int w = 0; w < NW; ++w) s_k0[(uint32_t)w * S + pos] = key.w[w];
+            }
+        }
+        __syncthreads();
+        const uint32_t m = s_m;
+        if (m > cap) {
+            if (hend - hstart > 1) {
+                // too many keys in the range: the same start, a narrower range
+                if (t == 0) s_width = max(1u, (uint32_t)(((uint64_t)(hend - hstart) * aim) / m));
+                __syncthreads();
+                continue;
+            }
+            // one hash value with more keys than the stage: the global table
+            // counts them (runs of equal keys in a wave row added once)
+            for (unsigned long long b0 = lo; b0 < hi; b0 += T) {
+                const unsigned long long i = b0 + t;
+                Key<NW> key;
+                if (i < hi) key = load_key<NW>(B, i);
+                const uint32_t run = run_head_length<NW>(key, i < hi);
+                if (run && (uint32_t)(count_mix<NW>(key) >> 48) == hstart) global_add<NW>(args, key, run);
+            }
+            __syncthreads();
+            if (t == 0) {
+                s_passes++;
+                s_hstart = hend;
+                s_width = 1;
+            }
+            __syncthreads();
+            if (s_hstart >= 65536u) break;
+            continue;
+        }
+        // bits that vary among the staged keys
+        {
+            unsigned long long vo[NW], va[NW];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                vo[w] = 0ULL;
+                va[w] = ~0ULL;
+            }
+            for (uint32_t i = t; i < m; i += T) {
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const unsigned long long x = s_k0[(uint32_t)w * S + i];
+                    vo[w] |= x;
+                    va[w] &= x;
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                if (vo[w]) atomicOr(&s_or[w], vo[w]);
+                if (va[w] != ~0ULL) atomicAnd(&s_and[w], va[w]);
+            }
+        }
+        __syncthreads();
+        // stable LSD radix sort: least significant word first, 6-bit digits
+        unsigned long long* src = s_k0;
+        unsigned long long* dst = s_k1;
+        for (int w = NW - 1; w >= 0; --w) {
+            const unsigned long long vary = s_or[w] ^ s_and[w];
+            if (!vary) continue;
+            const uint32_t lob = (uint32_t)__ffsll((long long)vary) - 1u;
+            const uint32_t hib = 63u - (uint32_t)__clzll((long long)vary);
+            for (uint32_t sh = lob; sh <= hib; sh += kSortDigitBits) {
+                for (uint32_t c = t; c < NC; c += T) s_cnt[c] = 0;
+                __syncthreads();
+                // rank among the wave's lanes of equal digit (ballot per digit bit);
+                // entry i = j * T + t, so (digit, j, wave, lane) order is entry order
+                uint32_t dg[P], rk[P];
+#pragma unroll
+                for (uint32_t j = 0; j < P; ++j) {
+                    const uint32_t i = j * T + t;
+                    const bool valid = i < m;
+                    const uint32_t d = valid ? (uint32_t)((src[(uint32_t)w * S + i] >> sh) & (kSortDigits - 1)) : 0u;
+                    uint64_t peers = __ballot(valid);
+#pragma unroll
+                    for (uint32_t bit = 0; bit < kSortDigitBits; ++bit) {
+                        const uint64_t bb = __ballot((d >> bit) & 1u);
+                        peers &= ((d >> bit) & 1u) ? bb : ~bb;
+                    }
+                    dg[j] = d;
+                    rk[j] = (uint32_t)__popcll(peers & below);
+                    if (valid && rk[j] == 0) s_cnt[(d * P + j) * NWV + wave] = (uint16_t)__popcll(peers);
+                }
+                __syncthreads();
+                // exclusive scan of the counters in (digit, step, wave) order
+                {
+                    constexpr uint32_t PER = NC / T;
+                    uint32_t v[PER], sum = 0;
+#pragma unroll
+                    for (uint32_t q = 0; q < PER; ++q) {
+                        v[q] = s_cnt[t * PER + q];
+                        sum += v[q];
+                    }
+                    uint32_t total;
+                    uint32_t run = block_exclusive_scan<uint32_t>(sum, s_scan, &total);
+#pragma unroll
+                    for (uint32_t q = 0; q < PER; ++q) {
+                        s_cnt[t * PER + q] = (uint16_t)run;
+                        run += v[q];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (uint32_t j = 0; j < P; ++j) {
+                    const uint32_t i = j * T + t;
+                    if (i < m) {
+                        const uint32_t pos = (uint32_t)s_cnt[(dg[j] * P + j) * NWV + wave] + rk[j];
+#pragma unroll
+                        for (int ww = 0; ww < NW; ++ww) dst[(uint32_t)ww * S + pos] = src[(uint32_t)ww * S + i];
+                    }
+                }
+                __syncthreads();
+                unsigned long long* x = src;
+                src = dst;
+                dst = x;
+            }
+        }
+        // run-length reduce: run heads (entries that differ from their
+        // predecessor), their positions listed in entry order in `dst`
+        uint32_t* heads = reinterpret_cast<uint32_t*>(dst);
+        {
+            uint32_t hf[P], nh = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < P; ++q) {
+                const uint32_t i = t * P + q;  // blocked: each thread's entries are consecutive
+                bool h = false;
+                if (i < m) {
+                    h = i == 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) h = h || src[(uint32_t)w * S + i] != src[(uint32_t)w * S + i - (i ? 1 : 0)];
+                }
+                hf[q] = h ? 1u : 0u;
+                nh += hf[q];
+            }
+            uint32_t total;
+            uint32_t at = block_exclusive_scan<uint32_t>(nh, s_scan, &total);
+#pragma unroll
+            for (uint32_t q = 0; q < P; ++q)
+                if (hf[q]) heads[at++] = t * P + q;
+            if (t == 0) heads[total] = m;
+            __syncthreads();
+            // one run per lane: first occurrence, and the record of c - 1
+            uint32_t dist = 0;
+            for (uint32_t h0 = 0; h0 < total; h0 += T) {
+                const uint32_t hidx = h0 + t;
+                const bool valid = hidx < total;
+                uint32_t c = 0;
+                Key<NW> key;
+                if (valid) {
+                    const uint32_t i = heads[hidx];
+                    c = heads[hidx + 1] - i;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) key.w[w] = src[(uint32_t)w * S + i];
+                }
+                const uint64_t bf = __ballot(valid);
+                uint32_t fbase = 0;
+                if (lane == 0 && bf) fbase = atomicAdd(&s_nfirst, (uint32_t)__popcll(bf));
+                fbase = __shfl(fbase, 0, kWave);
+                if (valid) store_key<NW>(args.firsts, lo + fbase + (uint32_t)__popcll(bf & below), key);
+                dist += c >= 2 ? 1u : 0u;
+                const uint32_t r = c - 1;
+                const bool keep = valid && c >= 2 && r >= args.min_count && r <= args.max_count;
+                const uint64_t bal = __ballot(keep);
+                uint32_t wbase = 0;
+                if (lane == 0 && bal) wbase = atomicAdd(&s_nrec, (uint32_t)__popcll(bal));
+                wbase = __shfl(wbase, 0, kWave);
+                if (keep) {
+                    const uint64_t pos = wbase + (uint32_t)__popcll(bal & below);
+                    if (pos < rec_cap) write_record<NW>(rec_region, pos, key, r);
+                    else append_overflow<NW>(args, key, r);
+                }
+            }
+            if (dist) atomicAdd(&s_dpass, dist);
+        }
+        __syncthreads();
+        if (t == 0) {
+            s_dist += s_dpass;
+            s_passes++;
+            s_hstart = hend;
+            // next range: about `aim` keys at the density this pass held
+            const uint64_t wd = ((uint64_t)(hend - hstart) * aim) / (m ? m : 1u);
+            s_width = (uint32_t)(wd < 1 ? 1 : (wd > 65536 ? 65536 : wd));
+        }
+        __syncthreads();
+        if (s_hstart >= 65536u) break;
+    }
+    if (t == 0) {
+        args.nfirst[f] = s_nfirst;
+        args.nrec[f] = (uint32_t)(s_nrec < rec_cap ? s_nrec : rec_cap);
+        args.ndist[f] = s_dist;
+        if (s_passes > 1) atomicAdd(args.extra_passes, (unsigned long long)(s_passes - 1));
+    }
+}
+
 // Chunks of the buckets handed over by the counting kernels, taken from a
 // work counter by a persistent grid (one workgroup per CU).  A chunk is
 // counted exactly in an LDS table (runs of equal keys in a wave counted once),
@@ -1688,13 +1966,15 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
 }
 
 template <int NW>
-hipError_t launch_count(uint32_t nf, bool bloom, const uint64_t* B, const unsigned long long* fine_off, const CountArgs& args,
-                        hipStream_t st)
+hipError_t launch_count(uint32_t nf, uint32_t counter, const uint64_t* B, const unsigned long long* fine_off,
+                        const CountArgs& args, hipStream_t st)
 {
-    if (bloom)
-        hipLaunchKernelGGL((k_count<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
-    else
+    if (counter == kCounterSort)
+        hipLaunchKernelGGL((k_count_sort<NW>), dim3(nf), dim3(kSortThreads), 0, st, B, fine_off, args);
+    else if (counter == kCounterExact)
         hipLaunchKernelGGL((k_count_exact<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
+    else
+        hipLaunchKernelGGL((k_count<NW>), dim3(nf), dim3(CountCfg<NW>::kThreads), 0, st, B, fine_off, args);
     KMC_LAUNCH_CHECK();
     return hipSuccess;
 }
@@ -1741,9 +2021,9 @@ template hipError_t launch_extract<4>(bool, const uint8_t*, uint64_t, uint32_t, 
 template hipError_t launch_sub<1>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_sub<2>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
 template hipError_t launch_sub<4>(bool, uint64_t, const uint64_t*, const unsigned long long*, const unsigned long long*, const Route&, unsigned long long*, unsigned long long*, uint64_t*, hipStream_t);
-template hipError_t launch_count<1>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
-template hipError_t launch_count<2>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
-template hipError_t launch_count<4>(uint32_t, bool, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<1>(uint32_t, uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<2>(uint32_t, uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
+template hipError_t launch_count<4>(uint32_t, uint32_t, const uint64_t*, const unsigned long long*, const CountArgs&, hipStream_t);
 template hipError_t launch_heavy<1>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
 template hipError_t launch_heavy<2>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);
 template hipError_t launch_heavy<4>(uint32_t, const uint64_t*, const CountArgs&, hipStream_t);

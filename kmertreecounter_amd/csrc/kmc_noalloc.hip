@@ -204,7 +204,7 @@ int kc_count_noalloc(const kc_config* cfg, const char* d_seq, uint64_t seq_len, 
     if (cfg->partition != KC_PARTITION_PREFIX) return fail(KC_ERR_ARG, "kc_count_noalloc uses the prefix partition only");
     if (cfg->sub_bits != 0 && cfg->sub_bits != KC_SUB_BITS_NONE)
         return fail(KC_ERR_ARG, "kc_count_noalloc counts the 256 root buckets (sub_bits 0)");
-    if (cfg->counter > KC_COUNTER_EXACT || (cfg->table_slots && cfg->table_slots < 16))
+    if (cfg->counter > KC_COUNTER_SORT || (cfg->table_slots && cfg->table_slots < 16))
         return fail(KC_ERR_ARG, "bad counter or table_slots");
     if (cfg->max_count && cfg->min_count > cfg->max_count) return fail(KC_ERR_ARG, "min_count above max_count");
     int ndev = 0;

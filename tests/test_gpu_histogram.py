@@ -37,7 +37,8 @@ INPUTS = {
 }
 
 
-@pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT], ids=["bloom", "exact"])
+@pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT, K.KC_COUNTER_SORT],
+                         ids=["bloom", "exact", "sort"])
 @pytest.mark.parametrize("name", sorted(INPUTS))
 def test_histogram_matches_oracle(name, counter):
     make, k = INPUTS[name]
@@ -90,7 +91,8 @@ def test_tree_run_writes_the_histogram_file(tmp_path):
     assert not (tmp_path / "tmp" / "low.bin").exists()  # histogram only: no files
 
 
-@pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT], ids=["bloom", "exact"])
+@pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT, K.KC_COUNTER_SORT],
+                         ids=["bloom", "exact", "sort"])
 @pytest.mark.parametrize("rounds", [False, True], ids=["one_round", "rounds"])
 def test_histogram_tail_totals(counter, rounds):
     # keys whose merged count is above max_freq are left out of the histogram

@@ -26,7 +26,8 @@ def as_rows(keys: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(keys).view(dt).ravel()
 
 
-COUNTERS = pytest.mark.parametrize("counter", [K.KC_COUNTER_EXACT, K.KC_COUNTER_BLOOM], ids=["exact", "bloom"])
+COUNTERS = pytest.mark.parametrize("counter", [K.KC_COUNTER_EXACT, K.KC_COUNTER_BLOOM, K.KC_COUNTER_SORT],
+                                   ids=["exact", "bloom", "sort"])
 
 
 def check_outputs(firsts, recs, stats, k, exp_keys, exp_counts, exact=False):
@@ -74,7 +75,7 @@ def test_small_random_stream(k, counter):
     seq = bytes(rng.choice(alphabet, size=300_000, p=p))
     keys, counts = C.count_table(C.extract(seq, k))
     f, r, s = run_engine([seq], k, counter=counter)
-    check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+    check_outputs(f, r, s, k, keys, counts, exact=counter != K.KC_COUNTER_BLOOM)
 
 
 @pytest.mark.parametrize("k", [1, 2, 5, 15, 16, 17, 31, 32, 33, 64, 65, 128])
@@ -196,7 +197,28 @@ def test_heavy_buckets_counted_in_chunks(counter, heavy_keys):
     assert 0 < s.heavy_buckets <= 256
     if heavy_keys == 9500:
         assert s.heavy_buckets < 256
-    check_outputs(f, r, s, 31, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+    check_outputs(f, r, s, 31, keys, counts, exact=counter != K.KC_COUNTER_BLOOM)
+
+
+@pytest.mark.parametrize("k", [31, 63, 100])
+@pytest.mark.parametrize("stage", [64, 1000])
+def test_sort_counter_small_stages(k, stage):
+    # a stage of 64 or 1000 keys: passes narrow their hash range when they
+    # overflow, and a hash value with more occurrences than the stage (a key
+    # repeated that often) is counted in the global spill table
+    seq = microsatellite_reads(20_000, 30 + k, tract_frac=0.3)
+    keys, counts = C.count_table(C.extract(seq, k))
+    assert counts.max() > 1000
+    f, r, s = run_engine([seq], k, counter=K.KC_COUNTER_SORT, table_slots=stage, sub_bits=2)
+    assert s.extra_passes > 0
+    assert s.spilled_keys > 0
+    check_outputs(f, r, s, k, keys, counts, exact=True)
+    # the split without spills: every distinct key is a first occurrence
+    seq = K.generate(1, scale=40, seed=3 + k, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, k))
+    f, r, s = run_engine([seq], k, counter=K.KC_COUNTER_SORT, table_slots=stage, sub_bits=2)
+    assert s.spilled_keys == 0 and s.firsts == keys.shape[0]
+    check_outputs(f, r, s, k, keys, counts, exact=True)
 
 
 def microsatellite_reads(n_reads, seed, tract_frac=0.3):
@@ -225,9 +247,9 @@ def test_microsatellite_heavy_hitters(counter, k):
     assert counts.max() > 100_000
     f, r, s = run_engine([seq], k, heavy_keys=50_000, counter=counter)
     assert s.heavy_buckets > 0
-    check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+    check_outputs(f, r, s, k, keys, counts, exact=counter != K.KC_COUNTER_BLOOM)
     f, r, s = run_engine([seq], k, counter=counter)   # default threshold: one workgroup per bucket
-    check_outputs(f, r, s, k, keys, counts, exact=counter == K.KC_COUNTER_EXACT)
+    check_outputs(f, r, s, k, keys, counts, exact=counter != K.KC_COUNTER_BLOOM)
 
 
 @COUNTERS
@@ -386,7 +408,8 @@ def test_autotune_keeps_the_tables_exact():
         e.finish()
         r = e.autotune()
         assert r.candidates >= 4
-        assert r.counter in (K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT)
+        assert r.counter in (K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT, K.KC_COUNTER_SORT)
+        assert r.candidates >= 12  # three counters, two fan-outs, two heavy thresholds
         assert r.best_ms > 0
         f, rr, s = e.firsts(), e.records(), e.stats()
         check_outputs(f, rr, s, 31, keys, counts)
