@@ -44,6 +44,14 @@ KC_API int kmc_stage0_sample_file(const char* path, uint32_t k, uint64_t block_s
 KC_API uint64_t kmc_packed_bytes(uint32_t k);
 KC_API void kmc_pack_low(const uint64_t* keys, uint64_t n, uint32_t k, uint8_t* out);
 
+/* One file in the low.bin format (ExportWriter, src/ExportWriter.h:522-550)
+ * from n keys in host memory, and one root file (FinalDrainWriter,
+ * src/FinalDrainWriter.h:31-91) from n ExportRecord<N> structs; both are
+ * written by up to 8 threads, each packing and pwrite-ing its slice at its
+ * offset (the writers kmc_write_outputs uses per emitted chunk). */
+KC_API int kmc_write_low_keys(const char* path, const uint64_t* keys, uint64_t n, uint32_t k);
+KC_API int kmc_write_root_records(const char* path, const void* records, uint64_t n, uint32_t k);
+
 /* Write tmp_dir/low.bin and tmp_dir/root_{0..255}.bin from finished engine
  * results (ExportWriter + FinalDrainWriter formats; all 256 root files are
  * created, empty ones included, as the reference does). */

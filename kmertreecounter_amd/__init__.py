@@ -126,7 +126,8 @@ EXPORTED = [
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
     "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident", "kmc_stage0_sample_file",
-    "kc_debug_fault", "kc_device_memory", "kc_count_noalloc", "kc_noalloc_max_bytes",
+    "kc_debug_fault", "kc_device_memory", "kc_count_noalloc", "kc_noalloc_max_bytes", "kmc_write_low_keys",
+    "kmc_write_root_records",
 ]
 
 _lib = None

@@ -427,6 +427,32 @@ int kmc_write_outputs(kc_engine* h, const char* tmp_dir)
     return KC_OK;
 }
 
+int kmc_write_low_keys(const char* path, const uint64_t* keys, uint64_t n, uint32_t k)
+{
+    if (!path || (!keys && n) || k == 0 || k > 128) return KC_ERR_ARG;
+    LowWriter lw;
+    lw.k = k;
+    lw.nw = k <= 32 ? 1 : (k <= 64 ? 2 : 4);
+    lw.pb = kmc_packed_bytes(k);
+    lw.fd = ::open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (lw.fd < 0) return KC_ERR_ARG;
+    const int rc = n ? low_cb(&lw, keys, n) : 0;
+    ::close(lw.fd);
+    return rc ? KC_ERR_ARG : KC_OK;
+}
+
+int kmc_write_root_records(const char* path, const void* records, uint64_t n, uint32_t k)
+{
+    if (!path || (!records && n) || k == 0 || k > 128) return KC_ERR_ARG;
+    RootWriter rw;
+    rw.rb = 8ULL * (k <= 32 ? 1 : (k <= 64 ? 2 : 4)) + 8;
+    rw.fd = ::open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (rw.fd < 0) return KC_ERR_ARG;
+    const int rc = n ? root_cb(&rw, records, n) : 0;
+    ::close(rw.fd);
+    return rc ? KC_ERR_ARG : KC_OK;
+}
+
 int kmc_write_histogram(const char* path, const int64_t* hist, uint32_t min_freq, uint32_t max_freq)
 {
     // `f\tn\n` for f = min_freq .. max_freq (tool/histogram_tool.cpp:425-442)

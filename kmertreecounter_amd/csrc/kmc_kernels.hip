@@ -20,6 +20,8 @@
 // per XCD and each XCD fills its own sub-range of a bucket (partial cache lines
 // merge in that XCD's L2 instead of crossing XCDs).
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 #include <stdint.h>
 
 #include <algorithm>
@@ -319,7 +321,7 @@ template <int NW, bool SCATTER>
 __global__ void __launch_bounds__(ExtractCfg<NW>::kThreads)
 k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Route rt,
           unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
-          uint64_t* __restrict__ out)
+          uint64_t* __restrict__ out, uint64_t tile0)
 {
     constexpr int T = ExtractCfg<NW>::kThreads;
     constexpr int H = (NW == 1) ? 2 : (NW == 2 ? 4 : 8);  // words before a thread's own for a k-base window
@@ -343,7 +345,8 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
 
     const int t = threadIdx.x;
-    const uint64_t base = (uint64_t)blockIdx.x * TILE;
+    // tile0: first tile of this launch (long inputs are launched in pieces)
+    const uint64_t base = (tile0 + blockIdx.x) * TILE;
     const uint32_t xcd = blockIdx.x % kXcds;
 
     // 16 bytes per thread, aligned; the buffer is padded with '\n' to a whole tile
@@ -585,7 +588,7 @@ __global__ void __launch_bounds__(WIDE ? SubWideCfg<NW>::kThreads : SubCfg<NW>::
 k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coarse_off,
       const unsigned long long* __restrict__ tile_start, Route rt,
       unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
-      uint64_t* __restrict__ B)
+      uint64_t* __restrict__ B, uint64_t tile0)
 {
     constexpr int T = WIDE ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads;
     constexpr int TILE = WIDE ? SubWideCfg<NW>::kTile : SubCfg<NW>::kTile;
@@ -609,7 +612,7 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     __shared__ BktT s_bkt[SCATTER ? TILE : 1];
 
     const int t = threadIdx.x;
-    const unsigned long long tid = blockIdx.x;
+    const unsigned long long tid = tile0 + blockIdx.x;  // tile0: first tile of this launch
     const unsigned long long ntiles = tile_start[256];
     if (tid >= ntiles) return;
     // coarse bucket of this tile: last r with tile_start[r] <= tid
@@ -1909,18 +1912,35 @@ k_ovf_scatter(const uint8_t* __restrict__ ovf, const unsigned long long* __restr
 
 #define KMC_LAUNCH_CHECK() do { hipError_t e_ = hipGetLastError(); if (e_ != hipSuccess) return e_; } while (0)
 
+// Workgroups of T threads per launch of a tile-per-workgroup kernel: a grid
+// dimension must stay below 2^32 threads, so inputs of 64 GiB and more of
+// sequence are launched in pieces (2^31 threads each).  KMC_MAX_GRID_TILES
+// lowers the piece size (tests of the piece logic).
+uint64_t launch_piece_tiles(uint32_t T)
+{
+    uint64_t piece = (1ULL << 31) / T;
+    if (const char* v = std::getenv("KMC_MAX_GRID_TILES")) {
+        const unsigned long long n = std::strtoull(v, nullptr, 10);
+        if (n > 0 && n < piece) piece = n;
+    }
+    return piece;
+}
+
 template <int NW>
 hipError_t launch_extract(bool scatter, const uint8_t* seq, uint64_t padded_len, uint32_t k, const Route& rt,
                           unsigned long long* counts, unsigned long long* cursor, uint64_t* out, hipStream_t st)
 {
     constexpr int T = ExtractCfg<NW>::kThreads;
     const uint64_t tiles = padded_len / (T * 16);
-    if (tiles == 0) return hipSuccess;
-    if (scatter)
-        hipLaunchKernelGGL((k_extract<NW, true>), dim3((uint32_t)tiles), dim3(T), 0, st, seq, padded_len, k, rt, counts, cursor, out);
-    else
-        hipLaunchKernelGGL((k_extract<NW, false>), dim3((uint32_t)tiles), dim3(T), 0, st, seq, padded_len, k, rt, counts, cursor, out);
-    KMC_LAUNCH_CHECK();
+    const uint64_t piece = launch_piece_tiles(T);
+    for (uint64_t t0 = 0; t0 < tiles; t0 += piece) {
+        const dim3 g((uint32_t)std::min<uint64_t>(piece, tiles - t0));
+        if (scatter)
+            hipLaunchKernelGGL((k_extract<NW, true>), g, dim3(T), 0, st, seq, padded_len, k, rt, counts, cursor, out, t0);
+        else
+            hipLaunchKernelGGL((k_extract<NW, false>), g, dim3(T), 0, st, seq, padded_len, k, rt, counts, cursor, out, t0);
+        KMC_LAUNCH_CHECK();
+    }
     return hipSuccess;
 }
 
@@ -1949,19 +1969,21 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
                       const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,
                       unsigned long long* cursor, uint64_t* B, hipStream_t st)
 {
-    if (max_tiles == 0) return hipSuccess;
     if (rt.sbits > kMaxSubBits) return hipErrorInvalidValue;
-    const dim3 g((uint32_t)max_tiles);
-    if (rt.sbits > 8) {
-        const dim3 b(SubWideCfg<NW>::kThreads);
-        if (scatter) hipLaunchKernelGGL((k_sub<NW, true, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
-        else hipLaunchKernelGGL((k_sub<NW, false, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
-    } else {
-        const dim3 b(SubCfg<NW>::kThreads);
-        if (scatter) hipLaunchKernelGGL((k_sub<NW, true, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
-        else hipLaunchKernelGGL((k_sub<NW, false, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B);
+    const uint32_t T = rt.sbits > 8 ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads;
+    const uint64_t piece = launch_piece_tiles(T);
+    for (uint64_t t0 = 0; t0 < max_tiles; t0 += piece) {
+        const dim3 g((uint32_t)std::min<uint64_t>(piece, max_tiles - t0));
+        const dim3 b(T);
+        if (rt.sbits > 8) {
+            if (scatter) hipLaunchKernelGGL((k_sub<NW, true, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
+            else hipLaunchKernelGGL((k_sub<NW, false, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
+        } else {
+            if (scatter) hipLaunchKernelGGL((k_sub<NW, true, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
+            else hipLaunchKernelGGL((k_sub<NW, false, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
+        }
+        KMC_LAUNCH_CHECK();
     }
-    KMC_LAUNCH_CHECK();
     return hipSuccess;
 }
 

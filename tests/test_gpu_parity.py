@@ -400,6 +400,25 @@ def test_rounds_reject_a_root_bucket_larger_than_a_round():
             e.finish()
 
 
+@pytest.mark.parametrize("partition", [K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED], ids=["prefix", "learned"])
+@pytest.mark.parametrize("sub_bits", [8, 9])
+def test_kernels_launched_in_pieces(monkeypatch, partition, sub_bits):
+    # inputs of 64 GiB and more are launched in pieces of at most 2^31 threads;
+    # KMC_MAX_GRID_TILES makes the pieces 13 tiles here (not a multiple of the
+    # 8 XCDs), so the count and scatter passes of extraction and
+    # sub-partition each run in hundreds of launches
+    seq = K.generate(2, scale=400, seed=23, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    monkeypatch.setenv("KMC_MAX_GRID_TILES", "13")
+    with K.Engine(31, partition=partition, sub_bits=sub_bits, learned_min_skew_pct=1) as e:
+        if partition == K.KC_PARTITION_LEARNED:
+            e.sample_block(seq[: len(seq) // 10])
+            e.plan()
+        e.push_block(seq)
+        e.finish()
+        check_outputs(e.firsts(), e.records(), e.stats(), 31, keys, counts)
+
+
 def test_autotune_keeps_the_tables_exact():
     seq = K.generate(2, scale=400, seed=15, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
@@ -409,7 +428,7 @@ def test_autotune_keeps_the_tables_exact():
         r = e.autotune()
         assert r.candidates >= 4
         assert r.counter in (K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT, K.KC_COUNTER_SORT)
-        assert r.candidates >= 12  # three counters, two fan-outs, two heavy thresholds
+        assert r.candidates >= 6  # three counters, two heavy thresholds (and a second fan-out on larger inputs)
         assert r.best_ms > 0
         f, rr, s = e.firsts(), e.records(), e.stats()
         check_outputs(f, rr, s, 31, keys, counts)
