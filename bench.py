@@ -76,7 +76,8 @@ def algorithmic_bytes(st, seq_bytes: int) -> dict:
         "sub_count": sub * Kk * W,                      # read the keys
         "sub_scatter": sub * 2 * Kk * W,                # read + write the keys
         "count": Kk * W + F * W + R * RB,               # read the keys, write firsts and records
-        "group_records": 3 * R * RB,                    # count read, scatter read + write
+        "heavy": 0,                                     # chunks of heavy buckets: their keys are in "count"
+        "group_records": 2 * F * W + 3 * R * RB,        # firsts read + write; records: root read, read + write
     }
 
 
@@ -84,17 +85,22 @@ def phase_ms(st) -> dict:
     return {
         "extract_count": st.ms_extract_count, "extract_scatter": st.ms_extract_scatter,
         "sub_count": st.ms_sub_count, "sub_scatter": st.ms_sub_scatter,
-        "count": st.ms_count, "group_records": st.ms_compact,
+        "count": st.ms_count, "heavy": st.ms_heavy, "group_records": st.ms_compact,
     }
 
 
-PHASES = ("extract_count", "extract_scatter", "sub_count", "sub_scatter", "count", "group_records")
+PHASES = ("extract_count", "extract_scatter", "sub_count", "sub_scatter", "count", "heavy", "group_records")
 
 
-def kernel_names(nw: int) -> dict:
+COUNT_KERNEL = {"bloom": "k_count", "exact": "k_count_exact", "sort": "k_count_sort"}
+
+
+def kernel_names(nw: int, counter: str = "bloom") -> dict:
     return {
         "extract_count": f"k_extract<{nw}, false>", "extract_scatter": f"k_extract<{nw}, true>",
-        "sub_count": f"k_sub<{nw}, false>", "sub_scatter": f"k_sub<{nw}, true>", "count": f"k_count<{nw}>",
+        "sub_count": f"k_sub<{nw}, false>", "sub_scatter": f"k_sub<{nw}, true>",
+        "count": f"{COUNT_KERNEL[counter]}<{nw}>",
+        "heavy": f"k_heavy<{nw}>",
         "group_records": f"k_spill_emit<{nw}> + k_seg_scan + k_compact_firsts<{nw}> + k_compact_recs<{nw}> + k_root_scan",
     }
 
@@ -245,18 +251,30 @@ def closed_form_kmers(seq: bytes, k: int, chunk: int = 1 << 28) -> int:
 
 
 def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, counter: str, graph: bool,
-                  autotune: bool = False, closed_k: int | None = None) -> dict:
-    """Resident run of one partition method: plan (learned only), copy, warm up, time `steps` passes."""
+                  autotune: bool = False, closed_k: int | None = None, bounds_io: dict | None = None) -> dict:
+    """Resident run of one partition method: plan (learned only), copy, warm up, time `steps` passes.
+    bounds_io: {"load": path} routes by saved learned boundaries instead of training
+    (for profilers that cannot follow the training), {"save": path} keeps them."""
+    import numpy as np
     part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
     cnt = {"bloom": K.KC_COUNTER_BLOOM, "exact": K.KC_COUNTER_EXACT, "sort": K.KC_COUNTER_SORT}[counter]
     eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt, graph=graph)
     plan_ms = 0.0
-    if part == K.KC_PARTITION_LEARNED:
+    bounds_io = bounds_io or {}
+    bounds = None
+    if part == K.KC_PARTITION_LEARNED and bounds_io.get("load"):
+        bounds = np.fromfile(bounds_io["load"], dtype=np.uint64)
+        eng.set_partition_bounds(bounds)
+        log(f"[{partition}] routing by the boundaries in {bounds_io['load']} (no training)")
+    elif part == K.KC_PARTITION_LEARNED:
         # Stage 0 sample: the first 1% of the reads; the network trains on the GPU
         t0 = time.time()
         eng.sample_block(seq[: len(seq) // 100])
         eng.plan()
         plan_ms = (time.time() - t0) * 1e3
+        bounds = eng.partition_bounds()
+        if bounds is not None and bounds_io.get("save"):
+            bounds.tofile(bounds_io["save"])
     t0 = time.time()
     eng.push_block(seq)
     eng.finish()  # H2D copy + first full pass (also the first warmup)
@@ -295,9 +313,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         if tuned:
             cfg_eager.update(sub_bits=tuned["sub_bits"] or K.KC_SUB_BITS_NONE, heavy_keys=tuned["heavy_keys"])
         eng2 = K.Engine(k, **cfg_eager)
-        if part == K.KC_PARTITION_LEARNED:
-            eng2.sample_block(seq[: len(seq) // 100])
-            eng2.plan()
+        if bounds is not None:  # the same partition, without training again
+            eng2.set_partition_bounds(bounds)
         eng2.push_block(seq)
         eng2.finish()
         for _ in range(2):
@@ -311,7 +328,6 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     hist = eng.histogram(1, 10000)
     hist_ms = (time.perf_counter() - t_h) * 1e3
     sh = eng.stats()
-    import numpy as np
     fk = int((np.arange(1, hist.size + 1, dtype=np.int64) * hist).sum())
     check = {"hist_sum_f_n": fk, "tail_kmers": sh.hist_tail_kmers, "tail_keys": sh.hist_tail_keys,
              "device_kmers": st.total_kmers}
@@ -335,6 +351,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     }
     if part == K.KC_PARTITION_LEARNED:
         r["learned"] = {
+            "bounds_from": bounds_io.get("load"),  # set: no training in this process
+            "fallback": st.learned_fallback,
             "plan_ms": plan_ms,
             "train_keys": st.learned_train_keys,
             "split_buckets": st.learned_split_buckets,
@@ -431,7 +449,8 @@ def run_workload(w: int, args, threads: int, primary: bool) -> dict:
     if closed_k is not None:
         log(f"[workload {w}] closed-form K = {closed_k} ({time.time() - t0:.1f} s)")
     parts = ["prefix", "learned"] if args.partition == "both" else [args.partition]
-    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter, graph, autotune, closed_k)
+    bounds_io = {"save": args.save_bounds, "load": args.load_bounds}
+    runs = {p: run_partition(seq, k, p, args.steps, args.warmup, args.counter, graph, autotune, closed_k, bounds_io)
             for p in parts}
     del seq
     main_run = runs["learned"] if "learned" in runs else runs["prefix"]
@@ -468,7 +487,7 @@ def run_workload(w: int, args, threads: int, primary: bool) -> dict:
     if "learned" in runs:
         res["learned"] = runs["learned"]["learned"]
     if primary:
-        names = kernel_names(st.key_words)
+        names = kernel_names(st.key_words, (main_run["autotune"] or {}).get("counter", args.counter))
         dominant = max(per_phase, key=per_phase.get)
         timed = per_phase[dominant] > 0
         achieved = alg[dominant] / (per_phase[dominant] * 1e-3) / 1e9 if timed else None
@@ -508,6 +527,8 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true", help="skip the closed-form K check")
     ap.add_argument("--ingest", action="store_true",
                     help="also time kmc_tree_run end to end from a FASTQ file, host reader vs device parser")
+    ap.add_argument("--save-bounds", help="write the learned partition's 4097 boundaries (uint64) to this file")
+    ap.add_argument("--load-bounds", help="route the learned partition by boundaries from --save-bounds, no training")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="after the run, copy this many GiB with k_copy16 (a plain copy whose "
                          "FETCH_SIZE/WRITE_SIZE readings scale those of the pipeline kernels)")

@@ -116,12 +116,15 @@ typedef struct kc_config {
                                  threshold the run keeps the prefix partition (kc_stats.learned_fallback) */
     uint64_t max_round_keys;  /* keys one pass over the device holds (0 = as many as the free device
                                  memory allows).  An input with more positions is counted in
-                                 bucket-range rounds: contiguous root-bucket ranges, one pass each,
-                                 whose outputs are kept in host memory */
+                                 bucket-range rounds: contiguous root-bucket ranges, one pass each;
+                                 the fewest rounds that fit, with ranges as even as the root buckets
+                                 allow so the key buffers leave room for the rounds' outputs */
     uint32_t graph;           /* 1: capture one pass of the pipeline in a HIP graph and replay it
                                  (one-round runs; the graph is captured again when buffers or
                                  options change); the stats then carry the total time only */
-    uint32_t reserved2;
+    uint32_t round_outputs;   /* bucket-range rounds: 0 = each round's outputs stay in device memory
+                              (one buffer sized from the first round) while they fit, the rest go
+                              to host memory; 1 = all to host memory */
     uint64_t est_file_size;   /* Stage 0: the reference's estimated FASTQ size, input of the Bloom
                                  sizing (FastqPreReader::get_estimated_raw_fastq_file_size: the size
                                  on disk, four times it for gzip; src/main.cpp:88-108); 0 = expected_bytes,
@@ -144,8 +147,8 @@ typedef struct kc_stats {
     double ms_extract_scatter;
     double ms_sub_count;
     double ms_sub_scatter;
-    double ms_count;
-    double ms_compact;
+    double ms_count;             /* the counting kernel; heavy-bucket chunks are ms_heavy */
+    double ms_compact;           /* spill emit and finalize */
     double ms_total;             /* first kernel start to last kernel end */
     double ms_h2d;               /* host -> device sequence copies (pushes) */
     uint64_t stage0_sample_kmers;
@@ -174,6 +177,10 @@ typedef struct kc_stats {
     uint64_t learned_fallback;          /* kc_plan with KC_PARTITION_LEARNED kept the prefix partition: 1 the
                                            sample's prefixes were balanced (learned_min_skew_pct), 2 the sample
                                            had fewer distinct keys than buckets */
+    double ms_heavy;                    /* device time of the heavy-bucket chunks (k_heavy); ms_count is the
+                                           counting kernel alone */
+    uint64_t rounds_in_device;          /* bucket-range rounds whose outputs stayed in device memory */
+    uint64_t round_key_cap;             /* keys the key buffers held in the last run */
 } kc_stats;
 
 typedef struct kc_engine kc_engine;
@@ -213,6 +220,21 @@ typedef int (*kc_keys_cb)(void* user, const uint64_t* keys, uint64_t n);
 KC_API int kc_create(const kc_config* cfg, kc_engine** out);
 KC_API int kc_sample_block(kc_engine* h, const char* data, uint64_t len);
 KC_API int kc_plan(kc_engine* h);
+
+/* The learned partition's boundaries, no reference counterpart: 4097
+ * nondecreasing words on key word 0 (bucket j holds the keys whose word 0 w
+ * has b[j] <= w < b[j+1]; b[0] = 0, b[4096] = ~0).
+ * kc_get_partition_bounds copies them into bounds (cap >= 4097) and sets *n to
+ * 4097, or sets *n to 0 when the handle has no learned boundaries (no kc_plan,
+ * a prefix handle, or a plan that kept the prefix partition).
+ * kc_set_partition_bounds makes them the partition of a KC_PARTITION_LEARNED
+ * handle without training, so a partition trained once serves later jobs on
+ * like input; the next kc_plan replaces them.  Any valid boundaries count
+ * exactly; only the bucket balance depends on them.  KC_ERR_ARG for n != 4097,
+ * b[0] != 0, b[4096] != ~0 or a decreasing pair; KC_ERR_STATE on a prefix
+ * handle. */
+KC_API int kc_get_partition_bounds(kc_engine* h, uint64_t* bounds, uint32_t cap, uint32_t* n);
+KC_API int kc_set_partition_bounds(kc_engine* h, const uint64_t* bounds, uint32_t n);
 KC_API int kc_push_block(kc_engine* h, const char* data, uint64_t len);
 KC_API int kc_finish(kc_engine* h);
 KC_API int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user);

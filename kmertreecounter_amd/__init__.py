@@ -30,6 +30,7 @@ KC_SUB_BITS_NONE = 0xFFFFFFFF
 KC_COUNTER_BLOOM = 0
 KC_COUNTER_EXACT = 1
 KC_COUNTER_SORT = 2
+LEARNED_BOUNDS = 4097  # boundaries of a learned partition (4096 buckets)
 KC_HISTOGRAM_PRECISE = 0
 KC_HISTOGRAM_APPROXIMATE = 1
 
@@ -48,7 +49,7 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_uint32), ("classifier_num", ctypes.c_uint32),
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
         ("heavy_keys", ctypes.c_uint32), ("learned_min_skew_pct", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
-        ("graph", ctypes.c_uint32), ("reserved2", ctypes.c_uint32), ("est_file_size", ctypes.c_uint64),
+        ("graph", ctypes.c_uint32), ("round_outputs", ctypes.c_uint32), ("est_file_size", ctypes.c_uint64),
     ]
 
 
@@ -86,7 +87,8 @@ class Stats(ctypes.Structure):
         ("heavy_buckets", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
         ("learned_split_buckets", ctypes.c_uint64), ("bucket4096_max_over_mean", ctypes.c_double),
         ("hist_tail_keys", ctypes.c_uint64), ("hist_tail_kmers", ctypes.c_uint64),
-        ("learned_fallback", ctypes.c_uint64),
+        ("learned_fallback", ctypes.c_uint64), ("ms_heavy", ctypes.c_double),
+        ("rounds_in_device", ctypes.c_uint64), ("round_key_cap", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -127,7 +129,7 @@ EXPORTED = [
     "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
     "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident", "kmc_stage0_sample_file",
     "kc_debug_fault", "kc_device_memory", "kc_count_noalloc", "kc_noalloc_max_bytes", "kmc_write_low_keys",
-    "kmc_write_root_records",
+    "kmc_write_root_records", "kc_get_partition_bounds", "kc_set_partition_bounds",
 ]
 
 _lib = None
@@ -150,6 +152,8 @@ def lib() -> ctypes.CDLL:
     L.kc_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
     L.kc_sample_block.argtypes = [vp, cp, u64]
     L.kc_plan.argtypes = [vp]
+    L.kc_get_partition_bounds.argtypes = [vp, ctypes.POINTER(u64), u32, ctypes.POINTER(u32)]
+    L.kc_set_partition_bounds.argtypes = [vp, ctypes.POINTER(u64), u32]
     L.kc_push_block.argtypes = [vp, cp, u64]
     L.kc_push_fastq.argtypes = [vp, cp, u64]
     L.kc_debug_resident.argtypes = [vp, vp, u64, ctypes.POINTER(u64)]
@@ -275,12 +279,13 @@ class Engine:
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
                  pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0,
                  max_round_keys: int = 0, graph: bool = False, est_file_size: int = 0,
-                 learned_min_skew_pct: int = 0):
+                 learned_min_skew_pct: int = 0, round_outputs_host: bool = False):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
-                     counter, table_slots, heavy_keys, learned_min_skew_pct, max_round_keys, 1 if graph else 0, 0,
+                     counter, table_slots, heavy_keys, learned_min_skew_pct, max_round_keys, 1 if graph else 0,
+                     1 if round_outputs_host else 0,
                      est_file_size)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
@@ -315,6 +320,23 @@ class Engine:
 
     def plan(self) -> None:
         self._check(self._L.kc_plan(self._h), "kc_plan")
+
+    def partition_bounds(self):
+        """The learned partition's 4097 boundaries (numpy uint64), or None when
+        the handle routes by prefix."""
+        import numpy as np
+        out = np.zeros(LEARNED_BOUNDS, dtype=np.uint64)
+        n = ctypes.c_uint32()
+        self._check(self._L.kc_get_partition_bounds(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                    LEARNED_BOUNDS, ctypes.byref(n)), "kc_get_partition_bounds")
+        return out if n.value else None
+
+    def set_partition_bounds(self, bounds) -> None:
+        """Route by these learned boundaries without training (kc_set_partition_bounds)."""
+        import numpy as np
+        b = np.ascontiguousarray(bounds, dtype=np.uint64)
+        self._check(self._L.kc_set_partition_bounds(self._h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                    b.shape[0]), "kc_set_partition_bounds")
 
     def push_block(self, data: bytes) -> None:
         self._check(self._L.kc_push_block(self._h, data, len(data)), "kc_push_block")

@@ -77,7 +77,8 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     }
     if (sbits) PIPE_CHECK(fill(b.fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
 
-    // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]
+    // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]; the
+    // heavy-bucket chunks run between ev[9] and ev[11]
     PIPE_CHECK(mark(0));
     PIPE_CHECK(launch_extract<NW>(false, b.seq, b.padded, o.k, o.rt, counts, cursor, nullptr, st));
     PIPE_CHECK(mark(1));
@@ -129,8 +130,9 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     a.heavy_buckets = b.ctr + kCtrHeavyBuckets;
     PIPE_CHECK(mark(8));
     PIPE_CHECK(launch_count<NW>(nf, o.counter, src, fine_off, a, st));
-    PIPE_CHECK(launch_heavy<NW>(o.heavy_workgroups, src, a, st));
     PIPE_CHECK(mark(9));
+    PIPE_CHECK(launch_heavy<NW>(o.heavy_workgroups, src, a, st));
+    PIPE_CHECK(mark(11));
     PIPE_CHECK(launch_spill_emit<NW>(a, st));
     SegArgs sa{};
     sa.nf = nf;
@@ -213,6 +215,11 @@ struct kc_engine {
     uint64_t* d_A = nullptr;
     uint64_t* d_B = nullptr;
     uint8_t* d_R = nullptr;
+    // bucket-range rounds: device buffer the rounds' outputs are copied into
+    // while they fit (sized from the first round), kept across runs
+    uint8_t* d_arena = nullptr;
+    uint64_t arena_cap = 0;
+    uint64_t rounds_in_device = 0;  // rounds of the last run whose outputs are in the arena
     unsigned long long* d_small = nullptr;  // coarse counts/cursors/offsets/tile starts, root offsets
     unsigned long long* d_fcounts = nullptr;
     unsigned long long* d_fcursor = nullptr;
@@ -289,9 +296,13 @@ struct kc_engine {
     struct RoundOut {
         HostBuf firsts;  // n_first x NW words
         HostBuf recs;    // n_rec records
+        // outputs kept in the device arena (null: in the host buffers)
+        const uint64_t* d_firsts = nullptr;
+        const uint8_t* d_recs = nullptr;
         uint64_t n_first = 0, n_rec = 0;
         std::vector<unsigned long long> root_off;  // 2 x 257, as root_off
         const uint64_t* first_words() const { return reinterpret_cast<const uint64_t*>(firsts.p); }
+        bool on_device() const { return d_firsts != nullptr || d_recs != nullptr; }
     };
     std::vector<RoundOut> rounds;  // the first rounds_used hold the last run's outputs
     uint32_t rounds_used = 0;
@@ -305,7 +316,7 @@ struct kc_engine {
     const uint64_t* res_firsts = nullptr;
     const uint8_t* res_recs = nullptr;
     std::vector<unsigned long long> root_off;
-    double phase_ms[7] = {};
+    double phase_ms[8] = {};  // P0, P1, P2a, P2b, P3, P4, total, heavy buckets
     char* h_bounce = nullptr;
     char* h_bounce2 = nullptr;             // second emit buffer: the next chunk copies while the callback runs
     hipEvent_t emit_ev[2] = {nullptr, nullptr};
@@ -464,12 +475,21 @@ int append_bytes(kc_engine* h, const char* data, uint64_t len)
     return KC_OK;
 }
 
+void free_arena(kc_engine* h)
+{
+    dfree(h->d_arena);
+    h->arena_cap = 0;
+    h->rounds_used = 0;  // round outputs kept there are gone
+    h->have_results = false;
+}
+
 void free_work(kc_engine* h)
 {
     dfree(h->d_A);
     dfree(h->d_B);
     dfree(h->d_R);
     h->key_cap = 0;
+    free_arena(h);
 }
 
 void free_fine(kc_engine* h)
@@ -542,6 +562,8 @@ int copy_to_host(kc_engine* h, kc_engine::HostBuf& hb, const void* src, uint64_t
     return KC_OK;
 }
 
+int ensure_small(kc_engine* h);
+
 int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
 {
     const uint64_t kb = 8ULL * h->nw;
@@ -560,16 +582,7 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
     // Every buffer group is allocated whole or not at all: a failed call leaves
     // null pointers and zero capacities, so the next call allocates again.
     auto nomem = [&](const char* what) { return h->fail(KC_ERR_NOMEM, std::string("device allocation for ") + what + " failed"); };
-    if (!h->d_small || !h->d_ctr) {
-        dfree(h->d_small);
-        dfree(h->d_ctr);
-        if (dmalloc(&h->d_small, sizeof(unsigned long long) * kSmTotal) != hipSuccess ||
-            dmalloc(&h->d_ctr, sizeof(unsigned long long) * kCtrCount) != hipSuccess) {
-            dfree(h->d_small);
-            dfree(h->d_ctr);
-            return nomem("the bucket counters");
-        }
-    }
+    if (int rc = ensure_small(h)) return rc;
     if (h->nf_alloc < nf) {
         free_fine(h);
         if (dmalloc(&h->d_fcounts, sizeof(unsigned long long) * kXcds * nf) != hipSuccess ||
@@ -638,7 +651,7 @@ uint64_t round_capacity(const kc_engine* h, uint64_t padded)
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
         const uint64_t per_key = 3ULL * 8 * NW;              // A, B and R
-        const uint64_t held = h->key_cap * per_key;          // buffers this engine already holds
+        const uint64_t held = h->key_cap * per_key + h->arena_cap;  // buffers this engine already holds
         const uint64_t reserve = 8ULL << 30;                 // tables, spill, histogram, outputs in flight
         const uint64_t avail = free_b + held > reserve ? free_b + held - reserve : 0;
         cap = std::min<uint64_t>(cap, avail / per_key);
@@ -666,6 +679,67 @@ GraphKey graph_key(const kc_engine* h, const Route& rt, uint32_t sbits, uint32_t
 template <int NW>
 int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded);
 
+// Contiguous root-bucket ranges of at most `cap` keys each (counts: 256 root
+// bucket sizes); returns the number of ranges, 0 if one bucket exceeds cap.
+uint32_t split_ranges(const uint64_t* counts, uint64_t cap, std::vector<std::pair<uint32_t, uint32_t>>* out)
+{
+    if (out) out->clear();
+    uint64_t acc = 0;
+    uint32_t start = 0, n = 0;
+    for (uint32_t b = 0; b < 256; ++b) {
+        if (counts[b] > cap) return 0;
+        if (acc + counts[b] > cap) {
+            if (out) out->emplace_back(start, b);
+            ++n;
+            start = b;
+            acc = 0;
+        }
+        acc += counts[b];
+    }
+    if (out) out->emplace_back(start, 256u);
+    return n + 1;
+}
+
+// Copies a round's outputs from HBM into the device arena if they fit there
+// (set out.d_*), else into the round's host buffers.
+template <int NW>
+int keep_round_outputs(kc_engine* h, kc_engine::RoundOut& out, uint64_t& arena_used)
+{
+    const uint64_t fb = h->n_first * 8ULL * NW, rb = h->n_rec * rec_bytes(NW);
+    const uint64_t fb_al = (fb + 255) / 256 * 256, rb_al = (rb + 255) / 256 * 256;
+    out.d_firsts = nullptr;
+    out.d_recs = nullptr;
+    if (!h->cfg.round_outputs && h->d_arena && arena_used + fb_al + rb_al <= h->arena_cap) {
+        uint8_t* f = h->d_arena + arena_used;
+        uint8_t* r = f + fb_al;
+        // stream order keeps the next round's kernels behind these copies
+        if (fb) HIPX(dcopy_async(f, h->res_firsts, fb, hipMemcpyDeviceToDevice, h->st));
+        if (rb) HIPX(dcopy_async(r, h->res_recs, rb, hipMemcpyDeviceToDevice, h->st));
+        out.d_firsts = reinterpret_cast<const uint64_t*>(f);
+        out.d_recs = r;
+        arena_used += fb_al + rb_al;
+        return KC_OK;
+    }
+    if (!out.firsts.ensure(fb) || !out.recs.ensure(rb)) return h->fail(KC_ERR_NOMEM, "host memory for a round's outputs");
+    int rc = copy_to_host(h, out.firsts, h->res_firsts, fb);
+    if (!rc) rc = copy_to_host(h, out.recs, h->res_recs, rb);
+    return rc;
+}
+
+int ensure_small(kc_engine* h)
+{
+    if (h->d_small && h->d_ctr) return KC_OK;
+    dfree(h->d_small);
+    dfree(h->d_ctr);
+    if (dmalloc(&h->d_small, sizeof(unsigned long long) * kSmTotal) != hipSuccess ||
+        dmalloc(&h->d_ctr, sizeof(unsigned long long) * kCtrCount) != hipSuccess) {
+        dfree(h->d_small);
+        dfree(h->d_ctr);
+        return h->fail(KC_ERR_NOMEM, "device allocation for the bucket counters failed");
+    }
+    return KC_OK;
+}
+
 template <int NW>
 int run_pipeline_t(kc_engine* h)
 {
@@ -675,15 +749,20 @@ int run_pipeline_t(kc_engine* h)
     const uint32_t sbits = choose_sbits(h, padded);
     const uint32_t nf = 256u << sbits;
     const uint64_t cap = round_capacity<NW>(h, padded > 0 ? padded : tile);
-    uint64_t want = std::min<uint64_t>(cap, padded > 0 ? padded : tile);
-    // when rounds are needed anyway, buffers within 10% of what fits are kept:
-    // reallocating hundreds of GB for a slightly larger round costs seconds
-    if (want < padded && h->key_cap < want && h->key_cap * 10 >= want * 9) want = h->key_cap;
-    int rc = ensure_work(h, want, nf);
-    if (rc) return rc;
+    const bool one_round = padded <= cap;
+    if (one_round) {
+        if (h->d_arena) free_arena(h);  // round outputs of an earlier, larger input
+        const int rc = ensure_work(h, padded > 0 ? padded : tile, nf);
+        if (rc) return rc;
+    } else {
+        // the key buffers are sized after the root histogram, below
+        const int rc = ensure_small(h);
+        if (rc) return rc;
+    }
     h->occ4096.clear();
     h->rounds_used = 0;
     h->n_rounds = 1;
+    h->rounds_in_device = 0;
 
     const bool learned = h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
     if (learned && h->bounds_sbits != sbits) {
@@ -707,12 +786,15 @@ int run_pipeline_t(kc_engine* h)
     rt.lut = h->d_lut;
     rt.rlo = 0;
     rt.rhi = 256;
-    if (padded <= h->key_cap) return run_round_t<NW>(h, rt, sbits, nf, padded);
+    if (one_round) return run_round_t<NW>(h, rt, sbits, nf, padded);
 
-    // Bucket-range rounds: more positions than one round holds.  The coarse
-    // histogram decides contiguous bucket ranges of at most key_cap keys; each
-    // round extracts the whole sequence again but keeps its own buckets, and
-    // its outputs move to host memory (key ranges of rounds are disjoint).
+    // Bucket-range rounds: more positions than one round holds.  The root
+    // histogram splits the 256 root buckets into the fewest contiguous ranges
+    // of at most `cap` keys, evened out to the smallest capacity that still
+    // needs no more ranges; the key buffers are sized for that, which leaves
+    // device memory for the rounds' outputs.  Each round extracts the whole
+    // sequence again but keeps its own buckets (key ranges of rounds are
+    // disjoint, so outputs, stats and histograms combine).
     unsigned long long* counts = h->d_small + kSmCounts;
     if (padded > h->seq_len) HIPX(hipMemsetAsync(h->d_seq + h->seq_len, '\n', padded - h->seq_len, st));
     HIPX(hipMemsetAsync(h->d_small, 0, sizeof(unsigned long long) * kSmTotal, st));
@@ -720,23 +802,28 @@ int run_pipeline_t(kc_engine* h)
     std::vector<unsigned long long> hc((size_t)kXcds * 256);
     HIPX(dcopy_async(hc.data(), counts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
-    std::vector<std::pair<uint32_t, uint32_t>> ranges;
-    uint64_t acc = 0;
-    uint32_t start = 0;
+    uint64_t root[256];
     for (uint32_t b = 0; b < 256; ++b) {
-        uint64_t c = 0;
-        for (int x = 0; x < kXcds; ++x) c += hc[(size_t)x * 256 + b];
-        if (c > h->key_cap) return h->fail(KC_ERR_CAPACITY, "one root bucket holds more keys than a round");
-        if (acc + c > h->key_cap) {
-            ranges.emplace_back(start, b);
-            start = b;
-            acc = 0;
-        }
-        acc += c;
+        root[b] = 0;
+        for (int x = 0; x < kXcds; ++x) root[b] += hc[(size_t)x * 256 + b];
     }
-    ranges.emplace_back(start, 256u);
+    const uint32_t nr = split_ranges(root, cap, nullptr);
+    if (nr == 0) return h->fail(KC_ERR_CAPACITY, "one root bucket holds more keys than a round");
+    uint64_t lo = *std::max_element(root, root + 256), hi = cap;
+    while (lo < hi) {  // smallest capacity with at most nr ranges
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (split_ranges(root, mid, nullptr) <= nr) hi = mid;
+        else lo = mid + 1;
+    }
+    // buffers already held are kept when they are large enough
+    uint64_t need = lo;
+    if (h->key_cap >= need && h->key_cap <= cap) need = h->key_cap;
+    std::vector<std::pair<uint32_t, uint32_t>> ranges;
+    split_ranges(root, need, &ranges);
+    int rc = ensure_work(h, need, nf);
+    if (rc) return rc;
     uint64_t K = 0, nfirst = 0, nrec = 0, spilled = 0, maxb = 0, dist = 0, xp = 0, heavy = 0;
-    float phase[7] = {};
+    float phase[8] = {};
     // KMC_TIMING=1: host wall time of each round's count and output copy on stderr
     const bool timing = std::getenv("KMC_TIMING") != nullptr;
     auto t_last = std::chrono::steady_clock::now();
@@ -748,23 +835,43 @@ int run_pipeline_t(kc_engine* h)
         t_last = now;
     };
     lap("ranges", 0);
+    uint64_t arena_used = 0;
     for (const auto& r : ranges) {
         rt.rlo = r.first;
         rt.rhi = r.second;
         rc = run_round_t<NW>(h, rt, sbits, nf, padded);
         if (rc) return rc;
         lap("count", h->rounds_used);
+        if (h->rounds_used == 0 && !h->cfg.round_outputs) {
+            // the arena is sized from the first round's outputs (rounds of even
+            // key counts have similar outputs), within the free device memory
+            // less a reserve for tables that may grow
+            const uint64_t first = h->n_first * 8ULL * NW + h->n_rec * rec_bytes(NW) + 512;
+            const uint64_t want = (uint64_t)((double)first * (double)ranges.size() * 1.15) + (64ULL << 20);
+            size_t free_b = 0, total_b = 0;
+            if (h->arena_cap < want && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+                const uint64_t reserve = 8ULL << 30;
+                const uint64_t avail = free_b + h->arena_cap > reserve ? free_b + h->arena_cap - reserve : 0;
+                uint64_t size = std::min(want, avail) / 256 * 256;
+                // KMC_MAX_ARENA_BYTES: a smaller arena (tests of rounds split between HBM and host memory)
+                if (const char* m = std::getenv("KMC_MAX_ARENA_BYTES")) size = std::min<uint64_t>(size, std::strtoull(m, nullptr, 10));
+                if (size > h->arena_cap) {
+                    dfree(h->d_arena);
+                    h->arena_cap = 0;
+                    if (size >= (1ULL << 20) && dmalloc(&h->d_arena, size) == hipSuccess) h->arena_cap = size;
+                    else (void)hipGetLastError();  // no arena: the outputs go to host memory
+                }
+            }
+        }
         if (h->rounds.size() <= h->rounds_used) h->rounds.emplace_back();
         kc_engine::RoundOut& out = h->rounds[h->rounds_used];
         out.n_first = h->n_first;
         out.n_rec = h->n_rec;
-        if (!out.firsts.ensure(h->n_first * 8ULL * NW) || !out.recs.ensure(h->n_rec * rec_bytes(NW)))
-            return h->fail(KC_ERR_NOMEM, "host memory for a round's outputs");
         out.root_off = h->root_off;
-        rc = copy_to_host(h, out.firsts, h->res_firsts, h->n_first * 8ULL * NW);
-        if (!rc) rc = copy_to_host(h, out.recs, h->res_recs, h->n_rec * rec_bytes(NW));
+        rc = keep_round_outputs<NW>(h, out, arena_used);
         if (rc) return rc;
-        lap("copy", h->rounds_used);
+        lap(out.on_device() ? "kept in HBM" : "copy", h->rounds_used);
+        h->rounds_in_device += out.on_device() ? 1 : 0;
         ++h->rounds_used;
         K += h->K;
         nfirst += h->n_first;
@@ -774,8 +881,9 @@ int run_pipeline_t(kc_engine* h)
         dist += h->n_distinct_rep;
         xp += h->extra_passes;
         heavy += h->heavy_buckets;
-        for (int p = 0; p < 7; ++p) phase[p] += h->phase_ms[p];
+        for (int p = 0; p < 8; ++p) phase[p] += h->phase_ms[p];
     }
+    HIPX(hipStreamSynchronize(st));  // the last device-to-device copies
     h->K = K;
     h->n_first = nfirst;
     h->n_rec = nrec;
@@ -784,7 +892,7 @@ int run_pipeline_t(kc_engine* h)
     h->n_distinct_rep = dist;
     h->extra_passes = xp;
     h->heavy_buckets = heavy;
-    for (int p = 0; p < 7; ++p) h->phase_ms[p] = phase[p];
+    for (int p = 0; p < 8; ++p) h->phase_ms[p] = phase[p];
     h->res_firsts = nullptr;
     h->res_recs = nullptr;
     h->n_rounds = (uint32_t)ranges.size();
@@ -922,20 +1030,18 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         h->res_recs = pipe_records<NW>(pb, sbits);
         h->root_off.assign(2 * 257, 0);
         HIPX(dcopy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
-        // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total
-        // (a graph replay records the total only)
-        const int pairs[6][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {9, 10}};
+        // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total;
+        // heavy-bucket chunks (a graph replay records the total only)
+        const int pairs[8][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {8, 9}, {11, 10}, {0, 10}, {9, 11}};
         float ms = 0;
-        for (int p = 0; p < 6; ++p) {
-            if (use_graph) {
+        for (int p = 0; p < 8; ++p) {
+            if (use_graph && p != 6) {
                 h->phase_ms[p] = 0.f;
                 continue;
             }
             HIPX(hipEventElapsedTime(&ms, h->ev[pairs[p][0]], h->ev[pairs[p][1]]));
             h->phase_ms[p] = ms;
         }
-        HIPX(hipEventElapsedTime(&ms, h->ev[0], h->ev[10]));
-        h->phase_ms[6] = ms;
         h->have_results = true;
         return KC_OK;
     }
@@ -1133,6 +1239,31 @@ int kc_plan(kc_engine* h)
     return KC_OK;
 }
 
+int kc_get_partition_bounds(kc_engine* h, uint64_t* bounds, uint32_t cap, uint32_t* n)
+{
+    if (!h || !n) return KC_ERR_ARG;
+    const std::vector<uint64_t>& b = h->plan.learned_b;
+    const bool have = h->cfg.partition == KC_PARTITION_LEARNED && b.size() == kLearnedBounds;
+    if (have && (!bounds || cap < kLearnedBounds)) return KC_ERR_ARG;
+    *n = have ? kLearnedBounds : 0u;
+    if (have) std::memcpy(bounds, b.data(), sizeof(uint64_t) * kLearnedBounds);
+    return KC_OK;
+}
+
+int kc_set_partition_bounds(kc_engine* h, const uint64_t* bounds, uint32_t n)
+{
+    if (!h || !bounds || n != kLearnedBounds) return KC_ERR_ARG;
+    if (h->cfg.partition != KC_PARTITION_LEARNED)
+        return h->fail(KC_ERR_STATE, "partition bounds need a KC_PARTITION_LEARNED handle");
+    if (bounds[0] != 0 || bounds[n - 1] != ~0ULL) return h->fail(KC_ERR_ARG, "bounds must run from 0 to ~0");
+    for (uint32_t i = 1; i < n; ++i)
+        if (bounds[i] < bounds[i - 1]) return h->fail(KC_ERR_ARG, "bounds must be nondecreasing");
+    h->plan.learned_b.assign(bounds, bounds + n);
+    h->plan.fallback = 0;
+    h->bounds_sbits = ~0u;  // routing tables are rebuilt at the next run
+    return KC_OK;
+}
+
 int kc_push_block(kc_engine* h, const char* data, uint64_t len)
 {
     if (!h || (!data && len)) return KC_ERR_ARG;
@@ -1291,10 +1422,18 @@ int kc_emit_firsts(kc_engine* h, kc_keys_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t kb = 8ULL * h->nw;
     const uint64_t per = kEmitChunk / kb;
-    if (h->rounds_used) {  // bucket-range rounds: outputs in host memory
+    if (h->rounds_used) {  // bucket-range rounds: outputs in the device arena or host memory
         for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
             const auto& r = h->rounds[ri];
             const uint64_t n = r.n_first;
+            if (r.on_device()) {
+                const int rc = emit_chunks(h, reinterpret_cast<const uint8_t*>(r.d_firsts), n, kb,
+                                           [&](const void* p, uint64_t m) {
+                                               return cb(user, static_cast<const uint64_t*>(p), m);
+                                           });
+                if (rc) return rc;
+                continue;
+            }
             for (uint64_t i = 0; i < n; i += per) {
                 int rc = cb(user, r.first_words() + i * h->nw, std::min(per, n - i));
                 if (rc) return h->fail(rc, "keys callback returned an error");
@@ -1314,11 +1453,17 @@ int kc_emit_root(kc_engine* h, uint32_t root_id, kc_records_cb cb, void* user)
     HIPX(hipSetDevice(h->device));
     const uint64_t rb = rec_bytes(h->nw);
     const uint64_t per = kEmitChunk / rb;
-    if (h->rounds_used) {  // bucket-range rounds: outputs in host memory
+    if (h->rounds_used) {  // bucket-range rounds: outputs in the device arena or host memory
         for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
             const auto& r = h->rounds[ri];
             for (int part = 0; part < 2; ++part) {
                 const uint64_t b = r.root_off[part * 257 + root_id], e = r.root_off[part * 257 + root_id + 1];
+                if (r.on_device()) {
+                    const int rc = emit_chunks(h, r.d_recs + b * rb, e - b, rb,
+                                               [&](const void* p, uint64_t m) { return cb(user, p, m); });
+                    if (rc) return rc;
+                    continue;
+                }
                 for (uint64_t i = b; i < e; i += per) {
                     int rc = cb(user, r.recs.p + i * rb, std::min(per, e - i));
                     if (rc) return h->fail(rc, "records callback returned an error");
@@ -1436,9 +1581,16 @@ int kc_histogram(kc_engine* h, uint32_t mode, uint32_t min_freq, uint32_t max_fr
         return histogram_pass(h, precise, min_freq, max_freq, h->res_recs, h->n_rec, h->res_firsts, h->n_first, h->d_R,
                               buf_bytes, hist);
     // bucket-range rounds: the key ranges are disjoint, so the rounds' histograms
-    // add up; each round's outputs go back into R and A, and B holds the table
+    // add up; a round kept in the device arena is read there with R as the
+    // table, a round in host memory goes back into R and A with B as the table
     for (uint32_t ri = 0; ri < h->rounds_used; ++ri) {
         const auto& r = h->rounds[ri];
+        if (r.on_device()) {
+            const int rc = histogram_pass(h, precise, min_freq, max_freq, r.d_recs, r.n_rec, r.d_firsts, r.n_first,
+                                          h->d_R, buf_bytes, hist);
+            if (rc) return rc;
+            continue;
+        }
         if (r.n_rec) HIPX(dcopy(h->d_R, r.recs.p, r.n_rec * rec_bytes(h->nw), hipMemcpyHostToDevice));
         if (r.n_first) HIPX(dcopy(h->d_A, r.firsts.p, r.n_first * 8ULL * h->nw, hipMemcpyHostToDevice));
         const int rc = histogram_pass(h, precise, min_freq, max_freq, reinterpret_cast<const uint8_t*>(h->d_R), r.n_rec,
@@ -1479,6 +1631,9 @@ int kc_stats_get(kc_engine* h, kc_stats* s)
     s->ms_count = h->phase_ms[4];
     s->ms_compact = h->phase_ms[5];
     s->ms_total = h->phase_ms[6];
+    s->ms_heavy = h->phase_ms[7];
+    s->rounds_in_device = h->rounds_in_device;
+    s->round_key_cap = h->key_cap;
     s->ms_h2d = h->h2d_ms;
     if (h->planned) {
         s->learned_train_keys = h->plan.train_keys;

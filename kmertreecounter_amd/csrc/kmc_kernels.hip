@@ -66,6 +66,23 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* scratch, T* total)
     return r;
 }
 
+// Workgroup copy of n units, four loads in flight per thread before their
+// stores (one load then its store per step waits out the load latency).
+template <typename U>
+__device__ __forceinline__ void copy_units(const U* __restrict__ src, U* __restrict__ dst, uint64_t n)
+{
+    const uint64_t T = blockDim.x;
+    uint64_t i = threadIdx.x;
+    for (; i + 3 * T < n; i += 4 * T) {
+        const U a = src[i], b = src[i + T], c = src[i + 2 * T], d = src[i + 3 * T];
+        dst[i] = a;
+        dst[i + T] = b;
+        dst[i + 2 * T] = c;
+        dst[i + 3 * T] = d;
+    }
+    for (; i < n; i += T) dst[i] = src[i];
+}
+
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 {
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -1670,46 +1687,50 @@ k_heavy(const uint64_t* __restrict__ B, CountArgs args)
     }
 }
 
+// Each workgroup owns a contiguous range of slots.  A first sweep counts the
+// records each thread keeps; one scan and one global reservation per workgroup
+// place them; a second sweep writes them.  (Reserving once per 256 slots put
+// tens of thousands of atomics on the one overflow counter: 0.79 ms for an
+// 8Mi-slot table, most of it waiting on that address.)
 template <int NW>
 __global__ void __launch_bounds__(256)
 k_spill_emit(CountArgs args)
 {
-    __shared__ uint32_t s_n;
-    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_scan[256 / kWave + 1];
     __shared__ uint32_t s_dist;
+    __shared__ unsigned long long s_base;
     if (*args.spilled == 0) return;
-    for (uint64_t base = (uint64_t)blockIdx.x * 256; base < args.spill_cap; base += (uint64_t)gridDim.x * 256) {
-        if (threadIdx.x == 0) { s_n = 0; s_dist = 0; }
-        __syncthreads();
-        const uint64_t s = base + threadIdx.x;
-        uint32_t cnt = 0;
-        Key<NW> key;
-        bool keep = false;
-        if (s < args.spill_cap) {
-            cnt = args.spill_cnt[s];
-            if (cnt) {
-                constexpr int TW = TableWords<NW>::value;
-                Key<TW> tk;
+    const uint64_t per = (args.spill_cap + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * per;
+    const uint64_t hi = lo + per < args.spill_cap ? lo + per : args.spill_cap;
+    if (threadIdx.x == 0) s_dist = 0;
+    uint32_t mine = 0, dist = 0;
+    for (uint64_t s = lo + threadIdx.x; s < hi; s += 256) {
+        const uint32_t cnt = args.spill_cnt[s];
+        dist += cnt != 0;
+        mine += cnt != 0 && cnt >= args.min_count && cnt <= args.max_count;
+    }
+    uint32_t total = 0;
+    const uint32_t off = block_exclusive_scan(mine, s_scan, &total);  // its barriers order the s_dist clear
+    if (dist) atomicAdd(&s_dist, dist);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_base = total ? atomicAdd(args.ovf_count, (unsigned long long)total) : 0ULL;
+        if (s_dist) atomicAdd(args.spill_distinct, (unsigned long long)s_dist);
+    }
+    __syncthreads();
+    if (!mine) return;
+    uint64_t pos = s_base + off;
+    constexpr int TW = TableWords<NW>::value;
+    for (uint64_t s = lo + threadIdx.x; s < hi; s += 256) {
+        const uint32_t cnt = args.spill_cnt[s];
+        if (!(cnt != 0 && cnt >= args.min_count && cnt <= args.max_count)) continue;
+        Key<TW> tk;
 #pragma unroll
-                for (int w = 0; w < TW; ++w) tk.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
-                key = from_table<NW>(tk);
-                keep = cnt >= args.min_count && cnt <= args.max_count;
-                atomicAdd(&s_dist, 1u);
-            }
-        }
-        const uint32_t my = keep ? atomicAdd(&s_n, 1u) : 0u;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            s_base = s_n ? atomicAdd(args.ovf_count, (unsigned long long)s_n) : 0ULL;
-            if (s_dist) atomicAdd(args.spill_distinct, (unsigned long long)s_dist);
-        }
-        __syncthreads();
-        if (keep) {
-            const uint64_t pos = s_base + my;
-            if (pos < args.ovf_cap) write_record<NW>(args.ovf, pos, key, cnt);
-            else atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
-        }
-        __syncthreads();
+        for (int w = 0; w < TW; ++w) tk.w[w] = args.spill_keys[(uint64_t)w * args.spill_cap + s];
+        if (pos < args.ovf_cap) write_record<NW>(args.ovf, pos, from_table<NW>(tk), cnt);
+        else atomicOr(args.error_flags, (unsigned long long)kErrRecFull);
+        ++pos;
     }
 }
 
@@ -1800,7 +1821,11 @@ k_compact_firsts(SegArgs a, const uint64_t* __restrict__ firsts_in, uint64_t* __
     const unsigned long long lo = a.fine_off[f];
     const uint32_t nfst = a.nfirst[f];
     const unsigned long long fdst = a.foff[f];
-    for (uint32_t j = threadIdx.x; j < nfst; j += 256) store_key<NW>(firsts_out, fdst + j, load_key<NW>(firsts_in, lo + j));
+    if constexpr (NW == 1)
+        copy_units(firsts_in + lo, firsts_out + fdst, nfst);
+    else  // keys of two or four words: 16-byte units
+        copy_units(reinterpret_cast<const ulonglong2*>(firsts_in) + lo * (NW / 2),
+                   reinterpret_cast<ulonglong2*>(firsts_out) + fdst * (NW / 2), (uint64_t)nfst * (NW / 2));
 }
 
 template <int NW>
@@ -1831,7 +1856,7 @@ k_compact_recs(SegArgs a, const uint8_t* __restrict__ rec_in, uint8_t* __restric
     __syncthreads();
     const uint32_t rmin = s_min, rmax = s_max;
     if (rmin == rmax) {
-        for (uint32_t w = t; w < nr * RW; w += 256) dst[w] = src[w];
+        copy_units(src, dst, (uint64_t)nr * RW);
         if (t == 0) atomicAdd(&a.root_main[rmin], (unsigned long long)nr);
         return;
     }

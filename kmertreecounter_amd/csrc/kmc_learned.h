@@ -15,6 +15,7 @@ namespace kmc {
 constexpr uint32_t kMlpIn = 64;
 constexpr uint32_t kMlpH = 64;
 constexpr uint32_t kMlpC = 4096;
+constexpr uint32_t kLearnedBounds = kMlpC + 1;  // bucket boundaries of a learned partition
 constexpr uint32_t kMlpOffB1 = kMlpH * kMlpIn;
 constexpr uint32_t kMlpOffW2 = kMlpOffB1 + kMlpH;
 constexpr uint32_t kMlpOffB2 = kMlpOffW2 + kMlpH * kMlpH;

@@ -91,7 +91,9 @@ inline uint8_t* pipe_records(const PipeBufs& b, uint32_t sbits)
 const char* noalloc_last_error();
 
 // Enqueues one pass on `st`.  graph: being captured (clears are fill kernels,
-// no events); otherwise ev[0..9] bracket the phases (ev may be null).
+// no events); otherwise ev[0..9] and ev[11] bracket the phases: P0-P2 kernels
+// ev[2p]..ev[2p+1], the counting kernel ev[8]..ev[9], the heavy-bucket chunks
+// ev[9]..ev[11], then the spill emit and finalize (ev may be null).
 template <int NW>
 hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st, bool graph, hipEvent_t* ev);
 

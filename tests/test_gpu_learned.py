@@ -159,3 +159,54 @@ def test_learned_plan_keeps_prefixes_when_the_sample_is_balanced():
         np.testing.assert_array_equal(mk, keys)
         np.testing.assert_array_equal(mc, counts)
         e.close()
+
+
+def test_saved_partition_bounds_route_a_new_handle_without_training():
+    # a partition trained once (kc_get_partition_bounds) serves another handle
+    # (kc_set_partition_bounds): same buckets, exact tables, no training
+    seq = K.generate(3, scale=100, seed=33, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    e = learned_engine_run(seq, 31, K.KC_PARTITION_LEARNED, sample_frac=0.05)
+    b = e.partition_bounds()
+    s1 = e.stats()
+    e.close()
+    assert b is not None and b.shape == (K.LEARNED_BOUNDS,)
+    assert b[0] == 0 and b[-1] == np.uint64(2**64 - 1) and (np.diff(b.astype(object)) >= 0).all()
+    with K.Engine(31, partition=K.KC_PARTITION_LEARNED, expected_bytes=len(seq)) as e2:
+        assert e2.partition_bounds() is None
+        e2.set_partition_bounds(b)
+        np.testing.assert_array_equal(e2.partition_bounds(), b)
+        e2.push_block(seq)
+        e2.finish()
+        s2 = e2.stats()
+        mk, mc = C.merge_outputs(e2.firsts(), e2.records(), 1)
+        np.testing.assert_array_equal(mk, keys)
+        np.testing.assert_array_equal(mc, counts)
+        assert s2.learned_train_keys == 0
+        assert s2.max_bucket_keys == s1.max_bucket_keys and s2.fine_buckets == s1.fine_buckets
+        # a second set replaces the partition: one bound moved keeps the tables exact
+        b2 = b.copy()
+        b2[2048] = b2[2047]
+        e2.set_partition_bounds(b2)
+        e2.count_resident()
+        mk, mc = C.merge_outputs(e2.firsts(), e2.records(), 1)
+        np.testing.assert_array_equal(mk, keys)
+        np.testing.assert_array_equal(mc, counts)
+
+
+def test_partition_bounds_are_checked():
+    KC_ERR_ARG, KC_ERR_STATE = -1, -4
+    b = (np.arange(K.LEARNED_BOUNDS, dtype=np.uint64) << np.uint64(52))
+    b[-1] = np.uint64(2**64 - 1)
+    with K.Engine(31, partition=K.KC_PARTITION_LEARNED) as e:
+        for bad in (b[:-1], np.concatenate([b[:1] + 1, b[1:]]), np.concatenate([b[:-1], b[-2:-1] - 1]),
+                    np.concatenate([b[:10], b[9:10] - 1, b[11:]])):
+            with pytest.raises(K.KmcError) as x:
+                e.set_partition_bounds(bad)
+            assert x.value.status == KC_ERR_ARG
+        e.set_partition_bounds(b)
+    with K.Engine(31) as e:
+        assert e.partition_bounds() is None
+        with pytest.raises(K.KmcError) as x:
+            e.set_partition_bounds(b)
+        assert x.value.status == KC_ERR_STATE

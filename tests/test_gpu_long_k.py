@@ -88,6 +88,9 @@ def test_long_k_spill_passes_and_heavy_chunks(k, counter):
     keys, counts = C.count_table(C.extract(seq, k))
     f, r, s = run_engine([seq], k, sub_bits=K.KC_SUB_BITS_NONE, pass_keys=3000, heavy_keys=20_000, counter=counter)
     assert s.heavy_buckets > 0
+    # the heavy chunks are timed apart from the counting kernel
+    assert s.ms_heavy > 0 and s.ms_count > 0
+    assert s.ms_count + s.ms_heavy + s.ms_compact <= s.ms_total
     check_outputs(f, r, s, k, keys, counts, exact=counter != K.KC_COUNTER_BLOOM)
     f, r, s = run_engine([seq], k, sub_bits=K.KC_SUB_BITS_NONE, pass_keys=1 << 30, counter=K.KC_COUNTER_BLOOM)
     assert s.spilled_keys > 0

@@ -360,16 +360,22 @@ def test_root_files_hold_their_prefix():
                 assert (recs["key"][:, 0] >> np.uint64(56) == rid).all()
 
 
+@pytest.mark.parametrize("outputs", ["device", "host", "split"])
 @pytest.mark.parametrize("k", [31, 63])
 @pytest.mark.parametrize("partition", [K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED], ids=["prefix", "learned"])
-def test_bucket_range_rounds(k, partition):
+def test_bucket_range_rounds(monkeypatch, k, partition, outputs):
     # a key capacity below the input's positions: the engine counts contiguous
     # root-bucket ranges in separate rounds and keeps each round's outputs in
-    # host memory; tables, root files and histograms must not change
+    # device memory (device), host memory (host), or device memory until a
+    # small arena is full (split); tables, root files and histograms must not change
     seq = K.generate(1, scale=10, seed=14, fmt="seq")
     nw = C.nwords(k)
     keys, counts = C.count_table(C.extract(seq, k))
-    with K.Engine(k, partition=partition, max_round_keys=1_500_000, learned_min_skew_pct=1) as e:
+    if outputs == "split":  # room for about half of the outputs
+        out_bytes = len(keys) * 8 * nw + int((counts > 1).sum()) * (8 * nw + 8)
+        monkeypatch.setenv("KMC_MAX_ARENA_BYTES", str(out_bytes // 2))
+    with K.Engine(k, partition=partition, max_round_keys=1_500_000, learned_min_skew_pct=1,
+                  round_outputs_host=outputs == "host") as e:
         if partition == K.KC_PARTITION_LEARNED:
             e.sample_block(seq[: len(seq) // 20])
             e.plan()
@@ -377,6 +383,15 @@ def test_bucket_range_rounds(k, partition):
         e.finish()
         f, r, s = e.firsts(), e.records(), e.stats()
         assert s.rounds >= max(2, -(-s.total_kmers // 1_500_000))
+        # the fewest rounds, evened out: the key buffers hold the largest range only
+        assert s.round_key_cap <= 1_500_000
+        assert s.rounds * s.round_key_cap >= s.total_kmers
+        if outputs == "device":
+            assert s.rounds_in_device == s.rounds
+        elif outputs == "host":
+            assert s.rounds_in_device == 0
+        else:
+            assert 0 < s.rounds_in_device < s.rounds
         check_outputs(f, r, s, k, keys, counts)
         for rid in (0, 17, 128, 255):
             rr = e.root(rid)
