@@ -128,6 +128,7 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     a.heavy_count = b.ctr + kCtrHeavyChunks;
     a.heavy_next = b.ctr + kCtrHeavyNext;
     a.heavy_buckets = b.ctr + kCtrHeavyBuckets;
+    a.order = d_ndist + nf;  // filled by the count launch: largest buckets first
     PIPE_CHECK(mark(8));
     PIPE_CHECK(launch_count<NW>(nf, o.counter, src, fine_off, a, st));
     PIPE_CHECK(mark(9));
@@ -226,7 +227,7 @@ struct kc_engine {
     unsigned long long* d_fine_off = nullptr;
     unsigned long long* d_foff = nullptr;
     unsigned long long* d_roff = nullptr;
-    uint32_t* d_bucket_u32 = nullptr;       // nfirst, nrec, ndist per fine bucket
+    uint32_t* d_bucket_u32 = nullptr;       // nfirst, nrec, ndist per fine bucket; counting order
     uint32_t nf_alloc = 0;
     uint8_t* d_ovf = nullptr;               // overflow records
     uint64_t ovf_cap = 0;
@@ -590,7 +591,7 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
             dmalloc(&h->d_fine_off, sizeof(unsigned long long) * (nf + 1)) != hipSuccess ||
             dmalloc(&h->d_foff, sizeof(unsigned long long) * (nf + 1)) != hipSuccess ||
             dmalloc(&h->d_roff, sizeof(unsigned long long) * (nf + 1)) != hipSuccess ||
-            dmalloc(&h->d_bucket_u32, sizeof(uint32_t) * 3 * nf) != hipSuccess) {
+            dmalloc(&h->d_bucket_u32, sizeof(uint32_t) * 4 * nf) != hipSuccess) {
             free_fine(h);
             return nomem("the fine-bucket arrays");
         }
@@ -855,7 +856,10 @@ int run_pipeline_t(kc_engine* h)
                 uint64_t size = std::min(want, avail) / 256 * 256;
                 // KMC_MAX_ARENA_BYTES: a smaller arena (tests of rounds split between HBM and host memory)
                 if (const char* m = std::getenv("KMC_MAX_ARENA_BYTES")) size = std::min<uint64_t>(size, std::strtoull(m, nullptr, 10));
-                if (size > h->arena_cap) {
+                // a larger arena when it holds every round, else only for a real
+                // gain: reallocating tens of GB costs seconds (measured 2.2-2.5 s
+                // for about 70 GB, every pass while the free memory moved a little)
+                if (size > h->arena_cap && (size >= want || size > h->arena_cap + h->arena_cap / 8)) {
                     dfree(h->d_arena);
                     h->arena_cap = 0;
                     if (size >= (1ULL << 20) && dmalloc(&h->d_arena, size) == hipSuccess) h->arena_cap = size;

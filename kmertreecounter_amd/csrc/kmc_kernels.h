@@ -119,6 +119,7 @@ struct CountArgs {
     unsigned long long* heavy_count;    // chunks listed
     unsigned long long* heavy_next;     // chunks taken by k_heavy
     unsigned long long* heavy_buckets;  // buckets handed over
+    const uint32_t* order;              // [nf] bucket of each workgroup, largest first (null: blockIdx)
 };
 
 struct SegArgs {

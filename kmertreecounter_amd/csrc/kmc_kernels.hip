@@ -967,7 +967,7 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
 
     const int t = threadIdx.x;
     const uint32_t lane = t & (kWave - 1);
-    const uint32_t f = blockIdx.x;
+    const uint32_t f = args.order ? args.order[blockIdx.x] : blockIdx.x;
     const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
     const unsigned long long n = hi - lo;
     const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
@@ -1201,7 +1201,7 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
 
     const int t = threadIdx.x;
     const uint32_t lane = t & (kWave - 1);
-    const uint32_t f = blockIdx.x;
+    const uint32_t f = args.order ? args.order[blockIdx.x] : blockIdx.x;
     const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
     const unsigned long long n = hi - lo;
     const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
@@ -1376,7 +1376,7 @@ k_count_sort(const uint64_t* __restrict__ B, const unsigned long long* __restric
 
     const int t = threadIdx.x;
     const uint32_t lane = t & (kWave - 1), wave = t / kWave;
-    const uint32_t f = blockIdx.x;
+    const uint32_t f = args.order ? args.order[blockIdx.x] : blockIdx.x;
     const unsigned long long lo = fine_off[f], hi = fine_off[f + 1];
     const unsigned long long n = hi - lo;
     const unsigned long long rec_cap = (n * 8ULL * NW) / RecBytes<NW>::value;
@@ -2012,10 +2012,75 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
     return hipSuccess;
 }
 
+// Workgroup order of the counting kernels: buckets by size class, largest
+// first, so the largest buckets start first and the small ones fill the end
+// of the grid instead of a large bucket starting last and running alone (a
+// workgroup owns one CU for its whole bucket).  Classes are quarter powers of
+// two; the order within a class is arbitrary and changes no result.  One
+// workgroup; a wave ranks its lanes of one class with one LDS atomic.
+__device__ __forceinline__ uint32_t size_class(unsigned long long n)
+{
+    if (n == 0) return 255;  // empty buckets last
+    const uint32_t top = 63 - __clzll(n);
+    const uint32_t frac = top >= 2 ? (uint32_t)(n >> (top - 2)) & 3u : (uint32_t)(n << (2 - top)) & 3u;
+    return 254 - min(254u, 4 * top + frac);
+}
+
+__global__ void __launch_bounds__(1024)
+k_bucket_order(const unsigned long long* __restrict__ fine_off, uint32_t nf, uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t s_cnt[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & (kWave - 1);
+    if (t < 256) s_cnt[t] = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nf; base += 1024) {
+        const uint32_t f = base + t;
+        const uint32_t c = f < nf ? size_class(fine_off[f + 1] - fine_off[f]) : 256u;
+        uint64_t pending = __ballot(f < nf);
+        while (pending) {
+            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+            const uint32_t lc = __shfl(c, lead, kWave);
+            const uint64_t same = __ballot(c == lc) & pending;
+            if (lane == lead) atomicAdd(&s_cnt[lc], (uint32_t)__popcll(same));
+            pending &= ~same;
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint32_t v = s_cnt[i];
+            s_cnt[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < nf; base += 1024) {
+        const uint32_t f = base + t;
+        const uint32_t c = f < nf ? size_class(fine_off[f + 1] - fine_off[f]) : 256u;
+        uint64_t pending = __ballot(f < nf);
+        while (pending) {
+            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+            const uint32_t lc = __shfl(c, lead, kWave);
+            const uint64_t same = __ballot(c == lc) & pending;
+            uint32_t at = 0;
+            if (lane == lead) at = atomicAdd(&s_cnt[lc], (uint32_t)__popcll(same));
+            at = __shfl(at, lead, kWave);
+            if ((same >> lane) & 1ULL) order[at + lanes_below(same)] = f;
+            pending &= ~same;
+        }
+    }
+}
+
 template <int NW>
 hipError_t launch_count(uint32_t nf, uint32_t counter, const uint64_t* B, const unsigned long long* fine_off,
                         const CountArgs& args, hipStream_t st)
 {
+    if (args.order) {
+        hipLaunchKernelGGL(k_bucket_order, dim3(1), dim3(1024), 0, st, fine_off, nf, const_cast<uint32_t*>(args.order));
+        KMC_LAUNCH_CHECK();
+    }
     if (counter == kCounterSort)
         hipLaunchKernelGGL((k_count_sort<NW>), dim3(nf), dim3(kSortThreads), 0, st, B, fine_off, args);
     else if (counter == kCounterExact)

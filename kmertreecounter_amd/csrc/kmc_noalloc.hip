@@ -71,7 +71,7 @@ Layout layout(uint32_t nw)
     L.ctr = take(sizeof(unsigned long long) * kCtrCount);
     L.foff = take(sizeof(unsigned long long) * (nf + 1));
     L.roff = take(sizeof(unsigned long long) * (nf + 1));
-    L.bucket = take(sizeof(uint32_t) * 3 * nf);
+    L.bucket = take(sizeof(uint32_t) * 4 * nf);
     L.ovf = take(L.ovf_cap * rec_bytes(nw));
     L.spill_keys = take(L.spill_cap * 8ULL * table_words(nw));
     L.spill_cnt = take(L.spill_cap * 4ULL);

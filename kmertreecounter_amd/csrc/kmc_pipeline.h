@@ -47,7 +47,7 @@ struct PipeBufs {
     unsigned long long* fine_off;    // nf + 1 (sbits > 0)
     unsigned long long* foff;        // nf + 1
     unsigned long long* roff;        // nf + 1
-    uint32_t* bucket_u32;            // 3 x nf
+    uint32_t* bucket_u32;            // 4 x nf: nfirst, nrec, ndist per bucket, counting order
     uint8_t* ovf;                    // ovf_cap overflow records
     uint64_t ovf_cap;
     unsigned long long* spill_keys;  // TableWords x spill_cap words
