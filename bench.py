@@ -46,6 +46,7 @@ sys.path.insert(0, str(ROOT))
 import kmertreecounter_amd as K  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+LEARNED_MAX_FINE = 0  # --learned-max-fine: fine buckets of one root, at most (0: the engine's default)
 METRIC = "k-mers counted/s at k=31, learned vs prefix partition; % HBM peak"
 
 WORKLOADS = {
@@ -258,7 +259,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     import numpy as np
     part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
     cnt = {"bloom": K.KC_COUNTER_BLOOM, "exact": K.KC_COUNTER_EXACT, "sort": K.KC_COUNTER_SORT}[counter]
-    eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt, graph=graph)
+    eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt, graph=graph,
+                   learned_max_fine=LEARNED_MAX_FINE)
     plan_ms = 0.0
     bounds_io = bounds_io or {}
     bounds = None
@@ -309,7 +311,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         # for the value) give the phase split of the same configuration
         eng2 = None
         per_phase = {name: 0.0 for name in PHASES}
-        cfg_eager = dict(partition=part, expected_bytes=len(seq), counter=eng_counter(eng, cnt, tuned), graph=False)
+        cfg_eager = dict(partition=part, expected_bytes=len(seq), counter=eng_counter(eng, cnt, tuned), graph=False,
+                         learned_max_fine=LEARNED_MAX_FINE)
         if tuned:
             cfg_eager.update(sub_bits=tuned["sub_bits"] or K.KC_SUB_BITS_NONE, heavy_keys=tuned["heavy_keys"])
         eng2 = K.Engine(k, **cfg_eager)
@@ -523,6 +526,8 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help="replay every workload as a captured HIP graph")
     ap.add_argument("--eager", action="store_true", help="no graph and no autotuner for any workload")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--learned-max-fine", type=int, default=0,
+                    help="learned partition: fine buckets of one root bucket, at most (0: the engine's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the closed-form K check")
     ap.add_argument("--ingest", action="store_true",
@@ -533,6 +538,8 @@ def main() -> int:
                     help="after the run, copy this many GiB with k_copy16 (a plain copy whose "
                          "FETCH_SIZE/WRITE_SIZE readings scale those of the pipeline kernels)")
     args = ap.parse_args()
+    global LEARNED_MAX_FINE
+    LEARNED_MAX_FINE = args.learned_max_fine
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (the k-mer exchange between "

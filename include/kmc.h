@@ -129,6 +129,9 @@ typedef struct kc_config {
                                  sizing (FastqPreReader::get_estimated_raw_fastq_file_size: the size
                                  on disk, four times it for gzip; src/main.cpp:88-108); 0 = expected_bytes,
                                  or the sample size when that is 0 too */
+    uint32_t learned_max_fine; /* KC_PARTITION_LEARNED: fine buckets of one root bucket, at most (0 = 1024,
+                                 the largest).  Each root takes its share of the fine buckets by the
+                                 learned CDF's mass on its key range */
 } kc_config;
 
 typedef struct kc_stats {
@@ -347,6 +350,15 @@ KC_API int kc_debug_mlp_grad(const float* params, const uint64_t* keys, const ui
  * bucket instead of the most likely one. */
 KC_API int kc_debug_learned_fit(const uint64_t* sorted, uint64_t n, uint32_t steps, float lr, uint32_t batch,
                                 uint64_t seed, uint32_t flags, uint64_t* bounds, float* loss_curve, uint32_t loss_cap);
+/* kc_debug_learned_routing: the fine buckets a learned partition routes by
+ * (host only, no device call).  From the 4097 learned boundaries (bounds[0] =
+ * 0, bounds[4096] = UINT64_MAX, nondecreasing): root r (the keys' top byte)
+ * gets fine buckets [fine_base[r], fine_base[r+1]) (257 entries, 1..1024 per
+ * root, about nf_target in all) bounded by fine_bounds[fine_base[r] - r ...]
+ * (fine_base[256] - 256 of them, cap at least that); lut: 256 x 4096 cells,
+ * cell c of root r = number of r's bounds <= (r << 56) + (c << 44). */
+KC_API int kc_debug_learned_routing(const uint64_t* bounds, uint32_t nf_target, uint32_t* fine_base,
+                                    uint64_t* fine_bounds, uint64_t cap, uint16_t* lut);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,7 @@ class Config(ctypes.Structure):
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
         ("heavy_keys", ctypes.c_uint32), ("learned_min_skew_pct", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
         ("graph", ctypes.c_uint32), ("round_outputs", ctypes.c_uint32), ("est_file_size", ctypes.c_uint64),
+        ("learned_max_fine", ctypes.c_uint32),
     ]
 
 
@@ -126,7 +127,7 @@ EXPORTED = [
     "kc_emit_root", "kc_stats_get", "kc_destroy", "kc_count_resident", "kc_words_for_k",
     "kc_last_error", "kcs_generate", "kcs_free", "kmc_fastq_blocks",
     "kmc_fastq_blocks_file", "kmc_packed_bytes", "kmc_pack_low", "kmc_write_outputs", "kmc_tree_run",
-    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_copy", "kc_histogram",
+    "kc_debug_gemm_bf16", "kc_debug_mlp_grad", "kc_debug_learned_fit", "kc_debug_learned_routing", "kc_debug_copy", "kc_histogram",
     "kmc_write_histogram", "kc_autotune", "kc_push_fastq", "kc_debug_resident", "kmc_stage0_sample_file",
     "kc_debug_fault", "kc_device_memory", "kc_count_noalloc", "kc_noalloc_max_bytes", "kmc_write_low_keys",
     "kmc_write_root_records", "kc_get_partition_bounds", "kc_set_partition_bounds",
@@ -180,6 +181,7 @@ def lib() -> ctypes.CDLL:
     L.kc_debug_gemm_bf16.argtypes = [vp, vp, vp, u32, u32, u32]
     L.kc_debug_mlp_grad.argtypes = [vp, vp, vp, u32, vp, vp]
     L.kc_debug_learned_fit.argtypes = [vp, u64, u32, ctypes.c_float, u32, u64, u32, vp, vp, u32]
+    L.kc_debug_learned_routing.argtypes = [vp, u32, vp, vp, u64, vp]
     L.kcs_generate.argtypes = [ctypes.POINTER(SynthSpec), ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(u64),
                                ctypes.POINTER(u64)]
     L.kcs_free.argtypes = [vp]
@@ -279,14 +281,14 @@ class Engine:
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
                  pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0,
                  max_round_keys: int = 0, graph: bool = False, est_file_size: int = 0,
-                 learned_min_skew_pct: int = 0, round_outputs_host: bool = False):
+                 learned_min_skew_pct: int = 0, round_outputs_host: bool = False, learned_max_fine: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
                      counter, table_slots, heavy_keys, learned_min_skew_pct, max_round_keys, 1 if graph else 0,
                      1 if round_outputs_host else 0,
-                     est_file_size)
+                     est_file_size, learned_max_fine)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -513,6 +515,20 @@ def debug_learned_fit(sorted_keys: np.ndarray, steps: int = 8000, lr: float = 1e
     if rc != 0:
         raise KmcError(f"kc_debug_learned_fit failed: {rc}")
     return b, lc
+
+
+def debug_learned_routing(bounds: np.ndarray, nf_target: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Fine buckets of a learned partition from its 4097 boundaries (host only):
+    (fine_base[257], fine_bounds, lut[256, 4096]) as kc_debug_learned_routing returns them."""
+    b = np.ascontiguousarray(bounds, dtype=np.uint64)
+    fb = np.zeros(257, dtype=np.uint32)
+    cap = 256 * 1023
+    bd = np.zeros(cap, dtype=np.uint64)
+    lut = np.zeros((256, 4096), dtype=np.uint16)
+    rc = lib().kc_debug_learned_routing(b.ctypes.data, nf_target, fb.ctypes.data, bd.ctypes.data, cap, lut.ctypes.data)
+    if rc != 0:
+        raise KmcError(f"kc_debug_learned_routing failed: {rc}")
+    return fb, bd[: int(fb[256]) - 256].copy(), lut
 
 
 def write_histogram(path: str, hist: np.ndarray, min_freq: int = 1) -> None:

@@ -200,25 +200,31 @@ __device__ __forceinline__ Key<NW> from_table(const Key<TableWords<NW>::value>& 
 }
 
 // ---------------------------------------------------------------- routing
-// Coarse bucket (0..255) and fine sub-bucket (0..2^sbits-1).
-//   prefix: coarse = top 8 key bits (the reference's root prefix, ROOT_BASES=4,
-//           src/definition.h:28), fine = the next sbits bits.
-//   learned: coarse/fine from monotone boundaries on key word 0 produced by the
-//           learned CDF model (bucket = number of boundaries <= key).
-// learned routing tables: coarse table, 256 fine tables, 256 shifts
+// Root bucket (0..255) and fine bucket.  Roots are the top 8 key bits (the
+// reference's root prefix, ROOT_BASES=4, src/definition.h:28) for both
+// partitions.  Fine buckets:
+//   prefix: the next sbits bits (fine index r << sbits | bits).
+//   learned: root r has nsub_r = fine_base[r+1] - fine_base[r] fine buckets
+//           (1..kMaxFine), bounded by nsub_r - 1 ascending keys taken from the
+//           learned CDF model (fine index fine_base[r] + number of r's bounds
+//           <= key).
+// learned routing tables: per root a table of kLutCells u16 cells over the
+// root's key range, cell c = the root's bounds <= (r << 56) + (c << 44), then
+// the 257 fine bases (u32)
 constexpr uint32_t kLutCells = 4096;
-// fine table entries are u8 up to sbits 8, u16 above (layout in kmc_learned.h)
-constexpr size_t kRouteLutBytes = kLutCells + 256 * (size_t)kLutCells * 2 + 256;
+constexpr uint32_t kLutShift = 44;  // a root spans 2^56 keys: 4096 cells of 2^44
+constexpr uint32_t kMaxFine = 1024;  // fine buckets of one root, at most
+constexpr size_t kRouteLutBytes = 256 * (size_t)kLutCells * 2 + 257 * 4;
+constexpr size_t kRouteFineBaseOffset = 256 * (size_t)kLutCells * 2;
 
 struct Route {
     uint32_t mode;   // 0 prefix, 1 learned
-    uint32_t sbits;  // 0..8
-    const uint64_t* coarse_bounds;  // learned: 255 ascending boundaries
-    const uint64_t* fine_bounds;    // learned: 256 x (2^sbits - 1) ascending boundaries
-    // (kernels stage one coarse bucket's fine bounds in LDS padded to 255 with UINT64_MAX)
-    // learned: 4096-entry coarse table, 256 x 4096 fine tables, 256 shifts
-    const uint8_t* lut;
-    // bucket-range round: the scatter keeps the keys of coarse buckets [rlo, rhi)
+    uint32_t sbits;  // prefix: fine bits 0..kMaxSubBits (learned: unused)
+    uint32_t nf;     // fine buckets in all (prefix: 256 << sbits)
+    const uint32_t* fine_base;    // learned: [257]
+    const uint64_t* fine_bounds;  // learned: root r's bounds at fine_bounds[fine_base[r] - r]
+    const uint16_t* lut;          // learned: 256 x kLutCells cells
+    // bucket-range round: the scatter keeps the keys of root buckets [rlo, rhi)
     uint32_t rlo, rhi;
 };
 

@@ -52,7 +52,8 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
         const hipError_t e_ = (expr);        \
         if (e_ != hipSuccess) return e_;     \
     } while (0)
-    const uint32_t sbits = o.sbits, nf = o.nf;
+    const bool sub = o.sub;
+    const uint32_t nf = o.nf;
     unsigned long long* counts = b.small + kSmCounts;
     unsigned long long* cursor = b.small + kSmCursor;
     unsigned long long* coarse_off = b.small + kSmCoarseOff;
@@ -61,8 +62,8 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     uint32_t* d_nrec = d_nfirst + nf;
     uint32_t* d_ndist = d_nrec + nf;
     // keys grouped by fine bucket (P3 input) and the per-bucket first-occurrence slices
-    uint64_t* src = sbits ? b.B : b.A;
-    uint64_t* slices = sbits ? b.A : b.B;
+    uint64_t* src = sub ? b.B : b.A;
+    uint64_t* slices = sub ? b.A : b.B;
     auto fill = [&](void* p, uint64_t bytes, uint8_t v) -> hipError_t {
         return graph ? launch_fill(p, bytes, v, st) : hipMemsetAsync(p, v, bytes, st);
     };
@@ -75,20 +76,20 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
         PIPE_CHECK(fill(b.spill_keys, b.spill_cap * 8ULL * TableWords<NW>::value, 0xFF));
         PIPE_CHECK(fill(b.spill_cnt, b.spill_cap * 4ULL, 0));
     }
-    if (sbits) PIPE_CHECK(fill(b.fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
+    if (sub) PIPE_CHECK(fill(b.fcounts, sizeof(unsigned long long) * kXcds * nf, 0));
 
     // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]; the
     // heavy-bucket chunks run between ev[9] and ev[11]
     PIPE_CHECK(mark(0));
     PIPE_CHECK(launch_extract<NW>(false, b.seq, b.padded, o.k, o.rt, counts, cursor, nullptr, st));
     PIPE_CHECK(mark(1));
-    PIPE_CHECK(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW, sbits), st, o.rt.rlo, o.rt.rhi));
+    PIPE_CHECK(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st, o.rt.rlo, o.rt.rhi));
     PIPE_CHECK(mark(2));
     PIPE_CHECK(launch_extract<NW>(true, b.seq, b.padded, o.k, o.rt, counts, cursor, b.A, st));
     PIPE_CHECK(mark(3));
     const unsigned long long* fine_off = coarse_off;
-    if (sbits) {
-        const uint64_t max_tiles = b.padded / sub_tile(NW, sbits) + 256;
+    if (sub) {
+        const uint64_t max_tiles = b.padded / sub_tile(NW) + 256;
         PIPE_CHECK(mark(4));
         PIPE_CHECK(launch_sub<NW>(false, max_tiles, b.A, coarse_off, tile_start, o.rt, b.fcounts, nullptr, nullptr, st));
         PIPE_CHECK(mark(5));
@@ -137,7 +138,6 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     PIPE_CHECK(launch_spill_emit<NW>(a, st));
     SegArgs sa{};
     sa.nf = nf;
-    sa.sbits = sbits;
     sa.fine_off = fine_off;
     sa.nfirst = d_nfirst;
     sa.nrec = d_nrec;
@@ -153,7 +153,7 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     sa.rec_out_cap = b.rec_out_cap;
     sa.firsts_out_cap = b.firsts_out_cap;
     // final firsts into the P3 key input (free now), final records into the slices buffer
-    PIPE_CHECK(launch_finalize<NW>(sa, a, slices, pipe_firsts<NW>(b, sbits), b.R, pipe_records<NW>(b, sbits), st));
+    PIPE_CHECK(launch_finalize<NW>(sa, a, slices, pipe_firsts<NW>(b, sub), b.R, pipe_records<NW>(b, sub), st));
     return hipSuccess;
 #undef PIPE_CHECK
 }
@@ -236,9 +236,11 @@ struct kc_engine {
     uint64_t spill_cap = 0;
     bool spill_dirty = true;
     unsigned long long* d_ctr = nullptr;
-    uint64_t* d_bounds = nullptr;  // learned: 255 coarse + 256*(2^sbits-1) fine boundaries
-    uint32_t bounds_sbits = ~0u;   // fan-out the uploaded bounds were built for
-    uint8_t* d_lut = nullptr;      // learned: routing lookup tables (kRouteLutBytes)
+    uint64_t* d_bounds = nullptr;  // learned: each root's fine bounds, up to 256 x (kMaxFine - 1)
+    uint32_t bounds_sbits = ~0u;   // fan-out (256 << bits fine buckets in all) the uploaded tables were built for
+    uint32_t bounds_max_fine = 0;  // and the cap on one root's fine buckets
+    uint32_t learned_nf = 0;       // fine buckets of the uploaded learned tables
+    uint8_t* d_lut = nullptr;      // learned: routing lookup tables and fine bases (kRouteLutBytes)
 
     hipEvent_t ev[12] = {};
 
@@ -624,6 +626,11 @@ int ensure_work(kc_engine* h, uint64_t padded_len, uint32_t nf)
     return KC_OK;
 }
 
+uint32_t learned_max_fine(const kc_engine* h)
+{
+    return h->cfg.learned_max_fine ? std::min<uint32_t>(h->cfg.learned_max_fine, kMaxFine) : kMaxFine;
+}
+
 uint32_t choose_sbits(const kc_engine* h, uint64_t padded_len)
 {
     if (h->cfg.sub_bits == KC_SUB_BITS_NONE) return 0;
@@ -661,13 +668,13 @@ uint64_t round_capacity(const kc_engine* h, uint64_t padded)
 }
 
 template <int NW>
-GraphKey graph_key(const kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded)
+GraphKey graph_key(const kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded)
 {
     GraphKey g{};
     const void* p[18] = {h->d_seq, h->d_A, h->d_B, h->d_R, h->d_small, h->d_ctr, h->d_spill_keys, h->d_spill_cnt,
                          h->d_ovf, h->d_heavy, h->d_fcounts, h->d_fcursor, h->d_fine_off, h->d_foff, h->d_roff,
                          h->d_bucket_u32, h->d_bounds, h->d_lut};
-    const uint64_t v[18] = {padded, h->seq_len, sbits, nf, h->key_cap, h->spill_cap, h->ovf_cap, h->heavy_cap,
+    const uint64_t v[18] = {padded, h->seq_len, (uint64_t)sub | ((uint64_t)rt.sbits << 1), nf, h->key_cap, h->spill_cap, h->ovf_cap, h->heavy_cap,
                             h->cu_count, rt.mode, h->k, h->cfg.min_count, h->cfg.max_count, h->cfg.pass_keys,
                             h->cfg.counter, h->cfg.table_slots, h->cfg.heavy_keys, h->bounds_sbits};
     for (int i = 0; i < 18; ++i) {
@@ -678,7 +685,7 @@ GraphKey graph_key(const kc_engine* h, const Route& rt, uint32_t sbits, uint32_t
 }
 
 template <int NW>
-int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded);
+int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded);
 
 // Contiguous root-bucket ranges of at most `cap` keys each (counts: 256 root
 // bucket sizes); returns the number of ranges, 0 if one bucket exceeds cap.
@@ -748,7 +755,29 @@ int run_pipeline_t(kc_engine* h)
     const uint64_t tile = ext_tile(h->nw);
     const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
     const uint32_t sbits = choose_sbits(h, padded);
-    const uint32_t nf = 256u << sbits;
+    const bool learned = h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
+    if (learned && (h->bounds_sbits != sbits || h->bounds_max_fine != learned_max_fine(h))) {
+        // the roots' fine buckets in proportion to the learned CDF's mass on
+        // their key ranges, 256 << sbits in all (about)
+        LearnedRouting lr;
+        learned_routing(h->plan.learned_b, 256u << sbits, learned_max_fine(h), lr);
+        if (!h->d_bounds && dmalloc(&h->d_bounds, sizeof(uint64_t) * 256 * (kMaxFine - 1)) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the learned bounds failed");
+        if (!lr.bounds.empty())
+            HIPX(dcopy(h->d_bounds, lr.bounds.data(), sizeof(uint64_t) * lr.bounds.size(), hipMemcpyHostToDevice));
+        std::vector<uint8_t> tables(kRouteLutBytes);
+        std::memcpy(tables.data(), lr.lut.data(), sizeof(uint16_t) * lr.lut.size());
+        std::memcpy(tables.data() + kRouteFineBaseOffset, lr.fine_base.data(), sizeof(uint32_t) * 257);
+        if (!h->d_lut && dmalloc(&h->d_lut, kRouteLutBytes) != hipSuccess)
+            return h->fail(KC_ERR_NOMEM, "device allocation for the routing tables failed");
+        HIPX(dcopy(h->d_lut, tables.data(), kRouteLutBytes, hipMemcpyHostToDevice));
+        h->learned_nf = lr.fine_base[256];
+        h->bounds_sbits = sbits;
+        h->bounds_max_fine = learned_max_fine(h);
+    }
+    // the learned partition always splits its roots (P2 runs, whatever the fan-out)
+    const bool sub = learned || sbits > 0;
+    const uint32_t nf = learned ? h->learned_nf : 256u << sbits;
     const uint64_t cap = round_capacity<NW>(h, padded > 0 ? padded : tile);
     const bool one_round = padded <= cap;
     if (one_round) {
@@ -765,29 +794,16 @@ int run_pipeline_t(kc_engine* h)
     h->n_rounds = 1;
     h->rounds_in_device = 0;
 
-    const bool learned = h->cfg.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
-    if (learned && h->bounds_sbits != sbits) {
-        std::vector<uint64_t> rb;
-        routing_bounds(h->plan.learned_b, sbits, rb);
-        if (!h->d_bounds && dmalloc(&h->d_bounds, sizeof(uint64_t) * (255 + 256 * ((1u << kMaxSubBits) - 1))) != hipSuccess)
-            return h->fail(KC_ERR_NOMEM, "device allocation for the learned bounds failed");
-        HIPX(dcopy(h->d_bounds, rb.data(), sizeof(uint64_t) * rb.size(), hipMemcpyHostToDevice));
-        std::vector<uint8_t> lut;
-        routing_luts(rb, sbits, lut);
-        if (!h->d_lut && dmalloc(&h->d_lut, kRouteLutBytes) != hipSuccess)
-            return h->fail(KC_ERR_NOMEM, "device allocation for the routing tables failed");
-        HIPX(dcopy(h->d_lut, lut.data(), kRouteLutBytes, hipMemcpyHostToDevice));
-        h->bounds_sbits = sbits;
-    }
     Route rt{};
     rt.mode = learned ? 1u : 0u;
-    rt.sbits = sbits;
-    rt.coarse_bounds = h->d_bounds;
-    rt.fine_bounds = h->d_bounds ? h->d_bounds + 255 : nullptr;
-    rt.lut = h->d_lut;
+    rt.sbits = learned ? 0u : sbits;
+    rt.nf = nf;
+    rt.fine_base = learned ? reinterpret_cast<const uint32_t*>(h->d_lut + kRouteFineBaseOffset) : nullptr;
+    rt.fine_bounds = learned ? h->d_bounds : nullptr;
+    rt.lut = learned ? reinterpret_cast<const uint16_t*>(h->d_lut) : nullptr;
     rt.rlo = 0;
     rt.rhi = 256;
-    if (one_round) return run_round_t<NW>(h, rt, sbits, nf, padded);
+    if (one_round) return run_round_t<NW>(h, rt, sub, nf, padded);
 
     // Bucket-range rounds: more positions than one round holds.  The root
     // histogram splits the 256 root buckets into the fewest contiguous ranges
@@ -840,7 +856,7 @@ int run_pipeline_t(kc_engine* h)
     for (const auto& r : ranges) {
         rt.rlo = r.first;
         rt.rhi = r.second;
-        rc = run_round_t<NW>(h, rt, sbits, nf, padded);
+        rc = run_round_t<NW>(h, rt, sub, nf, padded);
         if (rc) return rc;
         lap("count", h->rounds_used);
         if (h->rounds_used == 0 && !h->cfg.round_outputs) {
@@ -905,7 +921,7 @@ int run_pipeline_t(kc_engine* h)
 }
 
 template <int NW>
-int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint64_t padded)
+int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded)
 {
     hipStream_t st = h->st;
     unsigned long long* coarse_off = h->d_small + kSmCoarseOff;
@@ -938,7 +954,7 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         pb.firsts_out_cap = h->key_cap;
         PipeOpts po{};
         po.k = h->k;
-        po.sbits = sbits;
+        po.sub = sub;
         po.nf = nf;
         po.rt = rt;
         po.min_count = h->cfg.min_count ? h->cfg.min_count : 1u;
@@ -955,7 +971,7 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         };
         const bool use_graph = h->cfg.graph && rt.rlo == 0 && rt.rhi == 256;
         if (use_graph) {
-            const GraphKey key = graph_key<NW>(h, rt, sbits, nf, padded);
+            const GraphKey key = graph_key<NW>(h, rt, sub, nf, padded);
             if (!h->graph_exec || std::memcmp(&key, &h->graph_key, sizeof(key)) != 0) {
                 if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
                 h->graph_exec = nullptr;
@@ -1024,14 +1040,16 @@ int run_round_t(kc_engine* h, const Route& rt, uint32_t sbits, uint32_t nf, uint
         h->extra_passes = ctr[kCtrExtraPasses];
         h->heavy_buckets = ctr[kCtrHeavyBuckets];
         h->nf_last = nf;
-        if (sbits >= 4) {  // keys per group of 2^(sbits-4) fine buckets: learned buckets or 12-bit prefixes
+        if (sub && nf >= 4096) {
+            // keys per group of consecutive fine buckets (fine bucket f in group
+            // f * 4096 / nf): the 12-bit prefixes, or about the learned buckets
             std::vector<unsigned long long> fo(nf + 1);
             HIPX(dcopy(fo.data(), h->d_fine_off, sizeof(unsigned long long) * (nf + 1), hipMemcpyDeviceToHost));
             if (h->occ4096.size() != 4096) h->occ4096.assign(4096, 0);
-            for (uint32_t f = 0; f < nf; ++f) h->occ4096[f >> (sbits - 4)] += fo[f + 1] - fo[f];
+            for (uint32_t f = 0; f < nf; ++f) h->occ4096[(uint32_t)(((uint64_t)f * 4096) / nf)] += fo[f + 1] - fo[f];
         }
-        h->res_firsts = pipe_firsts<NW>(pb, sbits);
-        h->res_recs = pipe_records<NW>(pb, sbits);
+        h->res_firsts = pipe_firsts<NW>(pb, sub);
+        h->res_recs = pipe_records<NW>(pb, sub);
         h->root_off.assign(2 * 257, 0);
         HIPX(dcopy(h->root_off.data(), h->d_small + kSmRootOff, sizeof(unsigned long long) * 2 * 257, hipMemcpyDeviceToHost));
         // P0, P1, P2a, P2b, P3 kernels; P4 = spill emit + finalize; total;
@@ -1145,6 +1163,7 @@ int kc_create(const kc_config* cfg, kc_engine** out)
     if (cfg->partition > 1 || (cfg->sub_bits > kMaxSubBits && cfg->sub_bits != KC_SUB_BITS_NONE)) return KC_ERR_ARG;
     if (cfg->counter > KC_COUNTER_SORT || (cfg->table_slots && cfg->table_slots < 16)) return KC_ERR_ARG;
     if (cfg->max_count && cfg->min_count > cfg->max_count) return KC_ERR_ARG;
+    if (cfg->learned_max_fine > kMaxFine) return KC_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_ERR_NODEV;
     if ((int)cfg->device >= ndev) return KC_ERR_ARG;
@@ -1720,6 +1739,22 @@ int kc_debug_learned_fit(const uint64_t* sorted, uint64_t n, uint32_t steps, flo
     std::string err;
     const int rc = mlp_debug_fit(sorted, n, o, bounds, loss_curve, loss_curve ? loss_cap : 0, err);
     return rc ? KC_ERR_HIP : KC_OK;
+}
+
+int kc_debug_learned_routing(const uint64_t* bounds, uint32_t nf_target, uint32_t* fine_base, uint64_t* fine_bounds,
+                             uint64_t cap, uint16_t* lut)
+{
+    if (!bounds || !fine_base || !lut || nf_target == 0 || nf_target > 256u * kMaxFine) return KC_ERR_ARG;
+    if (bounds[0] != 0 || bounds[kLearnedBounds - 1] != ~0ULL) return KC_ERR_ARG;
+    for (uint32_t i = 1; i < kLearnedBounds; ++i)
+        if (bounds[i] < bounds[i - 1]) return KC_ERR_ARG;
+    LearnedRouting lr;
+    learned_routing(std::vector<uint64_t>(bounds, bounds + kLearnedBounds), nf_target, kMaxFine, lr);
+    if (lr.bounds.size() > cap || (!fine_bounds && !lr.bounds.empty())) return KC_ERR_CAPACITY;
+    std::memcpy(fine_base, lr.fine_base.data(), sizeof(uint32_t) * 257);
+    if (!lr.bounds.empty()) std::memcpy(fine_bounds, lr.bounds.data(), sizeof(uint64_t) * lr.bounds.size());
+    std::memcpy(lut, lr.lut.data(), sizeof(uint16_t) * lr.lut.size());
+    return KC_OK;
 }
 
 }  // extern "C"

@@ -23,15 +23,14 @@ template <> struct ExtractCfg<2> { static constexpr int kThreads = 256; };
 // 256 threads at least: the bucket histogram and its scan take one thread per bucket
 template <> struct ExtractCfg<4> { static constexpr int kThreads = 256; };
 
-// P2: kTile keys of one coarse bucket per workgroup.
+// P2: kTile keys of one root bucket per tile; the scatter (k_sub) takes one
+// tile per workgroup of kThreads, the count pass (k_sub_count) several per
+// workgroup of kCountThreads.  A root has up to kMaxFine fine buckets.
 template <int NW> struct SubCfg;
-template <> struct SubCfg<1> { static constexpr int kThreads = 512; static constexpr int kTile = 8192; };
-template <> struct SubCfg<2> { static constexpr int kThreads = 256; static constexpr int kTile = 4096; };
-template <> struct SubCfg<4> { static constexpr int kThreads = 256; static constexpr int kTile = 2048; };
-// sbits 9 and 10 (512 or 1024 fine buckets per coarse bucket): u16 bucket
-// indexes and larger histograms, so smaller tiles keep two workgroups per CU
+template <> struct SubCfg<1> { static constexpr int kThreads = 1024; static constexpr int kCountThreads = 512; static constexpr int kTile = 8192; };
+template <> struct SubCfg<2> { static constexpr int kThreads = 1024; static constexpr int kCountThreads = 512; static constexpr int kTile = 4096; };
+template <> struct SubCfg<4> { static constexpr int kThreads = 1024; static constexpr int kCountThreads = 512; static constexpr int kTile = 2048; };
 constexpr uint32_t kMaxSubBits = 10;
-template <int NW> struct SubWideCfg { static constexpr int kThreads = 256; static constexpr int kTile = 4096 / NW; };
 
 // P3: one workgroup per fine bucket (1 per CU).  LDS: Bloom words + count
 // table; buckets above kPassKeys keys are processed in hash-range passes.
@@ -124,7 +123,6 @@ struct CountArgs {
 
 struct SegArgs {
     uint32_t nf;
-    uint32_t sbits;
     const unsigned long long* fine_off;
     const uint32_t* nfirst;
     const uint32_t* nrec;

@@ -5,9 +5,9 @@
 //   S0  k_scan_coarse       bucket offsets, per-XCD write cursors, tile lists
 //   P1  k_extract<scatter>  bytes -> keys, LDS counting sort by coarse bucket,
 //                           coalesced runs into A (bucket-major, XCD-minor)
-//   P2a k_sub<count>        per coarse bucket: fine-bucket histogram
+//   P2a k_sub_count         per root bucket: fine-bucket histogram
 //   S1  k_scan_fine_local, k_scan_fine_add
-//   P2b k_sub<scatter>      A -> B grouped by fine bucket
+//   P2b k_sub               A -> B grouped by fine bucket
 //   P3  k_count             one workgroup per fine bucket: LDS Bloom filter
 //                           (first occurrences -> firsts) and LDS open-
 //                           addressing count table for repeats (hash-range
@@ -122,77 +122,23 @@ __device__ __forceinline__ void store_key(uint64_t* p, uint64_t i, const Key<NW>
     }
 }
 
-// Coarse buckets of N keys.  Prefix routing: the top byte.  Learned routing:
-// the number of the 255 bounds <= w0; the table entry of w0's 12-bit cell
-// counts the bounds at or below the cell's first key, so only bounds inside
-// the cell remain to be stepped over (usually none, rarely two).  Every table
-// read of the N keys is issued before the first result is used (a key's reads
-// depend on each other, different keys' do not).
+// Learned fine buckets of N keys inside root r (keys lo = r << 56 <= w0):
+// the root's table is indexed by (w0 - lo) >> kLutShift, then the bounds
+// inside the cell are stepped over (nb: the root's bounds, s_fb padded with
+// UINT64_MAX up to index nb + 1).  Every table read of the N keys is issued
+// before the first result is used (a key's reads depend on each other,
+// different keys' do not).  Lanes without a key pass lo, whose cell is 0.
 template <int N>
-__device__ __forceinline__ void coarse_of_all(const Route& rt, const uint64_t* s_cb, const uint8_t* s_lut,
-                                              const uint64_t* w0, uint32_t* b)
+__device__ __forceinline__ void fine_of_all(const uint64_t* s_fb, const uint16_t* s_lut, uint64_t lo, uint32_t nb,
+                                            const uint64_t* w0, uint32_t* f)
 {
-    if (rt.mode == 0) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) b[i] = (uint32_t)(w0[i] >> 56);
-        return;
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) b[i] = s_lut[w0[i] >> 52];
-    // the first two bounds at or above the cell's count (a cell rarely holds two)
+    for (int i = 0; i < N; ++i) f[i] = s_lut[(w0[i] - lo) >> kLutShift];
     uint64_t b1[N], b2[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        b1[i] = s_cb[b[i] < 255 ? b[i] : 254];
-        b2[i] = s_cb[b[i] < 254 ? b[i] + 1 : 254];
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        if (b[i] < 255 && b1[i] <= w0[i]) {
-            uint32_t x = b[i] + 1;
-            if (x < 255 && b2[i] <= w0[i]) {
-                ++x;
-                while (x < 255 && s_cb[x] <= w0[i]) ++x;
-            }
-            b[i] = x;
-        }
-    }
-}
-
-// Learned fine bucket inside coarse bucket r (keys lo <= w0): the table of r
-// is indexed by (w0 - lo) >> shift, then bounds inside the cell are stepped
-// over.  Table entries are u8 up to 256 fine buckets, u16 above.
-template <typename E>
-__device__ __forceinline__ uint32_t fine_of(const Route& rt, const uint64_t* s_fb, const E* s_lut, uint64_t lo,
-                                            uint32_t shift, uint64_t w0)
-{
-    if (rt.sbits == 0) return 0;
-    const uint32_t nb = (1u << rt.sbits) - 1u;
-    if (rt.mode == 0) return (uint32_t)((w0 >> (56 - rt.sbits)) & nb);
-    uint32_t f = s_lut[(w0 - lo) >> shift];
-    const bool past = f < nb && s_fb[f < nb ? f : nb - 1] <= w0;
-    if (past) {
-        ++f;
-        while (f < nb && s_fb[f] <= w0) ++f;
-    }
-    return f;
-}
-
-// fine_of for N keys at once (learned routing, sbits > 0), as coarse_of_all:
-// every table read issued before the first result is used.  Lanes without a
-// key pass lo, whose cell is 0.
-template <int N, typename E>
-__device__ __forceinline__ void fine_of_all(const Route& rt, const uint64_t* s_fb, const E* s_lut, uint64_t lo,
-                                            uint32_t shift, const uint64_t* w0, uint32_t* f)
-{
-    const uint32_t nb = (1u << rt.sbits) - 1u;  // bounds, >= 1
-#pragma unroll
-    for (int i = 0; i < N; ++i) f[i] = s_lut[(w0[i] - lo) >> shift];
-    uint64_t b1[N], b2[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        b1[i] = s_fb[f[i] < nb ? f[i] : nb - 1];
-        b2[i] = s_fb[f[i] + 1 < nb ? f[i] + 1 : nb - 1];
+        b1[i] = s_fb[f[i]];
+        b2[i] = s_fb[f[i] + 1];
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -332,8 +278,8 @@ __device__ __forceinline__ bool window_key<4>(const uint32_t* c, const uint32_t*
     return ok;
 }
 
-constexpr int kRouteGroup = 4;
-
+// Root bucket of a key: its top byte, for both partitions (the learned
+// partition splits roots into fine buckets in P2).
 template <int NW, bool SCATTER>
 __global__ void __launch_bounds__(ExtractCfg<NW>::kThreads)
 k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Route rt,
@@ -343,23 +289,18 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     constexpr int T = ExtractCfg<NW>::kThreads;
     constexpr int H = (NW == 1) ? 2 : (NW == 2 ? 4 : 8);  // words before a thread's own for a k-base window
     constexpr int TILE = T * 16;
-    // The encoded tile and the coarse bounds are dead once every thread has
-    // routed its keys (a barrier later), so they share LDS with the staged
-    // keys: this keeps the scatter variant under half of a CU's LDS.
-    constexpr size_t kCb = 2 * (T + H) * sizeof(uint32_t);
-    constexpr size_t kStage = kCb + 256 * sizeof(uint64_t) + kLutCells;
+    // The encoded tile is dead once every thread has its keys (a barrier
+    // later), so it shares LDS with the staged keys.
+    constexpr size_t kStage = 2 * (T + H) * sizeof(uint32_t);
     constexpr size_t kKeys = SCATTER ? TILE * sizeof(Key<NW>) : 0;
     __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
     uint32_t* s_code = reinterpret_cast<uint32_t*>(s_raw);
     uint32_t* s_valid = s_code + (T + H);
-    uint64_t* s_cb = reinterpret_cast<uint64_t*>(s_raw + kCb);
-    uint8_t* s_lut = s_raw + kCb + 256 * sizeof(uint64_t);
     Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_off[256];
     __shared__ unsigned long long s_dst[256];  // global index of the tile's slot 0 for bucket b
     __shared__ uint32_t s_scan[T / kWave + 1];
-    __shared__ uint8_t s_bkt[SCATTER ? TILE : 1];
 
     const int t = threadIdx.x;
     // tile0: first tile of this launch (long inputs are launched in pieces)
@@ -382,11 +323,6 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         s_valid[t] = vd;
     }
     if (t < 256) s_hist[t] = 0;
-    if (rt.mode == 1) {
-        if (t < 255) s_cb[t] = rt.coarse_bounds[t];
-        for (int i = t; i < (int)(kLutCells / 16); i += T)
-            reinterpret_cast<uint4*>(s_lut)[i] = reinterpret_cast<const uint4*>(rt.lut)[i];
-    }
     __syncthreads();
 
     uint32_t c[H + 1], v[H + 1];
@@ -398,80 +334,38 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     (void)padded_len;
     const Windows<NW> win(c, v, k);
 
-    Key<NW> keys[SCATTER ? 16 : 1];
-    uint32_t bk[SCATTER ? 16 : 1];
-    uint32_t rank[SCATTER ? 16 : 1];
-    uint32_t okm = 0;
-    if constexpr (SCATTER) {
-        if (rt.mode == 0) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                Key<NW> key;
-                if (win.key(i, key)) okm |= 1u << i;
-                keys[i] = key;
-                bk[i] = (uint32_t)(key.w[0] >> 56);
-            }
-        } else {
-            // learned routing: keys in two halves of eight, each half's
-            // buckets looked up together (coarse_of_all).  Measured: 11.1 ms
-            // key by key, 10.1 ms this way at 2.33e9 keys; the count pass and
-            // the prefix scatter, with less latency to hide or more waves to
-            // hide it, were faster key by key.
-#pragma unroll
-            for (int part = 0; part < 16 / kRouteGroup; ++part) {
-                uint64_t w0[kRouteGroup];
-#pragma unroll
-                for (int j = 0; j < kRouteGroup; ++j) {
-                    const int i = kRouteGroup * part + j;
-                    Key<NW> key;
-                    if (win.key(i, key)) okm |= 1u << i;
-                    w0[j] = key.w[0];
-                    keys[i] = key;
-                }
-                coarse_of_all<kRouteGroup>(rt, s_cb, s_lut, w0, bk + kRouteGroup * part);
-            }
-        }
-        // the scatter of a bucket-range round keeps its own buckets only, then
-        // the histogram atomics (after every key is routed)
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (bk[i] < rt.rlo || bk[i] >= rt.rhi) okm &= ~(1u << i);
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (okm & (1u << i)) rank[i] = atomicAdd(&s_hist[bk[i]], 1u);
-    } else if (rt.mode == 0) {
+    if constexpr (!SCATTER) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             Key<NW> key;
             if (win.key(i, key)) atomicAdd(&s_hist[(uint32_t)(key.w[0] >> 56)], 1u);
         }
-    } else {
-#pragma unroll
-        for (int part = 0; part < 16 / kRouteGroup; ++part) {
-            uint64_t w0[kRouteGroup];
-            uint32_t b[kRouteGroup];
-            uint32_t ok = 0;
-#pragma unroll
-            for (int j = 0; j < kRouteGroup; ++j) {
-                Key<NW> key;
-                if (win.key(kRouteGroup * part + j, key)) ok |= 1u << j;
-                w0[j] = key.w[0];
-            }
-            coarse_of_all<kRouteGroup>(rt, s_cb, s_lut, w0, b);
-#pragma unroll
-            for (int j = 0; j < kRouteGroup; ++j)
-                if (ok & (1u << j)) atomicAdd(&s_hist[b[j]], 1u);
-        }
-    }
-    __syncthreads();
-
-    if constexpr (!SCATTER) {
+        __syncthreads();
         if (t < 256) {
             const uint32_t h = s_hist[t];
             if (h) atomicAdd(&counts[(size_t)xcd * 256 + t], (unsigned long long)h);
         }
-        return;
     } else {
+        Key<NW> keys[16];
+        uint32_t rank[16];
+        uint32_t okm = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            Key<NW> key;
+            if (win.key(i, key)) okm |= 1u << i;
+            keys[i] = key;
+        }
+        // the scatter of a bucket-range round keeps its own buckets only, then
+        // the histogram atomics (after every key is computed)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t bk = (uint32_t)(keys[i].w[0] >> 56);
+            if (bk < rt.rlo || bk >= rt.rhi) okm &= ~(1u << i);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (okm & (1u << i)) rank[i] = atomicAdd(&s_hist[(uint32_t)(keys[i].w[0] >> 56)], 1u);
+        __syncthreads();
         uint32_t h = (t < 256) ? s_hist[t] : 0u;
         // the cursor reservation is issued first and its result stored only
         // after the scan and the staging: its round trip overlaps them (the
@@ -482,28 +376,16 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
         if (t < 256) s_off[t] = off;
         __syncthreads();
-        // prefix buckets are the keys' top byte: no bucket byte per slot then
-        const bool stage_bucket = rt.mode != 0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            if (okm & (1u << i)) {
-                const uint32_t slot = s_off[bk[i]] + rank[i];
-                s_keys[slot] = keys[i];
-                if (stage_bucket) s_bkt[slot] = (uint8_t)bk[i];
-            }
-        }
+        for (int i = 0; i < 16; ++i)
+            if (okm & (1u << i)) s_keys[s_off[(uint32_t)(keys[i].w[0] >> 56)] + rank[i]] = keys[i];
         // bucket b's keys go to gb + (slot - off): one base per bucket (the
         // difference wraps modulo 2^64 and the sum with slot is exact)
         if (h) s_dst[t] = gb - (unsigned long long)off;
         __syncthreads();
-        if (stage_bucket) {
-            for (uint32_t slot = t; slot < total; slot += T)
-                store_key<NW>(out, s_dst[s_bkt[slot]] + slot, s_keys[slot]);
-        } else {
-            for (uint32_t slot = t; slot < total; slot += T) {
-                const Key<NW> key = s_keys[slot];
-                store_key<NW>(out, s_dst[(uint32_t)(key.w[0] >> 56)] + slot, key);
-            }
+        for (uint32_t slot = t; slot < total; slot += T) {
+            const Key<NW> key = s_keys[slot];
+            store_key<NW>(out, s_dst[(uint32_t)(key.w[0] >> 56)] + slot, key);
         }
     }
 }
@@ -597,65 +479,175 @@ k_scan_fine_add(const unsigned long long* __restrict__ counts, uint32_t nf, unsi
 
 // ------------------------------------------------------------------ P2
 
-// Tiles of kTile keys inside one coarse bucket of A.  WIDE: up to 1024 fine
-// buckets per coarse bucket (sbits 9 and 10), with u16 bucket indexes and
-// fine tables (SubWideCfg); otherwise up to 256.
-template <int NW, bool SCATTER, bool WIDE>
-__global__ void __launch_bounds__(WIDE ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads)
-k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coarse_off,
-      const unsigned long long* __restrict__ tile_start, Route rt,
-      unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
-      uint64_t* __restrict__ B, uint64_t tile0)
+// Root bucket of tile tid: last r with tile_start[r] <= tid (tid < tile_start[256]).
+__device__ __forceinline__ uint32_t root_of_tile(const unsigned long long* tile_start, unsigned long long tid)
 {
-    constexpr int T = WIDE ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads;
-    constexpr int TILE = WIDE ? SubWideCfg<NW>::kTile : SubCfg<NW>::kTile;
-    constexpr int PER = TILE / T;
-    constexpr uint32_t NB = WIDE ? 1024u : 256u;       // fine buckets per coarse bucket, at most
-    constexpr uint32_t E = (NB + T - 1) / T;           // histogram entries per thread in the scan
-    using LutT = typename std::conditional<WIDE, uint16_t, uint8_t>::type;
-    using BktT = typename std::conditional<WIDE, uint16_t, uint8_t>::type;
-    // fine bounds and the bucket's fine table are dead once every thread has
-    // routed its keys (a barrier later): they share LDS with the staged keys
-    constexpr size_t kStage = NB * sizeof(uint64_t) + kLutCells * sizeof(LutT);
-    constexpr size_t kKeys = SCATTER ? TILE * sizeof(Key<NW>) : 0;
-    __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
-    uint64_t* s_fb = reinterpret_cast<uint64_t*>(s_raw);
-    LutT* s_lut = reinterpret_cast<LutT*>(s_raw + NB * sizeof(uint64_t));
-    Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
-    __shared__ uint32_t s_hist[NB];
-    __shared__ uint32_t s_off[NB];
-    __shared__ unsigned long long s_dst[NB];  // global index of the tile's slot 0 for fine bucket f
-    __shared__ uint32_t s_scan[T / kWave + 1];
-    __shared__ BktT s_bkt[SCATTER ? TILE : 1];
-
-    const int t = threadIdx.x;
-    const unsigned long long tid = tile0 + blockIdx.x;  // tile0: first tile of this launch
-    const unsigned long long ntiles = tile_start[256];
-    if (tid >= ntiles) return;
-    // coarse bucket of this tile: last r with tile_start[r] <= tid
     uint32_t lo = 0, hi = 256;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (tile_start[mid] <= tid) lo = mid; else hi = mid;
     }
-    const uint32_t r = lo;
+    return lo;
+}
+
+// P2a: fine bucket counts per XCD slot.  The scatter pass (k_sub) reserves
+// each tile's runs on the cursor of the XCD slot its workgroup gets, (tile %
+// piece) % 8 with `piece` tiles per scatter launch, so each tile is counted
+// into that slot.  A workgroup takes kSubCountTiles tiles 8 apart (one slot
+// when piece is a multiple of 8) and keeps one LDS histogram while root and
+// slot stay the same: one global add per fine bucket for up to kSubCountTiles
+// tiles instead of one per tile, and the root's learned tables loaded once.
+constexpr uint32_t kSubCountTiles = 8;
+
+template <int NW>
+__global__ void __launch_bounds__(SubCfg<NW>::kCountThreads)
+k_sub_count(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coarse_off,
+            const unsigned long long* __restrict__ tile_start, Route rt, unsigned long long* __restrict__ counts,
+            uint64_t piece, uint64_t group0)
+{
+    constexpr int T = SubCfg<NW>::kCountThreads;
+    constexpr int TILE = SubCfg<NW>::kTile;
+    constexpr int PER = TILE / T;
+    constexpr uint32_t NB = kMaxFine;
+    __shared__ uint64_t s_fb[NB + 2];
+    __shared__ __align__(16) uint16_t s_lut[kLutCells];
+    __shared__ uint32_t s_hist[NB];
+
+    const int t = threadIdx.x;
+    const unsigned long long ntiles = tile_start[256];
+    // tiles g0 + x + 8 j, j < kSubCountTiles: g0 = 8 kSubCountTiles (wg / 8), x = wg % 8
+    const unsigned long long wg = group0 + blockIdx.x;
+    const unsigned long long first = (wg / kXcds) * (kXcds * kSubCountTiles) + (wg % kXcds);
+    if (first >= ntiles) return;
+    const bool learned = rt.mode == 1;
+    const uint32_t nf = rt.nf;
+    for (uint32_t i = t; i < NB; i += T) s_hist[i] = 0;
+    uint32_t r = root_of_tile(tile_start, first);
+    uint32_t cur_r = ~0u, cur_x = 0, fbase = 0, nsub = 0;
+    auto flush = [&]() {
+        for (uint32_t i = t; i < nsub; i += T) {
+            const uint32_t h = s_hist[i];
+            if (h) {
+                atomicAdd(&counts[(size_t)cur_x * nf + fbase + i], (unsigned long long)h);
+                s_hist[i] = 0;
+            }
+        }
+    };
+    for (uint32_t j = 0; j < kSubCountTiles; ++j) {
+        const unsigned long long tid = first + (unsigned long long)kXcds * j;
+        if (tid >= ntiles) break;
+        while (tile_start[r + 1] <= tid) ++r;
+        const uint32_t x = (uint32_t)((tid % piece) % kXcds);
+        if (r != cur_r || x != cur_x) {
+            __syncthreads();  // the last tile's histogram atomics are done
+            flush();
+            cur_r = r;
+            cur_x = x;
+            fbase = learned ? rt.fine_base[r] : (r << rt.sbits);
+            nsub = learned ? rt.fine_base[r + 1] - fbase : (1u << rt.sbits);
+            if (learned) {
+                // the root's bounds and two pads (fine_of_all reads up to nb + 1)
+                const uint32_t nb = nsub - 1u;
+                const uint64_t* fb = rt.fine_bounds + (fbase - r);
+                for (uint32_t i = t; i < nb + 2; i += T) s_fb[i] = i < nb ? fb[i] : ~0ULL;
+                const uint4* tab = reinterpret_cast<const uint4*>(rt.lut + (size_t)r * kLutCells);
+                for (int i = t; i < (int)(kLutCells * sizeof(uint16_t) / 16); i += T) reinterpret_cast<uint4*>(s_lut)[i] = tab[i];
+            }
+            __syncthreads();
+        }
+        const unsigned long long beg = coarse_off[r] + (tid - tile_start[r]) * (unsigned long long)TILE;
+        const unsigned long long endr = coarse_off[r + 1];
+        const uint32_t n = (uint32_t)((endr - beg) < (unsigned long long)TILE ? (endr - beg) : TILE);
+        const uint64_t flo = (uint64_t)r << 56;
+        // chunks of at most 8 keys per thread (fewer registers, more waves)
+        constexpr int C = PER < 8 ? PER : 8;
+        constexpr int G = C < 4 ? C : 4;
+#pragma unroll 1
+        for (int c0 = 0; c0 < PER; c0 += C) {
+            uint64_t w[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const uint32_t idx = t + (c0 + q) * T;
+                w[q] = idx < n ? A[(beg + idx) * NW] : flo;  // word 0 routes
+            }
+            uint32_t sb[C];
+            if (learned) {
+#pragma unroll
+                for (int g = 0; g < C; g += G) fine_of_all<G>(s_fb, s_lut, flo, nsub - 1u, w + g, sb + g);
+            } else {
+#pragma unroll
+                for (int q = 0; q < C; ++q) sb[q] = (uint32_t)((w[q] >> (56 - rt.sbits)) & (nsub - 1u));
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q)
+                if (t + (c0 + q) * T < n) atomicAdd(&s_hist[sb[q]], 1u);
+        }
+    }
+    __syncthreads();
+    flush();
+}
+
+// P2b: tiles of kTile keys inside one root bucket of A, scattered to their
+// fine buckets in B.  A root's fine buckets are its prefix bits (sbits of
+// them) or, learned, its share of the learned partition's fine buckets (up to
+// kMaxFine: u16 bucket indexes).  The sorted tile is staged in two rounds of
+// half a tile each (the keys of slots [0, TILE/2), stored, then the rest):
+// 56 KB of LDS, so two workgroups share a CU and one's stores overlap the
+// other's loads and routing.
+template <int NW>
+__global__ void __launch_bounds__(SubCfg<NW>::kThreads)
+k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coarse_off,
+      const unsigned long long* __restrict__ tile_start, Route rt, unsigned long long* __restrict__ cursor,
+      uint64_t* __restrict__ B, uint64_t tile0)
+{
+    constexpr int T = SubCfg<NW>::kThreads;
+    constexpr int TILE = SubCfg<NW>::kTile;
+    constexpr int PER = TILE / T;
+    constexpr uint32_t NB = kMaxFine;                 // fine buckets per root, at most
+    constexpr uint32_t E = (NB + T - 1) / T;           // histogram entries per thread in the scan
+    constexpr int SR = 2;
+    constexpr uint32_t STAGE = TILE / SR;
+    // the root's fine bounds (NB + 1 entries, padded with UINT64_MAX; NB + 2
+    // slots keep the table 16-byte aligned) and table are dead once every
+    // thread has routed its keys (a barrier later): they share LDS with the
+    // staged keys
+    constexpr size_t kFbBytes = (NB + 2) * sizeof(uint64_t);
+    static_assert(kFbBytes % 16 == 0, "the table is copied in 16-byte units");
+    constexpr size_t kStage = kFbBytes + kLutCells * sizeof(uint16_t);
+    constexpr size_t kKeys = STAGE * sizeof(Key<NW>);
+    __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
+    uint64_t* s_fb = reinterpret_cast<uint64_t*>(s_raw);
+    uint16_t* s_lut = reinterpret_cast<uint16_t*>(s_raw + kFbBytes);
+    Key<NW>* s_keys = reinterpret_cast<Key<NW>*>(s_raw);
+    __shared__ uint32_t s_hist[NB];
+    __shared__ uint32_t s_off[NB];
+    __shared__ unsigned long long s_dst[NB];  // global index of the tile's slot 0 for fine bucket f
+    __shared__ uint32_t s_scan[T / kWave + 1];
+    __shared__ uint16_t s_bkt[STAGE];
+
+    const int t = threadIdx.x;
+    const unsigned long long tid = tile0 + blockIdx.x;  // tile0: first tile of this launch
+    const unsigned long long ntiles = tile_start[256];
+    if (tid >= ntiles) return;
+    const uint32_t r = root_of_tile(tile_start, tid);
+    const bool learned = rt.mode == 1;
+    const uint32_t fbase = learned ? rt.fine_base[r] : (r << rt.sbits);
+    const uint32_t nsub = learned ? rt.fine_base[r + 1] - fbase : (1u << rt.sbits);
     const unsigned long long beg = coarse_off[r] + (tid - tile_start[r]) * (unsigned long long)TILE;
     const unsigned long long endr = coarse_off[r + 1];
     const uint32_t n = (uint32_t)((endr - beg) < (unsigned long long)TILE ? (endr - beg) : TILE);
     const uint32_t xcd = blockIdx.x % kXcds;
-    const uint32_t nsub = 1u << rt.sbits;
-    const uint32_t nf = 256u << rt.sbits;
+    const uint32_t nf = rt.nf;
+    const uint32_t nb = nsub - 1u;  // the root's bounds (learned)
+    const uint64_t flo = (uint64_t)r << 56;
 
     for (uint32_t i = t; i < NB; i += T) s_hist[i] = 0;
-    uint64_t flo = 0;
-    uint32_t shift = 0;
-    if (rt.mode == 1 && rt.sbits) {
-        for (uint32_t i = t; i + 1 < NB; i += T) s_fb[i] = i + 1 < nsub ? rt.fine_bounds[(size_t)r * (nsub - 1) + i] : ~0ULL;
-        const uint8_t* tab = rt.lut + kLutCells + (size_t)r * kLutCells * sizeof(LutT);
-        for (int i = t; i < (int)(kLutCells * sizeof(LutT) / 16); i += T)
-            reinterpret_cast<uint4*>(s_lut)[i] = reinterpret_cast<const uint4*>(tab)[i];
-        flo = r ? rt.coarse_bounds[r - 1] : 0;
-        shift = rt.lut[kLutCells + 256 * (size_t)kLutCells * sizeof(LutT) + r];
+    if (learned) {
+        // the root's bounds and two pads (fine_of_all reads up to nb + 1)
+        const uint64_t* fb = rt.fine_bounds + (fbase - r);
+        for (uint32_t i = t; i < nb + 2; i += T) s_fb[i] = i < nb ? fb[i] : ~0ULL;
+        const uint4* tab = reinterpret_cast<const uint4*>(rt.lut + (size_t)r * kLutCells);
+        for (int i = t; i < (int)(kLutCells * sizeof(uint16_t) / 16); i += T) reinterpret_cast<uint4*>(s_lut)[i] = tab[i];
     }
     __syncthreads();
 
@@ -673,98 +665,89 @@ k_sub(const uint64_t* __restrict__ A, const unsigned long long* __restrict__ coa
     // dominates the later uses each use of a key would wait for every global
     // operation in flight, the cursor reservations below included
     __builtin_amdgcn_s_waitcnt(0x0F70);
-    if constexpr (SCATTER) {
-        // routed first, then the atomics (as in k_extract's scatter).  With
-        // 512 or 1024 learned fine buckets a quarter of the table cells hold a
-        // bound: the lookups go four keys at a time (fine_of_all; measured
-        // 57.7 -> 53.5 ms at 6.8e9 keys, and no gain with 256 fine buckets)
-        bool grouped = false;
-        if constexpr (WIDE) grouped = rt.mode == 1 && rt.sbits != 0;
-        if (grouped) {
+    if (learned) {
+        // learned fine buckets looked up four keys at a time (fine_of_all;
+        // with 512 or 1024 fine buckets a quarter of the table cells hold a
+        // bound: measured 57.7 -> 53.5 ms at 6.8e9 keys)
+        constexpr int G = PER < 4 ? PER : 4;
 #pragma unroll
-            for (int g = 0; g < PER; g += 4) {
-                uint64_t w[4];
-                uint32_t f[4];
+        for (int g = 0; g < PER; g += G) {
+            uint64_t w[G];
+            uint32_t f[G];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) w[u] = (t + (g + u) * T < n) ? keys[g + u].w[0] : flo;
-                fine_of_all<4>(rt, s_fb, s_lut, flo, shift, w, f);
+            for (int u = 0; u < G; ++u) w[u] = (t + (g + u) * T < n) ? keys[g + u].w[0] : flo;
+            fine_of_all<G>(s_fb, s_lut, flo, nb, w, f);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) sb[g + u] = (t + (g + u) * T < n) ? f[u] : 0u;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < PER; ++j) sb[j] = (t + j * T < n) ? fine_of(rt, s_fb, s_lut, flo, shift, keys[j].w[0]) : 0u;
+            for (int u = 0; u < G; ++u) sb[g + u] = f[u];
         }
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            if (t + j * T < n) rank[j] = atomicAdd(&s_hist[sb[j]], 1u);
     } else {
 #pragma unroll
-        for (int j = 0; j < PER; ++j)
-            if (t + j * T < n) atomicAdd(&s_hist[fine_of(rt, s_fb, s_lut, flo, shift, keys[j].w[0])], 1u);
+        for (int j = 0; j < PER; ++j) sb[j] = (uint32_t)((keys[j].w[0] >> (56 - rt.sbits)) & (nsub - 1u));
     }
+    // routed first, then the atomics (as in k_extract's scatter)
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+        if (t + j * T < n) rank[j] = atomicAdd(&s_hist[sb[j]], 1u);
     __syncthreads();
 
-    if constexpr (!SCATTER) {
-        for (uint32_t i = t; i < nsub; i += T) {
-            const uint32_t h = s_hist[i];
-            if (h) atomicAdd(&counts[(size_t)xcd * nf + ((size_t)r << rt.sbits) + i], (unsigned long long)h);
-        }
-        return;
-    } else {
-        // entries [E t, E t + E) per thread: local sums, then one workgroup scan
-        uint32_t h[E];
-        uint32_t mine = 0;
+    // entries [E t, E t + E) per thread: local sums, then one workgroup scan
+    uint32_t h[E];
+    uint32_t mine = 0;
 #pragma unroll
-        for (uint32_t q = 0; q < E; ++q) {
-            const uint32_t i = E * t + q;
-            h[q] = i < nsub ? s_hist[i] : 0u;
-            mine += h[q];
-        }
-        // cursor reservations issued before the scan, stored after the staging
-        // (their round trips overlap both, as in k_extract)
-        unsigned long long gb[E];
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = E * t + q;
+        h[q] = i < nsub ? s_hist[i] : 0u;
+        mine += h[q];
+    }
+    // cursor reservations issued before the scan, stored after the staging
+    // (their round trips overlap both, as in k_extract)
+    unsigned long long gb[E];
 #pragma unroll
-        for (uint32_t q = 0; q < E; ++q) {
-            const uint32_t i = E * t + q;
-            gb[q] = 0;
-            if (h[q]) gb[q] = atomicAdd(&cursor[(size_t)xcd * nf + ((size_t)r << rt.sbits) + i], (unsigned long long)h[q]);
-        }
-        uint32_t total;
-        uint32_t off = block_exclusive_scan<uint32_t>(mine, s_scan, &total);
-        uint32_t offq[E];
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = E * t + q;
+        gb[q] = 0;
+        if (h[q]) gb[q] = atomicAdd(&cursor[(size_t)xcd * nf + fbase + i], (unsigned long long)h[q]);
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive_scan<uint32_t>(mine, s_scan, &total);
+    uint32_t offq[E];
 #pragma unroll
-        for (uint32_t q = 0; q < E; ++q) {
-            const uint32_t i = E * t + q;
-            offq[q] = off;
-            if (i < nsub) s_off[i] = off;
-            off += h[q];
-        }
-        __syncthreads();
+    for (uint32_t q = 0; q < E; ++q) {
+        const uint32_t i = E * t + q;
+        offq[q] = off;
+        if (i < nsub) s_off[i] = off;
+        off += h[q];
+    }
+    __syncthreads();
+    // fine bucket f's keys go to gb + (slot - off), as in k_extract
+#pragma unroll
+    for (uint32_t q = 0; q < E; ++q)
+        if (h[q]) s_dst[E * t + q] = gb[q] - (unsigned long long)offq[q];
+    uint32_t slot[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) slot[j] = (t + j * T < n) ? s_off[sb[j]] + rank[j] : ~0u;
+#pragma unroll
+    for (int h2 = 0; h2 < SR; ++h2) {
+        const uint32_t s0 = h2 * STAGE;
+        if (h2) __syncthreads();  // the previous round's stores have read the stage
         // prefix fine buckets are key bits: no bucket index per slot then
-        const bool stage_bucket = rt.mode != 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const uint32_t idx = t + j * T;
-            if (idx < n) {
-                const uint32_t slot = s_off[sb[j]] + rank[j];
-                s_keys[slot] = keys[j];
-                if (stage_bucket) s_bkt[slot] = (BktT)sb[j];
+            if (slot[j] - s0 < STAGE) {  // slots of this round (~0u: no key)
+                s_keys[slot[j] - s0] = keys[j];
+                if (learned) s_bkt[slot[j] - s0] = (uint16_t)sb[j];
             }
         }
-        // fine bucket f's keys go to gb + (slot - off), as in k_extract
-#pragma unroll
-        for (uint32_t q = 0; q < E; ++q)
-            if (h[q]) s_dst[E * t + q] = gb[q] - (unsigned long long)offq[q];
         __syncthreads();
-        if (stage_bucket) {
-            for (uint32_t slot = t; slot < total; slot += T)
-                store_key<NW>(B, s_dst[s_bkt[slot]] + slot, s_keys[slot]);
+        const uint32_t end = min(total, s0 + STAGE);
+        if (learned) {
+            for (uint32_t sl = s0 + t; sl < end; sl += T)
+                store_key<NW>(B, s_dst[s_bkt[sl - s0]] + sl, s_keys[sl - s0]);
         } else {
-            for (uint32_t slot = t; slot < total; slot += T) {
-                const Key<NW> key = s_keys[slot];
-                const uint32_t f = rt.sbits ? (uint32_t)((key.w[0] >> (56 - rt.sbits)) & (nsub - 1u)) : 0u;
-                store_key<NW>(B, s_dst[f] + slot, key);
+            for (uint32_t sl = s0 + t; sl < end; sl += T) {
+                const Key<NW> key = s_keys[sl - s0];
+                const uint32_t f = (uint32_t)((key.w[0] >> (56 - rt.sbits)) & (nsub - 1u));
+                store_key<NW>(B, s_dst[f] + sl, key);
             }
         }
     }
@@ -1994,19 +1977,26 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
                       const unsigned long long* tile_start, const Route& rt, unsigned long long* counts,
                       unsigned long long* cursor, uint64_t* B, hipStream_t st)
 {
-    if (rt.sbits > kMaxSubBits) return hipErrorInvalidValue;
-    const uint32_t T = rt.sbits > 8 ? SubWideCfg<NW>::kThreads : SubCfg<NW>::kThreads;
-    const uint64_t piece = launch_piece_tiles(T);
+    if (rt.mode == 0 && (rt.sbits == 0 || rt.sbits > kMaxSubBits)) return hipErrorInvalidValue;
+    // tiles per scatter launch: the count pass files each tile under the XCD
+    // slot the scatter gives it
+    const uint64_t piece = launch_piece_tiles(SubCfg<NW>::kThreads);
+    if (!scatter) {
+        // workgroups of kSubCountTiles tiles 8 apart: ceil(tiles / (8 G)) * 8
+        const uint64_t span = (uint64_t)kXcds * kSubCountTiles;
+        const uint64_t groups = (max_tiles + span - 1) / span * kXcds;
+        const uint64_t gpiece = std::max<uint64_t>(kXcds, launch_piece_tiles(SubCfg<NW>::kCountThreads) / kXcds * kXcds);
+        for (uint64_t g0 = 0; g0 < groups; g0 += gpiece) {
+            const dim3 g((uint32_t)std::min<uint64_t>(gpiece, groups - g0));
+            hipLaunchKernelGGL((k_sub_count<NW>), g, dim3(SubCfg<NW>::kCountThreads), 0, st, A, coarse_off, tile_start,
+                               rt, counts, piece, g0);
+            KMC_LAUNCH_CHECK();
+        }
+        return hipSuccess;
+    }
     for (uint64_t t0 = 0; t0 < max_tiles; t0 += piece) {
         const dim3 g((uint32_t)std::min<uint64_t>(piece, max_tiles - t0));
-        const dim3 b(T);
-        if (rt.sbits > 8) {
-            if (scatter) hipLaunchKernelGGL((k_sub<NW, true, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
-            else hipLaunchKernelGGL((k_sub<NW, false, true>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
-        } else {
-            if (scatter) hipLaunchKernelGGL((k_sub<NW, true, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
-            else hipLaunchKernelGGL((k_sub<NW, false, false>), g, b, 0, st, A, coarse_off, tile_start, rt, counts, cursor, B, t0);
-        }
+        hipLaunchKernelGGL((k_sub<NW>), g, dim3(SubCfg<NW>::kThreads), 0, st, A, coarse_off, tile_start, rt, cursor, B, t0);
         KMC_LAUNCH_CHECK();
     }
     return hipSuccess;

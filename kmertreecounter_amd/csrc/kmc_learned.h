@@ -74,15 +74,16 @@ void boundaries_from_predictions(const std::vector<uint64_t>& sorted, const std:
 // of the sorted distinct sample `keys` takes half of that bucket's keys
 // (boundary moved to its median).  Returns the number of buckets split.
 uint32_t split_merged_buckets(const std::vector<uint64_t>& keys, std::vector<uint64_t>& b);
-// Coarse (255) then per-coarse fine (256 x (2^sbits - 1)) routing boundaries
-// from the 4097 learned boundaries; fine ones interpolate inside a learned bucket.
-void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<uint64_t>& out);
-// Lookup tables that start each routing search next to its answer:
-// [0, 4096): coarse table, entry p = number of coarse bounds <= p << 52;
-// then per coarse bucket r a 4096-entry fine table over r's key range
-// [lo_r, hi_r): entry c = number of r's fine bounds <= lo_r + (c << shift_r)
-// (u8 entries up to 256 fine buckets, little-endian u16 above);
-// then the 256 shifts.  `rb` is the output of routing_bounds.
-void routing_luts(const std::vector<uint64_t>& rb, uint32_t sbits, std::vector<uint8_t>& lut);
+// Fine buckets of the learned partition.  Roots are the keys' top byte;
+// root r gets nsub_r fine buckets (1..max_fine <= kMaxFine), in proportion to the learned
+// CDF's mass on its key range (about nf_target in all), bounded by the CDF's
+// quantiles inside the range.  The CDF is the piecewise-linear function
+// through the 4097 learned boundaries b (bucket j spans [b[j], b[j+1])).
+struct LearnedRouting {
+    std::vector<uint32_t> fine_base;  // [257]: root r's fine buckets are [fine_base[r], fine_base[r+1])
+    std::vector<uint64_t> bounds;     // root r's nsub_r - 1 bounds at [fine_base[r] - r, ...), nondecreasing
+    std::vector<uint16_t> lut;        // 256 x kLutCells: cell c of r = r's bounds <= (r << 56) + (c << kLutShift)
+};
+void learned_routing(const std::vector<uint64_t>& b, uint32_t nf_target, uint32_t max_fine, LearnedRouting& out);
 
 }  // namespace kmc

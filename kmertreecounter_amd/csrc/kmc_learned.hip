@@ -872,57 +872,71 @@ uint32_t split_merged_buckets(const std::vector<uint64_t>& keys, std::vector<uin
     return split;
 }
 
-void routing_bounds(const std::vector<uint64_t>& b, uint32_t sbits, std::vector<uint64_t>& out)
+namespace {
+
+// The learned CDF: the piecewise-linear function through (b[j], j / 4096),
+// j = 0..4096, with b[4096] read as 2^64; a run of equal boundaries (empty
+// buckets) is a step.  In bucket units (0..4096).
+long double learned_cdf(const std::vector<uint64_t>& b, long double x)
 {
-    const uint32_t F = 1u << sbits;
-    out.assign(255 + (size_t)256 * (F - 1), 0);
-    for (uint32_t c = 0; c < 255; ++c) out[c] = b[16 * (c + 1)];
-    for (uint32_t c = 0; c < 256; ++c) {
-        for (uint32_t s = 1; s < F; ++s) {
-            const uint32_t j = 16 * c + (16 * s) / F;
-            const uint32_t num = (16 * s) % F;
-            uint64_t v = b[j];
-            if (num) {
-                const uint64_t span = b[j + 1] - b[j];
-                v += (uint64_t)(((unsigned __int128)span * num) / F);
-            }
-            out[255 + (size_t)c * (F - 1) + (s - 1)] = v;
-        }
-    }
+    const long double top = 18446744073709551616.0L;  // 2^64
+    if (x >= top) return (long double)kMlpC;
+    const uint64_t xi = (uint64_t)x;
+    uint32_t j = (uint32_t)(std::upper_bound(b.begin(), b.begin() + kMlpC, xi) - b.begin()) - 1u;
+    const long double lo = (long double)b[j];
+    const long double hi = j + 1 < kMlpC ? (long double)b[j + 1] : top;
+    return (long double)j + (hi > lo ? (x - lo) / (hi - lo) : 0.0L);
 }
 
-void routing_luts(const std::vector<uint64_t>& rb, uint32_t sbits, std::vector<uint8_t>& lut)
+// Its inverse: the smallest key whose CDF reaches y (0 <= y < 4096).
+long double learned_quantile(const std::vector<uint64_t>& b, long double y)
 {
-    const uint32_t F = 1u << sbits;
-    lut.assign(kRouteLutBytes, 0);
-    const uint64_t* cb = rb.data();
-    for (uint32_t p = 0; p < kLutCells; ++p)
-        lut[p] = (uint8_t)(std::upper_bound(cb, cb + 255, (uint64_t)p << 52) - cb);
-    const size_t esz = F > 256 ? 2 : 1;  // fine table entry bytes
-    uint8_t* shifts = lut.data() + kLutCells + 256 * (size_t)kLutCells * esz;
+    const long double top = 18446744073709551616.0L;
+    uint32_t j = (uint32_t)std::floor(y);
+    if (j >= kMlpC) return top;
+    const long double lo = (long double)b[j];
+    const long double hi = j + 1 < kMlpC ? (long double)b[j + 1] : top;
+    return lo + (y - (long double)j) * (hi - lo);
+}
+
+}  // namespace
+
+void learned_routing(const std::vector<uint64_t>& b, uint32_t nf_target, uint32_t max_fine, LearnedRouting& out)
+{
+    max_fine = std::max<uint32_t>(1u, std::min<uint32_t>(max_fine, kMaxFine));
+    const long double span = 72057594037927936.0L;  // 2^56 keys per root
+    out.fine_base.assign(257, 0);
+    out.bounds.clear();
+    out.lut.assign(256 * (size_t)kLutCells, 0);
+    std::vector<uint32_t> nsub(256);
+    std::vector<long double> f_lo(257);
+    for (uint32_t r = 0; r <= 256; ++r) f_lo[r] = learned_cdf(b, (long double)r * span);
     for (uint32_t r = 0; r < 256; ++r) {
-        // keys of coarse bucket r: lo <= w0 <= last
-        const uint64_t lo = r ? cb[r - 1] : 0;
-        const uint64_t last = r < 255 ? cb[r] - 1 : ~0ULL;
-        const uint64_t span = last >= lo ? last - lo : 0;
-        const uint32_t bw = span ? 64 - (uint32_t)__builtin_clzll(span) : 0;
-        const uint32_t shift = bw > 12 ? bw - 12 : 0;
-        shifts[r] = (uint8_t)shift;
-        uint8_t* t = lut.data() + kLutCells + (size_t)r * kLutCells * esz;
-        if (F == 1) continue;
-        const uint64_t* fb = rb.data() + 255 + (size_t)r * (F - 1);
-        for (uint32_t c = 0; c < kLutCells; ++c) {
-            const uint64_t off = (uint64_t)c << shift;
-            uint32_t v = F - 1;  // cell past the bucket's last key: never looked up
-            if (shift < 64 && (off >> shift) == c && off <= span)
-                v = (uint32_t)(std::upper_bound(fb, fb + (F - 1), lo + off) - fb);
-            if (esz == 2) {
-                const uint16_t v16 = (uint16_t)v;
-                std::memcpy(t + 2 * (size_t)c, &v16, 2);
-            } else {
-                t[c] = (uint8_t)v;
-            }
+        const long double mass = (f_lo[r + 1] - f_lo[r]) / (long double)kMlpC;
+        const long double want = std::floor(mass * (long double)nf_target + 0.5L);
+        nsub[r] = (uint32_t)std::min<long double>((long double)max_fine, std::max<long double>(1.0L, want));
+        out.fine_base[r + 1] = out.fine_base[r] + nsub[r];
+    }
+    out.bounds.reserve(out.fine_base[256] - 256);
+    for (uint32_t r = 0; r < 256; ++r) {
+        const uint64_t lo = (uint64_t)r << 56;
+        const uint64_t last = r < 255 ? lo + ((1ULL << 56) - 1) : ~0ULL;
+        const size_t first = out.bounds.size();
+        uint64_t prev = lo;
+        for (uint32_t s = 1; s < nsub[r]; ++s) {
+            // key at the CDF level s / nsub of the way through the root's mass
+            const long double y = f_lo[r] + (f_lo[r + 1] - f_lo[r]) * (long double)s / (long double)nsub[r];
+            long double q = std::ceil(learned_quantile(b, y));
+            uint64_t v = q >= (long double)last ? last : (uint64_t)q;
+            v = std::max<uint64_t>(v, std::max<uint64_t>(prev, lo + 1));  // inside the root, nondecreasing
+            out.bounds.push_back(v);
+            prev = v;
         }
+        const uint64_t* fb = out.bounds.data() + first;
+        const uint32_t nb = nsub[r] - 1;
+        uint16_t* t = out.lut.data() + (size_t)r * kLutCells;
+        for (uint32_t c = 0; c < kLutCells; ++c)
+            t[c] = (uint16_t)(std::upper_bound(fb, fb + nb, lo + ((uint64_t)c << kLutShift)) - fb);
     }
 }
 

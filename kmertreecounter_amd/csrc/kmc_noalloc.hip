@@ -135,10 +135,11 @@ int count_noalloc_t(const kc_config& cfg, int dev, const char* d_seq, uint64_t s
     b.rec_out_cap = records_cap;
     PipeOpts o{};
     o.k = cfg.k;
-    o.sbits = 0;
+    o.sub = false;
     o.nf = 256;
     o.rt.mode = 0;
     o.rt.sbits = 0;
+    o.rt.nf = 256;
     o.rt.rlo = 0;
     o.rt.rhi = 256;
     o.min_count = cfg.min_count ? cfg.min_count : 1u;

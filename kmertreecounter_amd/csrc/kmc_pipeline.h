@@ -25,9 +25,8 @@ inline uint64_t ext_tile(uint32_t nw)
 {
     return (nw == 1 ? ExtractCfg<1>::kThreads : nw == 2 ? ExtractCfg<2>::kThreads : ExtractCfg<4>::kThreads) * 16ULL;
 }
-inline uint64_t sub_tile(uint32_t nw, uint32_t sbits)
+inline uint64_t sub_tile(uint32_t nw)
 {
-    if (sbits > 8) return nw == 1 ? SubWideCfg<1>::kTile : nw == 2 ? SubWideCfg<2>::kTile : SubWideCfg<4>::kTile;
     return nw == 1 ? SubCfg<1>::kTile : nw == 2 ? SubCfg<2>::kTile : SubCfg<4>::kTile;
 }
 inline uint64_t rec_bytes(uint32_t nw) { return 8ULL * nw + 8; }
@@ -42,9 +41,9 @@ struct PipeBufs {
     uint64_t key_cap;
     unsigned long long* small;       // kSmTotal words
     unsigned long long* ctr;         // kCtrCount words
-    unsigned long long* fcounts;     // kXcds x nf (sbits > 0)
-    unsigned long long* fcursor;     // kXcds x nf (sbits > 0)
-    unsigned long long* fine_off;    // nf + 1 (sbits > 0)
+    unsigned long long* fcounts;     // kXcds x nf (with a sub-partition)
+    unsigned long long* fcursor;     // kXcds x nf (with a sub-partition)
+    unsigned long long* fine_off;    // nf + 1 (with a sub-partition)
     unsigned long long* foff;        // nf + 1
     unsigned long long* roff;        // nf + 1
     uint32_t* bucket_u32;            // 4 x nf: nfirst, nrec, ndist per bucket, counting order
@@ -64,7 +63,8 @@ struct PipeBufs {
 
 struct PipeOpts {
     uint32_t k;
-    uint32_t sbits, nf;              // fine fan-out: nf = 256 << sbits buckets
+    bool sub;                        // fine buckets below the roots (P2 runs): prefix sbits > 0, or learned
+    uint32_t nf;                     // fine buckets (256 without a sub-partition)
     Route rt;
     uint32_t min_count, max_count;   // record filter (append_export_record)
     uint64_t pass_keys;              // 0: the counter's default
@@ -77,14 +77,14 @@ struct PipeOpts {
 
 // Where a pass leaves its outputs: firsts, records, and the P3 key input.
 template <int NW>
-inline uint64_t* pipe_firsts(const PipeBufs& b, uint32_t sbits)
+inline uint64_t* pipe_firsts(const PipeBufs& b, bool sub)
 {
-    return b.firsts_out ? b.firsts_out : (sbits ? b.B : b.A);
+    return b.firsts_out ? b.firsts_out : (sub ? b.B : b.A);
 }
 template <int NW>
-inline uint8_t* pipe_records(const PipeBufs& b, uint32_t sbits)
+inline uint8_t* pipe_records(const PipeBufs& b, bool sub)
 {
-    return b.recs_out ? b.recs_out : reinterpret_cast<uint8_t*>(sbits ? b.A : b.B);
+    return b.recs_out ? b.recs_out : reinterpret_cast<uint8_t*>(sub ? b.A : b.B);
 }
 
 // last error text of kc_count_noalloc on the calling thread

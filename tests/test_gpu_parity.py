@@ -118,19 +118,24 @@ def test_fanout_does_not_change_table(sub_bits):
 @pytest.mark.parametrize("k", [31, 63, 100])
 @pytest.mark.parametrize("sub_bits", [9, 10])
 def test_wide_fanout_with_learned_routing(k, sub_bits):
-    # 512 or 1024 fine buckets per coarse bucket: u16 bucket indexes and fine
-    # tables in the sub-partition kernel, for both partitions
+    # 512 or 1024 fine buckets per root (prefix), or learned roots of up to
+    # 1024: u16 bucket indexes in the sub-partition kernel, both partitions
     seq = K.generate(2, scale=200, seed=5 + k, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, k))
     for part in (K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED):
-        with K.Engine(k, partition=part, sub_bits=sub_bits, learned_min_skew_pct=1) as e:
+        # learned roots of up to 1024 fine buckets (the wide sub-partition
+        # variant next to the narrow one in the same pass)
+        with K.Engine(k, partition=part, sub_bits=sub_bits, learned_min_skew_pct=1, learned_max_fine=1024) as e:
             if part == K.KC_PARTITION_LEARNED:
                 e.sample_block(seq[: len(seq) // 5])
                 e.plan()
             e.push_block(seq)
             e.finish()
             f, r, s = e.firsts(), e.records(), e.stats()
-            assert s.fine_buckets == 256 << sub_bits
+            if part == K.KC_PARTITION_PREFIX:
+                assert s.fine_buckets == 256 << sub_bits
+            else:  # each root's share of 256 << sub_bits, rounded, 1..1024 per root (heavy roots capped)
+                assert 256 <= s.fine_buckets <= (256 << sub_bits) + 256
             check_outputs(f, r, s, k, keys, counts)
             for rid in (0, 99, 255):
                 rr = e.root(rid)
@@ -425,7 +430,8 @@ def test_kernels_launched_in_pieces(monkeypatch, partition, sub_bits):
     seq = K.generate(2, scale=400, seed=23, fmt="seq")
     keys, counts = C.count_table(C.extract(seq, 31))
     monkeypatch.setenv("KMC_MAX_GRID_TILES", "13")
-    with K.Engine(31, partition=partition, sub_bits=sub_bits, learned_min_skew_pct=1) as e:
+    # learned: roots of both sub-partition variants (up to 1024 fine buckets)
+    with K.Engine(31, partition=partition, sub_bits=sub_bits, learned_min_skew_pct=1, learned_max_fine=1024) as e:
         if partition == K.KC_PARTITION_LEARNED:
             e.sample_block(seq[: len(seq) // 10])
             e.plan()
