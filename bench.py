@@ -47,6 +47,7 @@ import kmertreecounter_amd as K  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 LEARNED_MAX_FINE = 0  # --learned-max-fine: fine buckets of one root, at most (0: the engine's default)
+LEARNED_STEPS = 0  # --learned-steps: training steps of the network (0: the engine's default)
 METRIC = "k-mers counted/s at k=31, learned vs prefix partition; % HBM peak"
 
 WORKLOADS = {
@@ -260,7 +261,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
     part = K.KC_PARTITION_LEARNED if partition == "learned" else K.KC_PARTITION_PREFIX
     cnt = {"bloom": K.KC_COUNTER_BLOOM, "exact": K.KC_COUNTER_EXACT, "sort": K.KC_COUNTER_SORT}[counter]
     eng = K.Engine(k, partition=part, expected_bytes=len(seq), counter=cnt, graph=graph,
-                   learned_max_fine=LEARNED_MAX_FINE)
+                   learned_max_fine=LEARNED_MAX_FINE, learned_steps=LEARNED_STEPS)
     plan_ms = 0.0
     bounds_io = bounds_io or {}
     bounds = None
@@ -526,6 +527,8 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help="replay every workload as a captured HIP graph")
     ap.add_argument("--eager", action="store_true", help="no graph and no autotuner for any workload")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--learned-steps", type=int, default=0,
+                    help="learned partition: training steps of the network (0: the engine's default)")
     ap.add_argument("--learned-max-fine", type=int, default=0,
                     help="learned partition: fine buckets of one root bucket, at most (0: the engine's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -538,8 +541,9 @@ def main() -> int:
                     help="after the run, copy this many GiB with k_copy16 (a plain copy whose "
                          "FETCH_SIZE/WRITE_SIZE readings scale those of the pipeline kernels)")
     args = ap.parse_args()
-    global LEARNED_MAX_FINE
+    global LEARNED_MAX_FINE, LEARNED_STEPS
     LEARNED_MAX_FINE = args.learned_max_fine
+    LEARNED_STEPS = args.learned_steps
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (the k-mer exchange between "

@@ -50,7 +50,7 @@ class Config(ctypes.Structure):
         ("pass_keys", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("table_slots", ctypes.c_uint32),
         ("heavy_keys", ctypes.c_uint32), ("learned_min_skew_pct", ctypes.c_uint32), ("max_round_keys", ctypes.c_uint64),
         ("graph", ctypes.c_uint32), ("round_outputs", ctypes.c_uint32), ("est_file_size", ctypes.c_uint64),
-        ("learned_max_fine", ctypes.c_uint32),
+        ("learned_max_fine", ctypes.c_uint32), ("learned_steps", ctypes.c_uint32),
     ]
 
 
@@ -281,14 +281,15 @@ class Engine:
                  expected_bytes: int = 0, sub_bits: int = 0, device: int = 0, classifier_num: int = 3,
                  pass_keys: int = 0, counter: int = 0, table_slots: int = 0, heavy_keys: int = 0,
                  max_round_keys: int = 0, graph: bool = False, est_file_size: int = 0,
-                 learned_min_skew_pct: int = 0, round_outputs_host: bool = False, learned_max_fine: int = 0):
+                 learned_min_skew_pct: int = 0, round_outputs_host: bool = False, learned_max_fine: int = 0,
+                 learned_steps: int = 0):
         self.k = k
         self.nw = words_for_k(k)
         self._L = lib()
         cfg = Config(k, partition, min_count, max_count, expected_bytes, sub_bits, device, classifier_num, pass_keys,
                      counter, table_slots, heavy_keys, learned_min_skew_pct, max_round_keys, 1 if graph else 0,
                      1 if round_outputs_host else 0,
-                     est_file_size, learned_max_fine)
+                     est_file_size, learned_max_fine, learned_steps)
         h = ctypes.c_void_p()
         rc = self._L.kc_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

@@ -1254,7 +1254,7 @@ int kc_plan(kc_engine* h)
     if (rc) return rc;
     if (h->cfg.partition == KC_PARTITION_LEARNED) {
         const double min_skew = (h->cfg.learned_min_skew_pct ? h->cfg.learned_min_skew_pct : 250u) / 100.0;
-        rc = learned_plan(h->k, h->sample, min_skew, h->st, h->plan, h->err);
+        rc = learned_plan(h->k, h->sample, min_skew, h->cfg.learned_steps, h->st, h->plan, h->err);
         if (rc) return rc;
         h->bounds_sbits = ~0u;  // routing bounds are rebuilt for the run's fan-out
     }

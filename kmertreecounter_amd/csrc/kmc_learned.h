@@ -42,8 +42,13 @@ __device__ __forceinline__ void write_bf16_copy(uint32_t i, float v, __bf16* w1b
 // max/mean 1.13-1.15 over distinct keys (p99 1.02) from 6000 to 16000 steps
 // and for several seeds; occurrence-weighted labels (a microsatellite key
 // spans ~29 quantile buckets) reach only 3.5, and the expected bucket 2.2.
+// The partition routes by the CDF the boundaries define, interpolated inside
+// each learned bucket, so the counting time hardly depends on how balanced
+// the 4096 learned buckets are: 1000 steps (0.15 s of training) count the
+// 2.4e9 keys of the skewed workload within 1% of 8000 steps (0.9 s), which is
+// the default of kc_plan (kc_config.learned_steps).
 struct MlpTrainOpts {
-    uint32_t steps = 8000;
+    uint32_t steps = 1000;
     uint32_t batch = 2048;  // multiple of 64
     float lr = 1e-2f;       // Adam, cosine decay to zero
     uint64_t seed = 1;

@@ -44,7 +44,8 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
 // empty: the run then uses the prefix partition.
 // The network is trained only when the sample's largest 12-bit prefix holds
 // more than min_skew times their mean distinct keys.
-int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, hipStream_t st, PlanResult& out,
+int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, uint32_t train_steps, hipStream_t st,
+                 PlanResult& out,
                  std::string& err);
 
 }  // namespace kmc

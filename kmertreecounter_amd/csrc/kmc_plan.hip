@@ -13,6 +13,8 @@
 #include <cstdio>
 #include <cstdlib>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "kmc_fault.h"
 #include "kmc_kernels.h"
 #include "kmc_learned.h"
@@ -59,14 +61,39 @@ void bloom_capacity(const uint64_t* counts, uint64_t est_file_size, uint64_t* ca
 
 namespace {
 
-uint64_t splitmix64(uint64_t z)
+uint32_t key_words(uint32_t k) { return k <= 32 ? 1u : k <= 64 ? 2u : 4u; }
+
+__device__ __forceinline__ uint64_t splitmix64_dev(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
 
-uint32_t key_words(uint32_t k) { return k <= 32 ? 1u : k <= 64 ? 2u : 4u; }
+// Word 0 of the sample keys whose hash is below keep_below (every key with
+// keep_all), appended to out: one counter add per wave.  A key is kept with
+// all its occurrences, so the kept multiset does not depend on the order the
+// extraction wrote the keys in.
+__global__ void __launch_bounds__(256)
+k_keep_by_hash(const uint64_t* __restrict__ keys, uint64_t n, uint32_t nw, uint64_t keep_below, uint32_t keep_all,
+               uint64_t* __restrict__ out, unsigned long long* __restrict__ count)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        uint64_t w0 = 0;
+        bool keep = false;
+        if (i < n) {
+            w0 = keys[i * nw];
+            keep = keep_all || splitmix64_dev(w0) < keep_below;
+        }
+        const uint64_t bal = __ballot(keep);
+        unsigned long long at = 0;
+        if (lane == 0 && bal) at = atomicAdd(count, (unsigned long long)__popcll(bal));
+        at = __shfl(at, 0, 64);
+        if (keep) out[at + __popcll(bal & ((1ULL << lane) - 1ULL))] = w0;
+    }
+}
 
 uint64_t extract_tile(uint32_t nw)
 {
@@ -131,33 +158,6 @@ int stage0_plan(uint32_t k, const kc_config& cfg, const std::vector<char>& sampl
 
 namespace {
 
-// LSD radix sort of 64-bit words, 16-bit digits; a digit that every word
-// shares is skipped (one counting pass for all four digits first).
-void radix_sort_u64(std::vector<uint64_t>& v)
-{
-    const size_t n = v.size();
-    if (n < 2) return;
-    std::vector<uint64_t> tmp(n);
-    std::vector<uint32_t> cnt(4 * 65536, 0);
-    for (uint64_t x : v)
-        for (int d = 0; d < 4; ++d) cnt[(size_t)d * 65536 + ((x >> (16 * d)) & 0xFFFF)]++;
-    uint64_t* src = v.data();
-    uint64_t* dst = tmp.data();
-    for (int d = 0; d < 4; ++d) {
-        uint32_t* c = cnt.data() + (size_t)d * 65536;
-        if (c[(src[0] >> (16 * d)) & 0xFFFF] == n) continue;  // one value of this digit: order kept
-        uint32_t run = 0;
-        for (uint32_t i = 0; i < 65536; ++i) {
-            const uint32_t k = c[i];
-            c[i] = run;
-            run += k;
-        }
-        for (size_t i = 0; i < n; ++i) dst[c[(src[i] >> (16 * d)) & 0xFFFF]++] = src[i];
-        std::swap(src, dst);
-    }
-    if (src != v.data()) std::memcpy(v.data(), src, n * sizeof(uint64_t));
-}
-
 // max/mean of bucket sizes, buckets given by sorted lower bounds over sorted keys
 double imbalance(const std::vector<uint64_t>& sorted, const std::vector<uint64_t>& lower_bounds)
 {
@@ -174,8 +174,8 @@ double imbalance(const std::vector<uint64_t>& sorted, const std::vector<uint64_t
 
 }  // namespace
 
-int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, hipStream_t st, PlanResult& out,
-                 std::string& err)
+int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, uint32_t train_steps, hipStream_t st,
+                 PlanResult& out, std::string& err)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t nw = key_words(k);
@@ -186,7 +186,9 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, h
     uint8_t* d_seq = nullptr;
     unsigned long long* d_small = nullptr;
     uint64_t* d_keys = nullptr;
-    auto cleanup = [&]() { hipFree(d_seq); hipFree(d_small); hipFree(d_keys); };
+    uint64_t* d_kept = nullptr;
+    void* d_tmp = nullptr;
+    auto cleanup = [&]() { hipFree(d_seq); hipFree(d_small); hipFree(d_keys); hipFree(d_kept); hipFree(d_tmp); };
     if (dmalloc(&d_seq, padded + 256) != hipSuccess || dmalloc(&d_small, sizeof(unsigned long long) * (16 * 256 + 600)) != hipSuccess ||
         dmalloc(&d_keys, padded * 8ULL * nw) != hipSuccess) {
         cleanup();
@@ -208,30 +210,40 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, h
     unsigned long long nkeys = 0;
     if (e == hipSuccess) e = dcopy_async(&nkeys, coff + 256, sizeof(nkeys), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    std::unique_ptr<uint64_t[]> all;  // not value-initialised: the copy fills it
-    if (e == hipSuccess && nkeys) {
-        all.reset(new (std::nothrow) uint64_t[nkeys * nw]);
-        e = all ? dcopy(all.get(), d_keys, nkeys * 8ULL * nw, hipMemcpyDeviceToHost) : hipErrorOutOfMemory;
-    }
-    if (e != hipSuccess) {
-        cleanup();
-        err = std::string("learned plan: ") + hipGetErrorString(e);
-        return KC_ERR_HIP;
-    }
-    // word 0 of the keys, at most about 4M of them, sorted.  A key is kept by
-    // its hash, so the sample does not depend on the order the extraction
-    // wrote the keys in, and a kept key keeps all its occurrences.
+    // word 0 of the keys, at most about 4M of them, sorted (on the device: the
+    // sample's 24M keys at workload 3 took 120 ms to copy and filter on the
+    // host).  A key is kept by its hash, so the sample does not depend on the
+    // order the extraction wrote the keys in, and a kept key keeps all its
+    // occurrences.
     const uint64_t cap = 4ULL << 20;
     const uint64_t stride = nkeys > cap ? (nkeys + cap - 1) / cap : 1;
     // kept: hash below 2^64 / stride (about one key in `stride`)
     const uint64_t keep_below = stride == 1 ? ~0ULL : ~0ULL / stride;
-    std::vector<uint64_t> sorted;
-    sorted.reserve(nkeys / stride + nkeys / (4 * stride) + 1);
-    for (uint64_t i = 0; i < nkeys; ++i) {
-        const uint64_t w0 = all[i * nw];
-        if (stride == 1 || splitmix64(w0) < keep_below) sorted.push_back(w0);
+    unsigned long long nkept = 0;
+    size_t tmp_bytes = 0;
+    if (e == hipSuccess && nkeys) e = dmalloc(&d_kept, 2 * 8ULL * nkeys);  // kept keys, then sorted
+    if (e == hipSuccess && nkeys) e = hipMemsetAsync(d_small, 0, sizeof(unsigned long long), st);
+    if (e == hipSuccess && nkeys) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(8192, (nkeys + 255) / 256);
+        hipLaunchKernelGGL(k_keep_by_hash, dim3(grid), dim3(256), 0, st, d_keys, (uint64_t)nkeys, nw, keep_below,
+                           stride == 1 ? 1u : 0u, d_kept, d_small);
+        e = hipGetLastError();
     }
-    all.reset();
+    if (e == hipSuccess && nkeys) e = dcopy_async(&nkept, d_small, sizeof(nkept), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && nkeys) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && nkept)
+        e = rocprim::radix_sort_keys(nullptr, tmp_bytes, d_kept, d_kept + nkeys, (size_t)nkept, 0, 64, st);
+    if (e == hipSuccess && nkept) e = dmalloc(&d_tmp, tmp_bytes);
+    if (e == hipSuccess && nkept)
+        e = rocprim::radix_sort_keys(d_tmp, tmp_bytes, d_kept, d_kept + nkeys, (size_t)nkept, 0, 64, st);
+    std::vector<uint64_t> sorted(nkept);
+    if (e == hipSuccess && nkept) e = dcopy_async(sorted.data(), d_kept + nkeys, 8ULL * nkept, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        cleanup();
+        err = std::string("learned plan: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? KC_ERR_NOMEM : KC_ERR_HIP;
+    }
     auto lap = [&, last = t0](const char* what) mutable {
         if (!std::getenv("KMC_TIMING")) return;
         const auto now = std::chrono::steady_clock::now();
@@ -239,10 +251,9 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, h
                      std::chrono::duration<double, std::milli>(now - last).count());
         last = now;
     };
-    lap("sample keys");
-    radix_sort_u64(sorted);
-    lap("sort");
+    lap("sample keys, sorted");
     MlpTrainOpts o;
+    if (train_steps) o.steps = train_steps;
     // the network learns the quantiles of the keys it is shown
     std::vector<uint64_t> uniq(sorted);
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
@@ -270,12 +281,22 @@ int learned_plan(uint32_t k, const std::vector<char>& sample, double min_skew, h
     std::vector<float> params;
     int rc = mlp_train(d_keys, train.size(), o, st, params, out.loss_curve, err);
     lap("train");
+    // predictions of every s-th training key (about 1M keys): the isotonic fit
+    // places each boundary within s keys of where all the keys would put it
+    const size_t s = std::max<size_t>(1, train.size() >> 20);
+    std::vector<uint64_t> grid;
+    grid.reserve(train.size() / s + 1);
+    for (size_t i = 0; i < train.size(); i += s) grid.push_back(train[i]);
+    if (!rc && s > 1 && dcopy(d_keys, grid.data(), grid.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        err = "learned plan: copy of the prediction keys failed";
+        rc = -2;
+    }
     std::vector<float> pred;
-    if (!rc) rc = mlp_predict(params.data(), d_keys, train.size(), o.expect, st, pred, err);
+    if (!rc) rc = mlp_predict(params.data(), d_keys, grid.size(), o.expect, st, pred, err);
     lap("predict");
     cleanup();
     if (rc) return KC_ERR_HIP;
-    boundaries_from_predictions(train, pred, out.learned_b);
+    boundaries_from_predictions(grid, pred, out.learned_b);
     lap("boundaries");
     out.split_buckets = split_merged_buckets(uniq, out.learned_b);
     out.train_keys = train.size();

@@ -72,7 +72,9 @@ def test_learned_partition_balances_skewed_input_and_stays_exact():
     keys, counts = C.count_table(C.extract(seq, 31))
     res = {}
     for part in (K.KC_PARTITION_PREFIX, K.KC_PARTITION_LEARNED):
-        e = learned_engine_run(seq, 31, part, sample_frac=0.05)
+        # 8000 training steps: the sample balance below is the network's own
+        # (the default 1000 steps count as fast, see the next test)
+        e = learned_engine_run(seq, 31, part, sample_frac=0.05, learned_steps=8000)
         f, r, s = e.firsts(), e.records(), e.stats()
         mk, mc = C.merge_outputs(f, r, 1)
         np.testing.assert_array_equal(mk, keys)
@@ -103,6 +105,31 @@ def test_learned_partition_balances_skewed_input_and_stays_exact():
     assert sl.extra_passes <= sp.extra_passes
 
 
+def test_default_training_length_partitions_like_the_long_one():
+    # the fine buckets interpolate the CDF the learned boundaries define, so
+    # the default 1000 training steps (about 0.15 s on the full-size sample)
+    # leave the 4096 sample buckets less even than 8000 steps but split the
+    # counted keys about as evenly, with as few extra counting passes
+    seq = K.generate(3, scale=100, seed=31, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    res = {}
+    for steps in (0, 8000):
+        e = learned_engine_run(seq, 31, K.KC_PARTITION_LEARNED, sample_frac=0.05, learned_steps=steps)
+        mk, mc = C.merge_outputs(e.firsts(), e.records(), 1)
+        np.testing.assert_array_equal(mk, keys)
+        np.testing.assert_array_equal(mc, counts)
+        res[steps] = e.stats()
+        e.close()
+    d, l = res[0], res[8000]
+    print(f"1000 steps: loss {d.learned_loss_first:.3f} -> {d.learned_loss_last:.3f}, train {d.learned_train_ms:.0f} ms, "
+          f"largest bucket {d.max_bucket_keys}, extra passes {d.extra_passes}; 8000 steps: train "
+          f"{l.learned_train_ms:.0f} ms, largest bucket {l.max_bucket_keys}, extra passes {l.extra_passes}")
+    assert d.learned_loss_last < 0.5 * d.learned_loss_first
+    assert d.learned_train_ms < l.learned_train_ms
+    assert d.max_bucket_keys <= 1.5 * l.max_bucket_keys
+    assert d.extra_passes <= 1.5 * l.extra_passes + 16
+
+
 def test_learned_plan_is_deterministic_and_balanced_on_uniform_input():
     # workload 2 shape; the sample (24M keys) is above the 4M-key cap, so the
     # plan keeps keys by hash: two plans see the same keys and give the same
@@ -110,7 +137,7 @@ def test_learned_plan_is_deterministic_and_balanced_on_uniform_input():
     seq = K.generate(2, scale=10, seed=7, fmt="seq")
     stats = []
     for _ in range(2):
-        with K.Engine(31, partition=K.KC_PARTITION_LEARNED, learned_min_skew_pct=1) as e:
+        with K.Engine(31, partition=K.KC_PARTITION_LEARNED, learned_min_skew_pct=1, learned_steps=8000) as e:
             e.sample_block(seq[: len(seq) // 10])
             e.plan()
             stats.append(e.stats())
