@@ -100,7 +100,7 @@ COUNT_KERNEL = {"bloom": "k_count", "exact": "k_count_exact", "sort": "k_count_s
 def kernel_names(nw: int, counter: str = "bloom") -> dict:
     return {
         "extract_count": f"k_extract<{nw}, false>", "extract_scatter": f"k_extract<{nw}, true>",
-        "sub_count": f"k_sub<{nw}, false>", "sub_scatter": f"k_sub<{nw}, true>",
+        "sub_count": f"k_sub_count<{nw}>", "sub_scatter": f"k_sub<{nw}>",
         "count": f"{COUNT_KERNEL[counter]}<{nw}>",
         "heavy": f"k_heavy<{nw}>",
         "group_records": f"k_spill_emit<{nw}> + k_seg_scan + k_compact_firsts<{nw}> + k_compact_recs<{nw}> + k_root_scan",

@@ -129,7 +129,8 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     a.heavy_count = b.ctr + kCtrHeavyChunks;
     a.heavy_next = b.ctr + kCtrHeavyNext;
     a.heavy_buckets = b.ctr + kCtrHeavyBuckets;
-    a.order = d_ndist + nf;  // filled by the count launch: largest buckets first
+    a.order = d_ndist + nf;  // largest buckets first
+    PIPE_CHECK(launch_bucket_order(nf, fine_off, const_cast<uint32_t*>(a.order), st));
     PIPE_CHECK(mark(8));
     PIPE_CHECK(launch_count<NW>(nf, o.counter, src, fine_off, a, st));
     PIPE_CHECK(mark(9));

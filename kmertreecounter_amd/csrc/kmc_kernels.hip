@@ -2063,14 +2063,17 @@ k_bucket_order(const unsigned long long* __restrict__ fine_off, uint32_t nf, uin
     }
 }
 
+hipError_t launch_bucket_order(uint32_t nf, const unsigned long long* fine_off, uint32_t* order, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bucket_order, dim3(1), dim3(1024), 0, st, fine_off, nf, order);
+    KMC_LAUNCH_CHECK();
+    return hipSuccess;
+}
+
 template <int NW>
 hipError_t launch_count(uint32_t nf, uint32_t counter, const uint64_t* B, const unsigned long long* fine_off,
                         const CountArgs& args, hipStream_t st)
 {
-    if (args.order) {
-        hipLaunchKernelGGL(k_bucket_order, dim3(1), dim3(1024), 0, st, fine_off, nf, const_cast<uint32_t*>(args.order));
-        KMC_LAUNCH_CHECK();
-    }
     if (counter == kCounterSort)
         hipLaunchKernelGGL((k_count_sort<NW>), dim3(nf), dim3(kSortThreads), 0, st, B, fine_off, args);
     else if (counter == kCounterExact)
