@@ -169,7 +169,54 @@ struct Windows {
     uint32_t k;
     __device__ __forceinline__ Windows(const uint32_t* c_, const uint32_t* v_, uint32_t k_) : c(c_), v(v_), k(k_) {}
     __device__ __forceinline__ bool key(int i, Key<NW>& out) const { return window_key<NW>(c, v, i, k, out); }
+    __device__ __forceinline__ bool root(int i, uint32_t& r) const;
 };
+
+// The root bucket of a window without its key.  The canonical key is the
+// smaller of seq = pack(reverse(window)) and rev = pack(complement(window)),
+// so its top byte is the smaller of their top bytes (when those are equal the
+// key's is the same either way): seq's holds the window's last four bases,
+// newest first, rev's the complements of its first four (for k < 4 only the
+// top 2k bits, as in the key).
+__device__ __forceinline__ uint32_t rev4(uint32_t b)  // reverse the four 2-bit groups of a byte
+{
+    b = __builtin_bitreverse32(b) >> 24;
+    return ((b >> 1) & 0x55u) | ((b & 0x55u) << 1);
+}
+
+template <int NW>
+__device__ __forceinline__ bool Windows<NW>::root(int i, uint32_t& r) const
+{
+    Key<NW> key;
+    const bool ok = window_key<NW>(c, v, i, k, key);
+    r = (uint32_t)(key.w[0] >> 56);
+    return ok;
+}
+
+// 33 <= k <= 64: the window's forward code string F (2k bits, oldest base on
+// top) as in window_key<2>, then its last and first four bases.
+template <>
+__device__ __forceinline__ bool Windows<2>::root(int i, uint32_t& r) const
+{
+    const int s = 2 * (15 - i);
+    const uint64_t x0 = ((uint64_t)c[3] << 32) | c[4];
+    const uint64_t x1 = ((uint64_t)c[1] << 32) | c[2];
+    const uint64_t x2 = c[0];
+    const uint64_t flo = s ? ((x0 >> s) | (x1 << (64 - s))) : x0;
+    const uint64_t fhi = s ? ((x1 >> s) | (x2 << (64 - s))) : x1;
+    const int t = 15 - i;
+    const uint64_t vlo = ((uint64_t)v[1] << 48) | ((uint64_t)v[2] << 32) | ((uint64_t)v[3] << 16) | (uint64_t)v[4];
+    const uint64_t vhi = v[0];
+    const uint64_t W = t ? ((vlo >> t) | (vhi << (64 - t))) : vlo;
+    const uint64_t km = (k == 64) ? ~0ULL : ((1ULL << k) - 1ULL);
+    const uint32_t seq8 = rev4((uint32_t)flo & 0xFFu);
+    const uint32_t p = 2 * k - 8;  // bit of F where its first four bases start, 58..120
+    const uint32_t first = (uint32_t)(p >= 64 ? (fhi >> (p - 64)) : ((fhi << (64 - p)) | (flo >> p))) & 0xFFu;
+    const uint32_t rev8 = ~first & 0xFFu;
+    r = seq8 < rev8 ? seq8 : rev8;
+    return (W & km) == km;
+}
+
 
 // k <= 32, one key word.  The reference keeps, left aligned,
 //   seq = pack(reverse(window))    (src/GetKmer.h:71-73, shift right, new base on top)
@@ -217,6 +264,20 @@ struct Windows<1> {
         const uint64_t seq = (sr == 32 ? (((uint64_t)r2 << 32) | r1)
                                        : (((uint64_t)funnel32(r2, r1, sr) << 32) | funnel32(r1, r0, sr))) & top;
         out.w[0] = seq < rev ? seq : rev;
+        return (okm >> (15 - i)) & 1u;
+    }
+    // the key's top byte only (see Windows<NW>::root): seq's from the reversed
+    // string's top word, rev's from the forward string's first four bases
+    __device__ __forceinline__ bool root(int i, uint32_t& r) const
+    {
+        const uint32_t s = 2 * (15 - i);
+        const uint64_t fwd = ((uint64_t)funnel32(c0, c1, s) << 32) | funnel32(c1, c2, s);
+        const uint32_t mask8 = (uint32_t)(top >> 56);
+        const uint32_t rev8 = (uint32_t)(~(fwd << shl) >> 56) & mask8;
+        const uint32_t sr = 2 * i + 2;
+        const uint32_t shi = sr == 32 ? r2 : funnel32(r2, r1, sr);
+        const uint32_t seq8 = (shi >> 24) & mask8;
+        r = seq8 < rev8 ? seq8 : rev8;
         return (okm >> (15 - i)) & 1u;
     }
 };
@@ -335,10 +396,11 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     const Windows<NW> win(c, v, k);
 
     if constexpr (!SCATTER) {
+        // the count pass needs each key's root only (its top byte)
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            Key<NW> key;
-            if (win.key(i, key)) atomicAdd(&s_hist[(uint32_t)(key.w[0] >> 56)], 1u);
+            uint32_t r;
+            if (win.root(i, r)) atomicAdd(&s_hist[r], 1u);
         }
         __syncthreads();
         if (t < 256) {

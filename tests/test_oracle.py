@@ -452,3 +452,29 @@ def test_mlp_oracle_zero_weights_and_finite_differences():
             lm, _ = mlp_oracle_loss_grad(q, keys, labels, grad=False)
             fd = (lp - lm) / (2 * h)
             assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (lo, i, fd, g[i])
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 17, 31, 32, 33, 35, 36, 63, 64])
+def test_root_is_the_smaller_top_byte_of_the_two_strands(k):
+    # The extraction's count pass takes a key's root (its top byte) without
+    # the key: the canonical key is the smaller of pack(reverse(window)) and
+    # pack(complement(window)), so its top byte is the smaller of their top
+    # bytes -- the window's last four bases newest first, and the complements
+    # of its first four (only the top 2k bits for k < 4).  Checked against the
+    # oracle's keys, which the reference's GetKmer vectors pin.
+    rng = np.random.default_rng(100 + k)
+    alpha = np.frombuffer(b"ACGTacgtUN", np.uint8)
+    p = np.array([.24, .24, .24, .24, .005, .005, .005, .005, .01, .01])
+    seq = bytes(rng.choice(alpha, size=4000, p=p / p.sum()).astype(np.uint8))
+    code = {ord(c): v for c, v in zip("ACGTacgtUu", (0, 1, 2, 3, 0, 1, 2, 3, 3, 3))}
+    roots, run = [], 0
+    for i, b in enumerate(seq):
+        run = run + 1 if b in code else 0
+        if run >= k:
+            w = [code[x] for x in seq[i - k + 1: i + 1]]
+            nb = min(4, k)
+            s8 = sum(w[::-1][j] << (6 - 2 * j) for j in range(nb))
+            r8 = sum((3 - w[j]) << (6 - 2 * j) for j in range(nb))
+            roots.append(min(s8, r8))
+    keys = C.extract(seq, k)
+    np.testing.assert_array_equal(np.array(roots, dtype=np.int64), (keys[:, 0] >> np.uint64(56)).astype(np.int64))
