@@ -81,7 +81,11 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     // events bracket the main kernel of each phase: ev[2p] .. ev[2p+1]; the
     // heavy-bucket chunks run between ev[9] and ev[11]
     PIPE_CHECK(mark(0));
-    PIPE_CHECK(launch_extract<NW>(false, b.seq, b.padded, o.k, o.rt, counts, cursor, nullptr, st));
+    if (o.root_counts)
+        PIPE_CHECK(hipMemcpyAsync(counts, o.root_counts, sizeof(unsigned long long) * kXcds * 256,
+                                  hipMemcpyDeviceToDevice, st));
+    else
+        PIPE_CHECK(launch_extract<NW>(false, b.seq, b.padded, o.k, o.rt, counts, cursor, nullptr, st));
     PIPE_CHECK(mark(1));
     PIPE_CHECK(launch_scan_coarse(counts, coarse_off, cursor, tile_start, (uint32_t)sub_tile(NW), st, o.rt.rlo, o.rt.rhi));
     PIPE_CHECK(mark(2));
@@ -686,7 +690,8 @@ GraphKey graph_key(const kc_engine* h, const Route& rt, bool sub, uint32_t nf, u
 }
 
 template <int NW>
-int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded);
+int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded,
+                const unsigned long long* root_counts = nullptr);
 
 // Contiguous root-bucket ranges of at most `cap` keys each (counts: 256 root
 // bucket sizes); returns the number of ranges, 0 if one bucket exceeds cap.
@@ -740,7 +745,8 @@ int ensure_small(kc_engine* h)
     if (h->d_small && h->d_ctr) return KC_OK;
     dfree(h->d_small);
     dfree(h->d_ctr);
-    if (dmalloc(&h->d_small, sizeof(unsigned long long) * kSmTotal) != hipSuccess ||
+    // kSmTotal words cleared by every pass, then the rounds' root histogram
+    if (dmalloc(&h->d_small, sizeof(unsigned long long) * (kSmTotal + kXcds * 256)) != hipSuccess ||
         dmalloc(&h->d_ctr, sizeof(unsigned long long) * kCtrCount) != hipSuccess) {
         dfree(h->d_small);
         dfree(h->d_ctr);
@@ -819,6 +825,10 @@ int run_pipeline_t(kc_engine* h)
     HIPX(launch_extract<NW>(false, h->d_seq, padded, h->k, rt, counts, nullptr, nullptr, st));
     std::vector<unsigned long long> hc((size_t)kXcds * 256);
     HIPX(dcopy_async(hc.data(), counts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st));
+    // kept past the words a pass clears: every round's pass copies it instead
+    // of running the count pass over the whole sequence again
+    unsigned long long* saved_counts = h->d_small + kSmTotal;
+    HIPX(dcopy_async(saved_counts, counts, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToDevice, st));
     HIPX(hipStreamSynchronize(st));
     uint64_t root[256];
     for (uint32_t b = 0; b < 256; ++b) {
@@ -857,7 +867,7 @@ int run_pipeline_t(kc_engine* h)
     for (const auto& r : ranges) {
         rt.rlo = r.first;
         rt.rhi = r.second;
-        rc = run_round_t<NW>(h, rt, sub, nf, padded);
+        rc = run_round_t<NW>(h, rt, sub, nf, padded, saved_counts);
         if (rc) return rc;
         lap("count", h->rounds_used);
         if (h->rounds_used == 0 && !h->cfg.round_outputs) {
@@ -922,7 +932,8 @@ int run_pipeline_t(kc_engine* h)
 }
 
 template <int NW>
-int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded)
+int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t padded,
+                const unsigned long long* root_counts)
 {
     hipStream_t st = h->st;
     unsigned long long* coarse_off = h->d_small + kSmCoarseOff;
@@ -966,6 +977,7 @@ int run_round_t(kc_engine* h, const Route& rt, bool sub, uint32_t nf, uint64_t p
         po.heavy_keys = heavy_keys(h);
         po.heavy_chunk = heavy_chunk_keys(h);
         po.heavy_workgroups = h->cu_count;
+        po.root_counts = root_counts;
         auto enqueue = [&](bool graph) -> int {
             const hipError_t e = enqueue_pipeline<NW>(pb, po, st, graph, graph ? nullptr : h->ev);
             return e == hipSuccess ? KC_OK : h->hip_fail(e, "enqueue of the counting pipeline");

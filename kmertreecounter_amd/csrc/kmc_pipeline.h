@@ -73,6 +73,10 @@ struct PipeOpts {
     uint64_t heavy_keys;
     uint32_t heavy_chunk;
     uint32_t heavy_workgroups;       // persistent k_heavy grid (one per CU)
+    // bucket-range rounds: the per-XCD root histogram of the whole input
+    // (kXcds x 256, the count pass's own layout), computed once before the
+    // rounds; the pass copies it instead of running the count pass again
+    const unsigned long long* root_counts;
 };
 
 // Where a pass leaves its outputs: firsts, records, and the P3 key input.
