@@ -286,7 +286,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         t_t = time.perf_counter()
         r = eng.autotune()
         tuned = {"sub_bits": r.sub_bits, "counter": {K.KC_COUNTER_EXACT: "exact", K.KC_COUNTER_SORT: "sort"}.get(r.counter, "bloom"),
-                 "heavy_keys": r.heavy_keys, "candidates": r.candidates, "best_ms": r.best_ms,
+                 "heavy_keys": r.heavy_keys, "learned_max_fine": r.learned_max_fine,
+                 "candidates": r.candidates, "best_ms": r.best_ms,
                  "tune_ms": (time.perf_counter() - t_t) * 1e3}
         log(f"[{partition}] autotune: {tuned}")
     st = eng.stats()
@@ -315,7 +316,8 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         cfg_eager = dict(partition=part, expected_bytes=len(seq), counter=eng_counter(eng, cnt, tuned), graph=False,
                          learned_max_fine=LEARNED_MAX_FINE)
         if tuned:
-            cfg_eager.update(sub_bits=tuned["sub_bits"] or K.KC_SUB_BITS_NONE, heavy_keys=tuned["heavy_keys"])
+            cfg_eager.update(sub_bits=tuned["sub_bits"] or K.KC_SUB_BITS_NONE, heavy_keys=tuned["heavy_keys"],
+                             learned_max_fine=tuned["learned_max_fine"])
         eng2 = K.Engine(k, **cfg_eager)
         if bounds is not None:  # the same partition, without training again
             eng2.set_partition_bounds(bounds)

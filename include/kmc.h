@@ -265,6 +265,7 @@ typedef struct kc_tune_result {
     uint32_t heavy_keys;  /* chosen heavy-bucket threshold */
     uint32_t candidates;  /* configurations timed */
     double best_ms;       /* device time of the chosen one */
+    uint32_t learned_max_fine; /* chosen cap on a root's fine buckets (learned partition; 0 = the default) */
 } kc_tune_result;
 KC_API int kc_autotune(kc_engine* h, kc_tune_result* out);
 

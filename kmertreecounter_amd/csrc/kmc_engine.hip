@@ -1400,8 +1400,10 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
     if (h->stage_len || h->fq_unsettled) return h->fail(KC_ERR_STATE, "blocks pushed since the last kc_finish");
     if (!h->seq_len) return h->fail(KC_ERR_STATE, "no resident input: push blocks and call kc_finish first");
     // candidates: for each counter (Bloom, exact, sort) its automatic fine
-    // fan-out and one bit less, two heavy-bucket thresholds; each is timed on
-    // one pass over the resident input
+    // fan-out and one bit less, two heavy-bucket thresholds, and with a learned
+    // partition and the Bloom counter two caps on a root's fine buckets (1024:
+    // heavy roots split as finely as the rest, on the wide sub-partition path;
+    // 256); each is timed on one pass over the resident input
     const kc_config saved = h->cfg;
     const uint64_t tile = ext_tile(h->nw);
     const uint64_t padded = ((h->seq_len + tile - 1) / tile) * tile;
@@ -1419,24 +1421,29 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
         std::vector<uint32_t> sbs{sb_auto};
         if (sb_auto > 1) sbs.push_back(sb_auto - 1);
         if (saved.sub_bits && saved.sub_bits != KC_SUB_BITS_NONE && saved.sub_bits != sb_auto) sbs.push_back(saved.sub_bits);
+        const bool learned = saved.partition == KC_PARTITION_LEARNED && !h->plan.learned_b.empty();
+        std::vector<uint32_t> caps{saved.learned_max_fine};
+        if (learned && c == KC_COUNTER_BLOOM) caps.push_back(saved.learned_max_fine == 256u ? 0u : 256u);
         for (uint32_t sb : sbs)
-            for (uint32_t hv : heavies) {
-                kc_config t = saved;
-                t.sub_bits = sb ? sb : KC_SUB_BITS_NONE;
-                t.counter = c;
-                t.heavy_keys = hv;
-                h->cfg = t;
-                const int rc = run_pipeline(h);
-                if (rc) {
-                    h->cfg = saved;
-                    return rc;
+            for (uint32_t hv : heavies)
+                for (uint32_t cap : caps) {
+                    kc_config t = saved;
+                    t.sub_bits = sb ? sb : KC_SUB_BITS_NONE;
+                    t.counter = c;
+                    t.heavy_keys = hv;
+                    t.learned_max_fine = cap;
+                    h->cfg = t;
+                    const int rc = run_pipeline(h);
+                    if (rc) {
+                        h->cfg = saved;
+                        return rc;
+                    }
+                    ++tried;
+                    if (h->phase_ms[6] < best) {
+                        best = h->phase_ms[6];
+                        best_cfg = t;
+                    }
                 }
-                ++tried;
-                if (h->phase_ms[6] < best) {
-                    best = h->phase_ms[6];
-                    best_cfg = t;
-                }
-            }
     }
     h->cfg = best_cfg;
     const int rc = run_pipeline(h);  // results of the chosen configuration
@@ -1447,6 +1454,7 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
         out->heavy_keys = best_cfg.heavy_keys ? best_cfg.heavy_keys : (uint32_t)(1u << 20);
         out->candidates = tried;
         out->best_ms = best;
+        out->learned_max_fine = best_cfg.learned_max_fine;
     }
     return KC_OK;
 }

@@ -237,3 +237,22 @@ def test_partition_bounds_are_checked():
         with pytest.raises(K.KmcError) as x:
             e.set_partition_bounds(b)
         assert x.value.status == KC_ERR_STATE
+
+
+def test_autotune_of_a_learned_partition_keeps_the_tables_exact():
+    # the autotuner also times the cap on a root's fine buckets (256 or 1024)
+    # for the learned partition with the Bloom counter
+    seq = K.generate(3, scale=100, seed=17, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    e = learned_engine_run(seq, 31, K.KC_PARTITION_LEARNED, sample_frac=0.05)
+    r = e.autotune()
+    assert r.candidates >= 8
+    assert r.learned_max_fine in (0, 256)
+    mk, mc = C.merge_outputs(e.firsts(), e.records(), 1)
+    np.testing.assert_array_equal(mk, keys)
+    np.testing.assert_array_equal(mc, counts)
+    e.count_resident()  # the tuned configuration stays in the handle
+    mk, mc = C.merge_outputs(e.firsts(), e.records(), 1)
+    np.testing.assert_array_equal(mk, keys)
+    np.testing.assert_array_equal(mc, counts)
+    e.close()
