@@ -287,7 +287,7 @@ def run_partition(seq: bytes, k: int, partition: str, steps: int, warmup: int, c
         r = eng.autotune()
         tuned = {"sub_bits": r.sub_bits, "counter": {K.KC_COUNTER_EXACT: "exact", K.KC_COUNTER_SORT: "sort"}.get(r.counter, "bloom"),
                  "heavy_keys": r.heavy_keys, "learned_max_fine": r.learned_max_fine,
-                 "candidates": r.candidates, "best_ms": r.best_ms,
+                 "candidates": r.candidates, "skipped": r.skipped, "best_ms": r.best_ms,
                  "tune_ms": (time.perf_counter() - t_t) * 1e3}
         log(f"[{partition}] autotune: {tuned}")
     st = eng.stats()

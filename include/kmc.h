@@ -258,7 +258,10 @@ KC_API int kc_count_resident(kc_engine* h);
  * parameters): times one pass over the resident input for each candidate --
  * the Bloom, exact and sort counters, each with its automatic fine fan-out and
  * one bit less (tile and bucket size), two heavy-bucket thresholds -- keeps the
- * fastest in the handle's configuration and leaves its results in place. */
+ * fastest in the handle's configuration and leaves its results in place.
+ * A candidate that does not fit the device (KC_ERR_NOMEM or KC_ERR_CAPACITY,
+ * e.g. a spill table that cannot grow) is skipped and counted in `skipped`;
+ * the call fails only when no candidate fits. */
 typedef struct kc_tune_result {
     uint32_t sub_bits;    /* chosen fine fan-out bits */
     uint32_t counter;     /* chosen kc_counter */
@@ -266,6 +269,7 @@ typedef struct kc_tune_result {
     uint32_t candidates;  /* configurations timed */
     double best_ms;       /* device time of the chosen one */
     uint32_t learned_max_fine; /* chosen cap on a root's fine buckets (learned partition; 0 = the default) */
+    uint32_t skipped;     /* candidates skipped because they did not fit the device (KC_ERR_NOMEM / KC_ERR_CAPACITY) */
 } kc_tune_result;
 KC_API int kc_autotune(kc_engine* h, kc_tune_result* out);
 

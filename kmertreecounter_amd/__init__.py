@@ -62,7 +62,8 @@ class NoallocResult(ctypes.Structure):
 
 class TuneResult(ctypes.Structure):
     _fields_ = [("sub_bits", ctypes.c_uint32), ("counter", ctypes.c_uint32), ("heavy_keys", ctypes.c_uint32),
-                ("candidates", ctypes.c_uint32), ("best_ms", ctypes.c_double), ("learned_max_fine", ctypes.c_uint32)]
+                ("candidates", ctypes.c_uint32), ("best_ms", ctypes.c_double), ("learned_max_fine", ctypes.c_uint32),
+                ("skipped", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):

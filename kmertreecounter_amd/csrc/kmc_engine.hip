@@ -1411,7 +1411,8 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
     const uint32_t heavies[2] = {0u, 1u << 18};
     double best = 1e300;
     kc_config best_cfg = saved;
-    uint32_t tried = 0;
+    uint32_t tried = 0, skipped = 0;
+    int skip_rc = KC_OK;
     for (uint32_t c : counters) {
         kc_config probe = saved;
         probe.sub_bits = 0;
@@ -1434,6 +1435,14 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
                     t.learned_max_fine = cap;
                     h->cfg = t;
                     const int rc = run_pipeline(h);
+                    // a candidate that does not fit the device (its spill
+                    // table or buffers cannot grow) is skipped; any other
+                    // error ends the tuning
+                    if (rc == KC_ERR_NOMEM || rc == KC_ERR_CAPACITY) {
+                        ++skipped;
+                        skip_rc = rc;
+                        continue;
+                    }
                     if (rc) {
                         h->cfg = saved;
                         return rc;
@@ -1445,10 +1454,15 @@ int kc_autotune(kc_engine* h, kc_tune_result* out)
                     }
                 }
     }
+    if (!tried) {
+        h->cfg = saved;
+        return h->fail(skip_rc, "autotune: no candidate fits the device");
+    }
     h->cfg = best_cfg;
     const int rc = run_pipeline(h);  // results of the chosen configuration
     if (rc) return rc;
     if (out) {
+        out->skipped = skipped;
         out->sub_bits = best_cfg.sub_bits == KC_SUB_BITS_NONE ? 0u : best_cfg.sub_bits;
         out->counter = best_cfg.counter;
         out->heavy_keys = best_cfg.heavy_keys ? best_cfg.heavy_keys : (uint32_t)(1u << 20);

@@ -35,39 +35,50 @@ constexpr uint32_t kMaxSubBits = 10;
 // P3: one workgroup per fine bucket (1 per CU).  LDS: Bloom words + count
 // table; buckets above kPassKeys keys are processed in hash-range passes.
 template <int NW> struct CountCfg;
-// Bloom counter: 128 KiB of LDS per workgroup (Bloom 32 KiB, table 96 KiB),
-// one 1024-thread workgroup per CU.  (Half the table with two workgroups per
-// CU measured slower: the extra passes cost more than the occupancy gains.)
+// Bloom counter: one 1024-thread workgroup per fine bucket, two workgroups per
+// CU (8 waves per SIMD, so at most 64 VGPRs and 80 KiB of LDS each): a 16 KiB
+// filter and a count table of 5120 (k <= 32) or 3072 (k <= 64) slots.  The
+// kernel is bound by the latency of its LDS and memory round trips: with one
+// workgroup per CU (a 32 KiB filter and twice the table) it ran 25-30% slower
+// in spite of needing fewer hash-range passes.  A pass aims at about 70% table
+// load; kPassKeys bounds the occurrences of a bucket's first pass.
 template <> struct CountCfg<1> {
     static constexpr int kThreads = 1024;
-    static constexpr uint32_t kBloomWords = 8192;  // 32-bit filter words (32 KiB)
-    static constexpr uint32_t kTableSlots = 8192;
-    static constexpr uint32_t kPassKeys = 98304;
-    static constexpr uint32_t kPassDistinct = 5000;  // target distinct keys per pass (61% table load)
+    static constexpr int kWaves = 8;  // waves per SIMD
+    static constexpr uint32_t kBloomWords = 4096;  // 32-bit filter words (16 KiB)
+    static constexpr uint32_t kTableSlots = 5120;
+    static constexpr uint32_t kPassKeys = 73728;
+    static constexpr uint32_t kPassDistinct = 3600;  // target distinct keys per pass
     static constexpr uint32_t kProbe = 64;
-    static constexpr int kLoads = 4;  // key loads in flight per thread
+    static constexpr int kLoads = 2;  // key loads in flight per thread
     static constexpr uint32_t kExactSlots = 10240;  // exact counter: 120 KiB of keys + counts
+    static constexpr int kExactLoads = 4;  // exact counter and k_heavy (one workgroup per CU)
 };
 template <> struct CountCfg<2> {
     static constexpr int kThreads = 1024;
-    static constexpr uint32_t kBloomWords = 8192;  // 32-bit filter words (32 KiB)
-    static constexpr uint32_t kTableSlots = 4096;
-    static constexpr uint32_t kPassKeys = 49152;
-    static constexpr uint32_t kPassDistinct = 2500;
+    static constexpr int kWaves = 8;
+    static constexpr uint32_t kBloomWords = 4096;  // 32-bit filter words (16 KiB)
+    static constexpr uint32_t kTableSlots = 3072;
+    static constexpr uint32_t kPassKeys = 30720;
+    static constexpr uint32_t kPassDistinct = 2150;
     static constexpr uint32_t kProbe = 64;
-    static constexpr int kLoads = 4;
+    static constexpr int kLoads = 1;
     static constexpr uint32_t kExactSlots = 6144;
+    static constexpr int kExactLoads = 4;
 };
-// k 65..128: a table slot is five 63-bit words plus its count (44 bytes)
+// k 65..128: a table slot is five 63-bit words plus its count (44 bytes); one
+// workgroup per CU
 template <> struct CountCfg<4> {
     static constexpr int kThreads = 1024;
+    static constexpr int kWaves = 4;
     static constexpr uint32_t kBloomWords = 8192;  // 32-bit filter words (32 KiB)
     static constexpr uint32_t kTableSlots = 2048;
     static constexpr uint32_t kPassKeys = 24576;
     static constexpr uint32_t kPassDistinct = 1250;
     static constexpr uint32_t kProbe = 64;
-    static constexpr int kLoads = 4;
+    static constexpr int kLoads = 3;
     static constexpr uint32_t kExactSlots = 2560;
+    static constexpr int kExactLoads = 4;
 };
 
 // Sort counter: one 1024-thread workgroup per fine bucket; a pass stages kKeys

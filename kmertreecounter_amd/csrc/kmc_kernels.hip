@@ -66,6 +66,16 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* scratch, T* total)
     return r;
 }
 
+// Waits until every vector memory operation this wave has issued is complete
+// (s_waitcnt vmcnt(0); the LDS/scalar and export counters are left alone).
+// One counter covers both loads and stores on CDNA, and a wait for loads has
+// to wait for every store issued after them whenever the number of those
+// stores varies (stores under a lane condition).  A kernel that prefetches
+// its next keys and then issues such stores waits here, before the stores,
+// for the prefetch only; the first use of the prefetched keys then needs no
+// wait at all.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Workgroup copy of n units, four loads in flight per thread before their
 // stores (one load then its store per step waits out the load latency).
 template <typename U>
@@ -981,6 +991,37 @@ __device__ __forceinline__ bool hand_over_heavy(const CountArgs& args, uint32_t 
     return true;
 }
 
+// Open addressing over the first ts (any size) of TS LDS slots (word w of
+// slot s at words[w * TS + s]), same claim protocol as table_find_or_claim;
+// returns the slot or -1 after `limit` probes.
+template <int NW, uint32_t TS>
+__device__ __forceinline__ int32_t lds_find_or_claim(unsigned long long* words, uint32_t ts, uint32_t start,
+                                                     uint32_t limit, const Key<NW>& key, bool& claimed)
+{
+    claimed = false;
+    uint32_t s = start;
+    for (uint32_t p = 0; p < limit; ++p) {
+        bool match = true;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (!match) break;
+            unsigned long long* slotw = &words[(uint32_t)w * TS + s];
+            unsigned long long cur = *slotw;
+            if (cur == kEmpty) {
+                cur = atomicCAS(slotw, (unsigned long long)kEmpty, (unsigned long long)key.w[w]);
+                if (cur == kEmpty) {
+                    cur = key.w[w];
+                    if (w == NW - 1) claimed = true;
+                }
+            }
+            match = (cur == key.w[w]);
+        }
+        if (match) return (int32_t)s;
+        if (++s == ts) s = 0;
+    }
+    return -1;
+}
+
 // One workgroup per fine bucket f, keys B[fine_off[f], fine_off[f+1]).
 // A bucket larger than pass_keys is processed in hash-range passes (range of
 // the top 16 hash bits), each with a fresh LDS Bloom filter and count table, so
@@ -991,7 +1032,7 @@ __device__ __forceinline__ bool hand_over_heavy(const CountArgs& args, uint32_t 
 // written to the bucket's own slice of `firsts`; root records to the bucket's
 // own slice of `rec_tmp` (n_f*8*NW bytes); no global atomics on the hot path.
 template <int NW>
-__global__ void __launch_bounds__(CountCfg<NW>::kThreads)
+__global__ void __launch_bounds__(CountCfg<NW>::kThreads) __attribute__((amdgpu_waves_per_eu(CountCfg<NW>::kWaves)))
 k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ fine_off, CountArgs args)
 {
     constexpr int T = CountCfg<NW>::kThreads;
@@ -1054,6 +1095,9 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
             const unsigned long long i = lo + (unsigned long long)u * T + t;
             if (i < hi) nb[u] = load_key<NW>(B, i);
         }
+        // every step's keys are complete when the step starts (the first
+        // step's here, the others before the previous step's stores)
+        wait_vmem();
         for (unsigned long long it = 0; it < iters; ++it) {
             Key<NW> kb[kLoads];
 #pragma unroll
@@ -1066,8 +1110,9 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
             uint64_t fbal[kLoads];  // first occurrences of each key slot, one reservation per step
             // per key slot: run length (a run of r equal keys is one occurrence
             // followed by r - 1 repeats: the head lane tests the filter once and
-            // counts the rest), hash (bits 0..27 filter word and bits, 32..44
-            // table slot, 48..63 pass range) and filter bits (0: not tested)
+            // counts the rest), hash (bits 0..27 filter word and bits, 32..47
+            // home slot scaled to the table size, 48..63 pass range) and
+            // filter bits (0: not tested)
             uint32_t run[kLoads], fmask[kLoads], fold[kLoads];
             uint64_t hk[kLoads];
             // k <= 64: the step's filter atomics issued back to back; k > 64
@@ -1097,9 +1142,9 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
                     const uint32_t add = run[u] - (first ? 1u : 0u);
                     if (add) {
                         bool claimed;
-                        const int64_t s = table_find_or_claim<TW>(s_tkey, (uint64_t)TS, (uint64_t)(TS - 1),
-                                                                 (hk[u] >> 32), CountCfg<NW>::kProbe,
-                                                                 to_table<NW>(kb[u]), claimed);
+                        const int32_t s = lds_find_or_claim<TW, TS>(
+                            s_tkey, TS, (uint32_t)((((hk[u] >> 32) & 0xFFFFu) * TS) >> 16), CountCfg<NW>::kProbe,
+                            to_table<NW>(kb[u]), claimed);
                         if (s >= 0) atomicAdd(&s_tcnt[s], add);
                         else global_add<NW>(args, kb[u], add);
                     }
@@ -1112,7 +1157,10 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
             for (int u = 0; u < kLoads; ++u) nf += (uint32_t)__popcll(fbal[u]);
             uint32_t wbase = 0;
             if (lane == 0 && nf) wbase = atomicAdd(&s_nfirst, nf);
-            wbase = __shfl(wbase, 0, kWave);
+            wbase = __builtin_amdgcn_readlane(wbase, 0);
+            // the next step's keys (loaded a step ago) are waited for before
+            // this step's stores are issued, not after them
+            wait_vmem();
 #pragma unroll
             for (int u = 0; u < kLoads; ++u) {
                 if ((fbal[u] >> lane) & 1ULL) store_key<NW>(args.firsts, lo + wbase + lanes_below(fbal[u]), kb[u]);
@@ -1187,37 +1235,6 @@ __device__ __forceinline__ uint32_t count_hash(const Key<NW>& k)
     return x;
 }
 
-// Open addressing over the first ts (any size) of TS LDS slots (word w of
-// slot s at words[w * TS + s]), same claim protocol as table_find_or_claim;
-// returns the slot or -1 after `limit` probes.
-template <int NW, uint32_t TS>
-__device__ __forceinline__ int32_t lds_find_or_claim(unsigned long long* words, uint32_t ts, uint32_t start,
-                                                     uint32_t limit, const Key<NW>& key, bool& claimed)
-{
-    claimed = false;
-    uint32_t s = start;
-    for (uint32_t p = 0; p < limit; ++p) {
-        bool match = true;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            if (!match) break;
-            unsigned long long* slotw = &words[(uint32_t)w * TS + s];
-            unsigned long long cur = *slotw;
-            if (cur == kEmpty) {
-                cur = atomicCAS(slotw, (unsigned long long)kEmpty, (unsigned long long)key.w[w]);
-                if (cur == kEmpty) {
-                    cur = key.w[w];
-                    if (w == NW - 1) claimed = true;
-                }
-            }
-            match = (cur == key.w[w]);
-        }
-        if (match) return (int32_t)s;
-        if (++s == ts) s = 0;
-    }
-    return -1;
-}
-
 // Exact counter: every occurrence is counted in the bucket's LDS table; a key
 // seen c times leaves one first occurrence (the low.bin key) and, for c >= 2,
 // the root record (key, c - 1) under the min/max filter -- the split the
@@ -1238,7 +1255,7 @@ k_count_exact(const uint64_t* __restrict__ B, const unsigned long long* __restri
 {
     constexpr int T = CountCfg<NW>::kThreads;
     constexpr uint32_t TS = CountCfg<NW>::kExactSlots;
-    constexpr int kLoads = CountCfg<NW>::kLoads;
+    constexpr int kLoads = CountCfg<NW>::kExactLoads;
     constexpr int TW = TableWords<NW>::value;
     __shared__ unsigned long long s_tkey[TW * TS];
     __shared__ uint32_t s_tcnt[TS];
@@ -1678,7 +1695,7 @@ k_heavy(const uint64_t* __restrict__ B, CountArgs args)
 {
     constexpr int T = CountCfg<NW>::kThreads;
     constexpr uint32_t TS = CountCfg<NW>::kExactSlots;
-    constexpr int kLoads = CountCfg<NW>::kLoads;
+    constexpr int kLoads = CountCfg<NW>::kExactLoads;
     constexpr int TW = TableWords<NW>::value;
     __shared__ unsigned long long s_tkey[TW * TS];
     __shared__ uint32_t s_tcnt[TS];

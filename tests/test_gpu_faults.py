@@ -124,3 +124,35 @@ def test_every_failure_point(kind, mode):
 def test_fault_hook_rejects_bad_kind():
     assert K.debug_fault(0, 1) == -1
     assert K.debug_fault(3, 1) == -1
+
+
+def test_autotune_skips_a_candidate_that_does_not_fit():
+    # a candidate whose allocation fails (as a spill table that cannot grow on a
+    # full device would) is skipped; the others are timed and the kept
+    # configuration counts exactly
+    seq = K.generate(2, scale=400, seed=19, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    with K.Engine(31) as e:
+        e.push_block(seq)
+        e.finish()
+        K.debug_fault(K.FAULT_ALLOC, 0)
+        full = e.autotune()
+        calls = K.debug_fault(K.FAULT_ALLOC, 0)
+    assert full.skipped == 0 and full.candidates > 1
+    assert calls > 0
+    skipped_seen = False
+    for nth in range(1, calls + 1):
+        with K.Engine(31) as e:
+            e.push_block(seq)
+            e.finish()
+            K.debug_fault(K.FAULT_ALLOC, nth)
+            try:
+                r = e.autotune()
+            finally:
+                K.debug_fault(K.FAULT_ALLOC, 0)
+            assert r.candidates + r.skipped == full.candidates, (nth, r.candidates, r.skipped)
+            assert_exact(C.merge_outputs(e.firsts(), e.records(), 1), e.histogram(1, 200), keys, counts)
+        if r.skipped:
+            skipped_seen = True
+            break
+    assert skipped_seen
