@@ -14,13 +14,18 @@ enum : uint32_t { kFaultAlloc = 1, kFaultCopy = 2 };
 // Counts a call of `kind`; true when the armed hook fails this one.
 bool fault_hit(uint32_t kind);
 
+// A failed allocation is returned to the caller, which handles it; the
+// runtime's last-error state is cleared so that the next launch check
+// (hipGetLastError after a kernel launch) does not report it again.
 inline hipError_t dmalloc(void** p, size_t bytes)
 {
     if (fault_hit(kFaultAlloc)) {
         *p = nullptr;
         return hipErrorOutOfMemory;
     }
-    return hipMalloc(p, bytes);
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
 }
 template <class T>
 inline hipError_t dmalloc(T** p, size_t bytes)
@@ -34,7 +39,9 @@ inline hipError_t dhost_malloc(void** p, size_t bytes, unsigned flags)
         *p = nullptr;
         return hipErrorOutOfMemory;
     }
-    return hipHostMalloc(p, bytes, flags);
+    const hipError_t e = hipHostMalloc(p, bytes, flags);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
 }
 template <class T>
 inline hipError_t dhost_malloc(T** p, size_t bytes, unsigned flags)

@@ -352,7 +352,7 @@ __device__ __forceinline__ bool window_key<4>(const uint32_t* c, const uint32_t*
 // Root bucket of a key: its top byte, for both partitions (the learned
 // partition splits roots into fine buckets in P2).
 template <int NW, bool SCATTER>
-__global__ void __launch_bounds__(ExtractCfg<NW>::kThreads)
+__global__ void __launch_bounds__(ExtractCfg<NW>::kThreads) __attribute__((amdgpu_waves_per_eu(ExtractCfg<NW>::kWaves)))
 k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Route rt,
           unsigned long long* __restrict__ counts, unsigned long long* __restrict__ cursor,
           uint64_t* __restrict__ out, uint64_t tile0)
@@ -363,7 +363,9 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     // The encoded tile is dead once every thread has its keys (a barrier
     // later), so it shares LDS with the staged keys.
     constexpr size_t kStage = 2 * (T + H) * sizeof(uint32_t);
-    constexpr size_t kKeys = SCATTER ? TILE * sizeof(Key<NW>) : 0;
+    constexpr int kRounds = ExtractCfg<NW>::kStageRounds;
+    constexpr uint32_t kHalf = TILE / kRounds;  // slots staged per round
+    constexpr size_t kKeys = SCATTER ? kHalf * sizeof(Key<NW>) : 0;
     __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
     uint32_t* s_code = reinterpret_cast<uint32_t*>(s_raw);
     uint32_t* s_valid = s_code + (T + H);
@@ -448,16 +450,28 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
         const uint32_t off = block_exclusive_scan<uint32_t>(h, s_scan, &total);
         if (t < 256) s_off[t] = off;
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (okm & (1u << i)) s_keys[s_off[(uint32_t)(keys[i].w[0] >> 56)] + rank[i]] = keys[i];
         // bucket b's keys go to gb + (slot - off): one base per bucket (the
         // difference wraps modulo 2^64 and the sum with slot is exact)
         if (h) s_dst[t] = gb - (unsigned long long)off;
-        __syncthreads();
-        for (uint32_t slot = t; slot < total; slot += T) {
-            const Key<NW> key = s_keys[slot];
-            store_key<NW>(out, s_dst[(uint32_t)(key.w[0] >> 56)] + slot, key);
+        // the sorted tile is staged and stored kHalf slots at a time
+#pragma unroll
+        for (int round = 0; round < kRounds; ++round) {
+            const uint32_t r0 = (uint32_t)round * kHalf;
+            if (round > 0) {
+                if (total <= r0) break;  // uniform: total is the workgroup's
+                __syncthreads();         // the previous round's stores have read their keys
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t slot = s_off[(uint32_t)(keys[i].w[0] >> 56)] + rank[i];
+                if ((okm & (1u << i)) && slot - r0 < kHalf) s_keys[slot - r0] = keys[i];
+            }
+            __syncthreads();
+            const uint32_t end = min(total, r0 + kHalf);
+            for (uint32_t slot = r0 + t; slot < end; slot += T) {
+                const Key<NW> key = s_keys[slot - r0];
+                store_key<NW>(out, s_dst[(uint32_t)(key.w[0] >> 56)] + slot, key);
+            }
         }
     }
 }

@@ -156,3 +156,18 @@ def test_autotune_skips_a_candidate_that_does_not_fit():
             skipped_seen = True
             break
     assert skipped_seen
+
+
+def test_a_refused_device_allocation_leaves_no_error_behind():
+    # the runtime refuses a 64 TiB sequence buffer (nothing is allocated); the
+    # engine reports KC_ERR_NOMEM, and the runtime's last-error state must not
+    # surface as a launch failure in the next job on the same device
+    with pytest.raises(K.KmcError) as x:
+        K.Engine(31, expected_bytes=64 << 40)
+    assert x.value.status == KC_ERR_NOMEM
+    seq = K.generate(2, scale=400, seed=19, fmt="seq")
+    keys, counts = C.count_table(C.extract(seq, 31))
+    with K.Engine(31) as e:
+        e.push_block(seq)
+        e.finish()
+        assert_exact(C.merge_outputs(e.firsts(), e.records(), 1), e.histogram(1, 200), keys, counts)
