@@ -18,17 +18,18 @@ constexpr unsigned long long kErrFirstOut = 16ULL; // first occurrences exceed t
 
 // P0/P1: one 16-byte word of sequence per thread, kThreads*16 positions per tile.
 // kStageRounds: the scatter stages its sorted tile in LDS in this many rounds
-// (two halve its LDS, so a third workgroup fits on a CU where registers allow
-// it: k > 32, whose key registers keep it at two waves per SIMD otherwise)
+// (two halve its LDS, 36 KB instead of 70, so a third workgroup fits on a CU);
+// kWaves: waves per SIMD its register allocation must allow (k <= 32: 6, at
+// 80 VGPRs; k <= 64: 3, at 148; 8 waves would spill)
 template <int NW> struct ExtractCfg;
 template <> struct ExtractCfg<1> {
     static constexpr int kThreads = 512;
-    static constexpr int kStageRounds = 1;
-    static constexpr int kWaves = 1;  // waves per SIMD the register allocation must allow (1: no limit)
+    static constexpr int kStageRounds = 2;
+    static constexpr int kWaves = 6;
 };
 template <> struct ExtractCfg<2> { static constexpr int kThreads = 256; static constexpr int kStageRounds = 2; static constexpr int kWaves = 3; };
 // 256 threads at least: the bucket histogram and its scan take one thread per bucket
-template <> struct ExtractCfg<4> { static constexpr int kThreads = 256; static constexpr int kStageRounds = 2; static constexpr int kWaves = 1; };
+template <> struct ExtractCfg<4> { static constexpr int kThreads = 256; static constexpr int kStageRounds = 2; static constexpr int kWaves = 1; };  // (1: no limit)
 
 // P2: kTile keys of one root bucket per tile; the scatter (k_sub) takes one
 // tile per workgroup of kThreads, the count pass (k_sub_count) several per
