@@ -366,6 +366,11 @@ k_extract(const uint8_t* __restrict__ seq, uint64_t padded_len, uint32_t k, Rout
     constexpr int kRounds = ExtractCfg<NW>::kStageRounds;
     constexpr uint32_t kHalf = TILE / kRounds;  // slots staged per round
     constexpr size_t kKeys = SCATTER ? kHalf * sizeof(Key<NW>) : 0;
+    // the scatter's staging must let ExtractCfg::kWaves waves per SIMD share a
+    // CU's 160 KiB of LDS (workgroups of T threads, T / 256 waves per SIMD each)
+    static_assert(!SCATTER || ExtractCfg<NW>::kWaves == 1 ||
+                      (size_t)(ExtractCfg<NW>::kWaves * 256 / T) * (kKeys + 4096 + 64) <= 160 * 1024,
+                  "k_extract's staging does not allow the configured waves per SIMD");
     __shared__ __align__(16) uint8_t s_raw[kStage > kKeys ? kStage : kKeys];
     uint32_t* s_code = reinterpret_cast<uint32_t*>(s_raw);
     uint32_t* s_valid = s_code + (T + H);
@@ -1064,6 +1069,11 @@ k_count(const uint64_t* __restrict__ B, const unsigned long long* __restrict__ f
     __shared__ uint32_t s_hstart;
     __shared__ uint32_t s_hwidth;
     __shared__ uint32_t s_passes;
+    // the waves per SIMD the configuration asks for must fit the CU's 160 KiB
+    // of LDS: two 1024-thread workgroups per CU for 8 waves
+    static_assert(CountCfg<NW>::kWaves * 64 * 4 / T == 1 ||
+                      (CountCfg<NW>::kWaves * 64 * 4 / T) * (BW * 4 + TW * TS * 8 + TS * 4 + 64) <= 160 * 1024,
+                  "k_count's LDS does not allow the configured waves per SIMD");
 
     const int t = threadIdx.x;
     const uint32_t lane = t & (kWave - 1);
