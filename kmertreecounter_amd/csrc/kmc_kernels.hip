@@ -859,14 +859,17 @@ __device__ __forceinline__ uint32_t run_head_length(const Key<NW>& key, bool val
 {
     constexpr int kRowShr1 = 0x111;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    bool same = valid && (lane & 15u) != 0;
+    // the key's words XORed with the previous lane's (zero where equal); a
+    // row's first lane always heads a run
+    uint32_t diff = (lane & 15u) ? 0u : 1u;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
         const int lo = (int)(uint32_t)key.w[w], hi = (int)(uint32_t)(key.w[w] >> 32);
         const int plo = __builtin_amdgcn_update_dpp(0, lo, kRowShr1, 0xF, 0xF, true);
         const int phi = __builtin_amdgcn_update_dpp(0, hi, kRowShr1, 0xF, 0xF, true);
-        same = same && plo == lo && phi == hi;
+        diff |= (uint32_t)(plo ^ lo) | (uint32_t)(phi ^ hi);
     }
+    const bool same = valid && diff == 0u;
     const uint64_t heads = __ballot(!same);  // invalid lanes head empty runs, so no run crosses them
     if (heads == ~0ULL) return valid ? 1u : 0u;  // no repeated neighbours in the wave (the usual case)
     if (!valid || same) return 0;
