@@ -134,7 +134,8 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     a.heavy_next = b.ctr + kCtrHeavyNext;
     a.heavy_buckets = b.ctr + kCtrHeavyBuckets;
     a.order = d_ndist + nf;  // largest buckets first
-    PIPE_CHECK(launch_bucket_order(nf, fine_off, const_cast<uint32_t*>(a.order), st));
+    PIPE_CHECK(launch_bucket_order(nf, fine_off, const_cast<uint32_t*>(a.order),
+                                   reinterpret_cast<uint32_t*>(b.small + kSmOrder), st));
     PIPE_CHECK(mark(8));
     PIPE_CHECK(launch_count<NW>(nf, o.counter, src, fine_off, a, st));
     PIPE_CHECK(mark(9));

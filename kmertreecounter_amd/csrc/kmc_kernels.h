@@ -175,7 +175,10 @@ hipError_t launch_sub(bool scatter, uint64_t max_tiles, const uint64_t* A, const
 // counter: the kc_counter values
 enum : uint32_t { kCounterBloom = 0, kCounterExact = 1, kCounterSort = 2 };
 // order[nf]: the counting kernels' bucket order (args.order), largest first
-hipError_t launch_bucket_order(uint32_t nf, const unsigned long long* fine_off, uint32_t* order, hipStream_t st);
+// order[0..nf): the fine buckets by size class, largest first; scratch: 512
+// zeroed 32-bit words
+hipError_t launch_bucket_order(uint32_t nf, const unsigned long long* fine_off, uint32_t* order, uint32_t* scratch,
+                               hipStream_t st);
 template <int NW>
 hipError_t launch_count(uint32_t nf, uint32_t counter, const uint64_t* B, const unsigned long long* fine_off,
                         const CountArgs& args, hipStream_t st);
