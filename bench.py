@@ -103,7 +103,7 @@ def kernel_names(nw: int, counter: str = "bloom") -> dict:
         "sub_count": f"k_sub_count<{nw}>", "sub_scatter": f"k_sub<{nw}>",
         "count": f"{COUNT_KERNEL[counter]}<{nw}>",
         "heavy": f"k_heavy<{nw}>",
-        "group_records": f"k_spill_emit<{nw}> + k_seg_scan + k_compact_firsts<{nw}> + k_compact_recs<{nw}> + k_root_scan",
+        "group_records": f"k_spill_emit<{nw}> + k_seg_part + k_seg_apply + k_compact_firsts<{nw}> + k_compact_recs<{nw}> + k_root_scan",
     }
 
 

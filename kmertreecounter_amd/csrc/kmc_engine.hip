@@ -158,6 +158,7 @@ hipError_t enqueue_pipeline(const PipeBufs& b, const PipeOpts& o, hipStream_t st
     sa.ctr = b.ctr;
     sa.rec_out_cap = b.rec_out_cap;
     sa.firsts_out_cap = b.firsts_out_cap;
+    sa.part = b.small + kSmSegPart;
     // final firsts into the P3 key input (free now), final records into the slices buffer
     PIPE_CHECK(launch_finalize<NW>(sa, a, slices, pipe_firsts<NW>(b, sub), b.R, pipe_records<NW>(b, sub), st));
     return hipSuccess;

@@ -156,6 +156,7 @@ struct SegArgs {
     unsigned long long* ctr;
     uint64_t rec_out_cap;               // records the final record buffer holds
     uint64_t firsts_out_cap;            // keys the final firsts buffer holds
+    unsigned long long* part;           // [2 x ceil(nf / 1024)] firsts and records per chunk of 1024 buckets
 };
 
 template <int NW>

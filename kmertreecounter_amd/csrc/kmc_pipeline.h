@@ -13,11 +13,11 @@ namespace kmc {
 // small arrays (unsigned long long units): per-XCD coarse counts and cursors,
 // coarse offsets, P2 tile starts, root histograms and ranges of the records,
 // scan scratch, the counting order's size-class counts and cursors (2 x 256
-// 32-bit words)
+// 32-bit words), the output scan's chunk sums (2 x 256 words: 2^18 buckets)
 constexpr uint64_t kSmCounts = 0, kSmCursor = 8 * 256, kSmCoarseOff = 16 * 256, kSmTileStart = kSmCoarseOff + 257,
                    kSmRootOvf = kSmTileStart + 257, kSmRootMain = kSmRootOvf + 256, kSmRootOff = kSmRootMain + 256,
                    kSmOvfOff = kSmRootOff + 257, kSmOvfCursor = kSmOvfOff + 257, kSmScanPart = kSmOvfCursor + 256,
-                   kSmOrder = kSmScanPart + kScanGroups, kSmTotal = kSmOrder + 256;
+                   kSmOrder = kSmScanPart + kScanGroups, kSmSegPart = kSmOrder + 256, kSmTotal = kSmSegPart + 512;
 
 // positions per extraction tile, keys per sub-partition tile, bytes per
 // ExportRecord<N>, words per key in the open-addressing tables (k > 64 keys
