@@ -115,6 +115,21 @@ def test_fanout_does_not_change_table(sub_bits):
     check_outputs(f, r, s, 31, keys, counts)
 
 
+@pytest.mark.parametrize("sub_bits", [K.KC_SUB_BITS_NONE, 8, 10])
+def test_bucket_statistics(sub_bits):
+    # the largest fine bucket and the mean, against the keys' prefix buckets
+    # (with the prefix partition a fine bucket is the key's top 8 + sub_bits
+    # bits), over up to 2^18 buckets: the output scan reduces them per chunk
+    seq = K.generate(1, scale=20, seed=5, fmt="seq")
+    words = C.extract(seq, 31)[:, 0]
+    sb = 0 if sub_bits == K.KC_SUB_BITS_NONE else sub_bits
+    sizes = np.bincount((words >> np.uint64(56 - sb)).astype(np.int64), minlength=256 << sb)
+    f, r, s = run_engine([seq], 31, sub_bits=sub_bits)
+    assert s.fine_buckets == sizes.size
+    assert s.max_bucket_keys == int(sizes.max())
+    assert s.mean_bucket_keys == pytest.approx(words.size / sizes.size)
+
+
 @pytest.mark.parametrize("k", [31, 63, 100])
 @pytest.mark.parametrize("sub_bits", [9, 10])
 def test_wide_fanout_with_learned_routing(k, sub_bits):
