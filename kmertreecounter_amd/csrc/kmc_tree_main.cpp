@@ -6,8 +6,8 @@
 // ./tmp/root_{0..255}.bin (./tmp/ must exist, as for the reference) for
 // histogram_tool.  The counting runs on the GPU.  KMC_PARTITION=learned picks
 // the learned partitioner (default: the 256 root-prefix partition);
-// KMC_COUNTER=exact the exact counter (default: the reference's Bloom
-// first-occurrence split); KMC_HISTOGRAM=<file> also writes histogram_tool's
+// KMC_COUNTER=exact the exact counter, KMC_COUNTER=sort the radix-sort
+// counter (default: the reference's Bloom first-occurrence split); KMC_HISTOGRAM=<file> also writes histogram_tool's
 // precise histogram computed on the device (KMC_HISTOGRAM_ONLY=1: no tmp/ files).
 #include <cstdio>
 #include <cstdlib>
@@ -63,7 +63,9 @@ int main(int argc, char** argv)
     const char* hist_only = std::getenv("KMC_HISTOGRAM_ONLY");
     o.histogram_only = (o.histogram_path && hist_only && std::strcmp(hist_only, "1") == 0) ? 1u : 0u;
     const char* counter = std::getenv("KMC_COUNTER");
-    o.counter = (counter && std::strcmp(counter, "exact") == 0) ? KC_COUNTER_EXACT : KC_COUNTER_BLOOM;
+    o.counter = KC_COUNTER_BLOOM;
+    if (counter && std::strcmp(counter, "exact") == 0) o.counter = KC_COUNTER_EXACT;
+    if (counter && std::strcmp(counter, "sort") == 0) o.counter = KC_COUNTER_SORT;
     const char* parse = std::getenv("KMC_PARSE");
     o.host_parse = (parse && std::strcmp(parse, "host") == 0) ? 1u : 0u;
     o.verbose = 1;

@@ -154,6 +154,28 @@ def test_tree_run_parses_on_the_device_like_the_host_reader(tmp_path, gz):
         np.testing.assert_array_equal(got_c, counts)
 
 
+@pytest.mark.parametrize("counter", [K.KC_COUNTER_EXACT, K.KC_COUNTER_SORT], ids=["exact", "sort"])
+def test_tree_run_files_with_the_other_counters(tmp_path, counter):
+    # kmc_tree_run with the exact or the sort counter (KMC_COUNTER=exact|sort
+    # on the kmc_tree command line): low.bin and the root files merge to the
+    # oracle's table, as with the Bloom counter
+    fq = K.generate(4, scale=2000, seed=23)
+    path = tmp_path / "in.fq"
+    path.write_bytes(fq)
+    out = tmp_path / "tmp"
+    out.mkdir()
+    s, _ = K.tree_run(str(path), str(out), 31, counter=counter)
+    recs = np.concatenate([np.fromfile(out / f"root_{i}.bin", dtype=K.record_dtype(1)) for i in range(256)])
+    low = (out / "low.bin").read_bytes()
+    firsts = np.zeros(len(low) // 8, dtype=np.uint64)
+    C.oracle().orc_unpack_low(low, len(low) // 8, 31, 1, firsts.ctypes.data)
+    got_k, got_c = C.merge_outputs(firsts, recs, 1)
+    keys, counts = C.count_table(C.extract_blocks(K.fastq_blocks(fq, 31), 31))
+    np.testing.assert_array_equal(got_k, keys)
+    np.testing.assert_array_equal(got_c, counts)
+    assert s.total_kmers == int(counts.sum())
+
+
 @pytest.mark.parametrize("counter", [K.KC_COUNTER_BLOOM, K.KC_COUNTER_EXACT, K.KC_COUNTER_SORT],
                          ids=["bloom", "exact", "sort"])
 def test_device_parsing_with_rounds_graph_and_filters(counter):
